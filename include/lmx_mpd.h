@@ -1,0 +1,163 @@
+/*
+ * lmx_mpd: MI355X-native MPD-RDME timestep behind a C ABI.
+ *
+ * This is the drop-in boundary for lm::rdme::MpdRdmeSolver.  The reference
+ * exposes a C++ virtual-class protocol (src/me/MESolver.h:70-77,
+ * src/cme/CMESolver.h:200-201, src/rdme/RDMESolver.h:59-60,
+ * src/rdme/MpdRdmeSolver.h:66-156); every entry point below replaces one step
+ * of that protocol and cites it.  Plain pointers and sizes only; no C++ types,
+ * no exceptions across the ABI.  All per-replicate state lives in the opaque
+ * handle (the reference keeps the model in __constant__ symbols,
+ * src/cuda/constant.cu:45-89, so two solvers on one device collide; these do
+ * not).
+ *
+ * Status codes: every function returns LMX_OK (0) or an LMX_ERR_* code;
+ * lmx_last_error() gives the calling thread's last message.
+ *
+ * Layouts (the reference's, src/rdme/ByteLattice.cpp:371-435):
+ *   row-major particles : uint8 [X][Y][Z][P]      (file/IO layout)
+ *   row-major sites     : uint8 [X][Y][Z]
+ *   native particles    : uint32 [P/4][Z][Y][X]    (word w holds slots 4w..4w+3,
+ *                                                   little-endian bytes)
+ *   native sites        : uint8 [Z][Y][X]
+ */
+#ifndef LMX_MPD_H
+#define LMX_MPD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LMX_ABI_VERSION 1
+
+enum lmx_status {
+    LMX_OK = 0,
+    LMX_ERR_ARG = 1,                    /* InvalidArgException / Exception on bad input */
+    LMX_ERR_HIP = 2,                    /* HIP runtime failure (CUDAException analogue) */
+    LMX_ERR_OVERFLOW_CAP = 3,           /* "Too many particle overflows" (MpdRdmeSolver.cu:1123) */
+    LMX_ERR_OVERFLOW_UNPLACEABLE = 4,   /* "Unable to find an available site" (:1166) */
+    LMX_ERR_OVERFLOW_OUT_OF_RANGE = 5,  /* getNearbySites reached outside the lattice (Lattice.cpp:106-128) */
+    LMX_ERR_REACTION_UNDERFLOW = 6,     /* reaction applied without its reactants (byte_reaction_dev.cu:226) */
+    LMX_ERR_STATE = 7,                  /* call out of protocol order */
+    LMX_ERR_NOMEM = 8                   /* device or host allocation failed */
+};
+
+/* Boundary conditions.  Every reference build is periodic
+ * (src/config.h.in:61 always defines MPD_BOUNDARY_PERIODIC); reflecting is the
+ * reference's dormant branch (lattice_sim_1d_dev.cu:84-90). */
+enum lmx_boundary { LMX_BOUNDARY_PERIODIC = 0, LMX_BOUNDARY_REFLECTING = 1 };
+
+/* Step implementations.  Every one gives bit-identical lattices. */
+enum lmx_path {
+    LMX_PATH_SWEEPS = 0,   /* x, y, z sweep kernels + reaction kernel + resolver */
+    LMX_PATH_FUSED = 1     /* one fused diffuse+react kernel per step + resolver */
+};
+
+typedef struct lmx_solver lmx_solver;
+
+/* Message for the last failing call on this thread ("" if none). */
+const char *lmx_last_error(void);
+int lmx_abi_version(void);
+
+/* Number of HIP devices visible (0 on a host without a GPU). */
+int lmx_device_count(int *count);
+
+/* MpdRdmeSolver::MpdRdmeSolver + the stream/overflow-list part of
+ * MpdRdmeSolver::initialize (MpdRdmeSolver.cu:95-162).  device selects the
+ * GPU (ReplicateRunner.cpp:92 setCurrentDevice). */
+int lmx_create(int device, lmx_solver **out);
+/* MpdRdmeSolver::~MpdRdmeSolver (MpdRdmeSolver.cu:164-190). */
+int lmx_destroy(lmx_solver *s);
+
+/* MpdRdmeSolver::initialize (MpdRdmeSolver.cu:107-162): parameters "seed",
+ * "timestep" (tau, seconds) and "rdme.mpd.overflowhandler" (0 = classic; the
+ * relaxed handler is not offered, see its launch at :1087).  seedTop == 0
+ * draws the seed from the clock as the reference does (:113-122).
+ * seed32 = (seedTop<<16) | (replicate & 0xFFFF) (:123). */
+int lmx_initialize(lmx_solver *s, uint32_t replicate, uint32_t seedTop, double timestep,
+                   int overflow_mode, int boundary);
+/* The seed actually used (after a clock seed). */
+int lmx_get_seed(lmx_solver *s, uint32_t *seed32);
+
+/* CMESolver::buildModel + MpdRdmeSolver::buildModel (CMESolver.cpp:307-460,
+ * MpdRdmeSolver.cu:232-364).  types[R] in {0,1,2,3}; K[R*kCols]; S and D
+ * species-major [nS][R] as in the ReactionModel message. */
+int lmx_build_model(lmx_solver *s, uint32_t numberSpecies, uint32_t numberReactions,
+                    const uint32_t *initialSpeciesCounts, const uint32_t *reactionTypes,
+                    const double *K, const int32_t *S, const uint32_t *D, uint32_t kCols);
+
+/* RDMESolver::buildDiffusionModel + MpdRdmeSolver::buildDiffusionModel
+ * (RDMESolver.cpp:110-141, MpdRdmeSolver.cu:366-483): DF [nST][nST][nS]
+ * (m^2/s), RL [R][nST], lattice and sites in row-major layout (rowMajor must
+ * be 1, as the reference requires).  Any X, Y, Z >= 1 is accepted (the
+ * reference's X%32, Y%8, Z%8 limits of :201-217 are launch-shape limits, not
+ * semantics); particlesPerSite in {4, 8, 16}; bytesPerParticle == 1. */
+int lmx_build_diffusion_model(lmx_solver *s, uint32_t numberSiteTypes, const double *DF,
+                              const uint32_t *RL, uint32_t X, uint32_t Y, uint32_t Z,
+                              uint32_t particlesPerSite, uint32_t bytesPerParticle,
+                              double latticeSpacing, const uint8_t *latticeData,
+                              const uint8_t *latticeSitesData, int rowMajorData);
+
+/* MpdRdmeSolver::setReactionRate (MpdRdmeSolver.cu:670-695): tables are
+ * recomputed (computePropensities, :697-774) and uploaded before the next step. */
+int lmx_set_reaction_rate(lmx_solver *s, uint32_t rxid, float rate);
+
+/* Select the step implementation (enum lmx_path); results do not depend on it. */
+int lmx_set_path(lmx_solver *s, int path);
+
+/* runTimestep (MpdRdmeSolver.cu:996-1193) nsteps times, enqueued on the
+ * solver's stream without a host synchronization per step (the reference
+ * synchronizes every step, :1091).  Device-side errors are sticky and are
+ * reported by the next lmx_synchronize. */
+int lmx_run_steps(lmx_solver *s, uint32_t nsteps);
+int lmx_synchronize(lmx_solver *s);
+/* Current timestep counter (number of steps taken since the lattice was set). */
+int lmx_get_timestep(lmx_solver *s, uint32_t *timestep);
+int lmx_set_timestep(lmx_solver *s, uint32_t timestep);
+
+/* CudaByteLattice::copyFromGPU + ByteLattice::copyNativeToRowMajorByte
+ * (CudaByteLattice.cu:128-141, ByteLattice.cpp:371-403). */
+int lmx_get_lattice(lmx_solver *s, uint8_t *rowMajor, size_t size);
+int lmx_get_lattice_native(lmx_solver *s, uint32_t *native, size_t words);
+int lmx_get_sites(lmx_solver *s, uint8_t *rowMajor, size_t size);
+/* Hook write-back (hookCheckSimulation return 1/2, MpdRdmeSolver.cu:958-985). */
+int lmx_set_lattice(lmx_solver *s, const uint8_t *rowMajor, size_t size);
+int lmx_set_lattice_native(lmx_solver *s, const uint32_t *native, size_t words);
+int lmx_set_sites(lmx_solver *s, const uint8_t *rowMajor, size_t size);
+
+/* ByteLattice::getParticleCounts (ByteLattice.cpp:495-520), computed on the
+ * device; counts[nS] for species 1..nS. */
+int lmx_get_species_counts(lmx_solver *s, uint32_t *counts, uint32_t n);
+
+/* Overflow statistics (overflowTimesteps/overflowListUses, :1104-1187):
+ * total entries resolved and number of steps that had any. */
+int lmx_get_overflow_stats(lmx_solver *s, uint64_t *entries, uint64_t *steps);
+
+/* Measurement (no reference counterpart; replaces the lptf PROF_CUDA_* events,
+ * MpdRdmeSolver.cu:1014-1083).  Runs nsteps steps bracketed by HIP events on
+ * the solver's stream.  If kernel_ms != NULL, every launch is also bracketed
+ * and the summed milliseconds per kernel class (enum lmx_kernel) are written
+ * to kernel_ms[0..nkernels-1] and the launch counts to kernel_launches. */
+enum lmx_kernel {
+    LMX_K_SWEEP_X = 0, LMX_K_SWEEP_Y = 1, LMX_K_SWEEP_Z = 2, LMX_K_REACTION = 3,
+    LMX_K_RESOLVE = 4, LMX_K_FUSED = 5, LMX_K_COUNT = 6
+};
+int lmx_time_steps(lmx_solver *s, uint32_t nsteps, double *total_ms, double *kernel_ms,
+                   uint64_t *kernel_launches, uint32_t nkernels);
+
+/* Debug tables (for parity tests of the host-side table construction):
+ * which: 0 T, 1 qp0, 2 qp1, 3 qp2, 4 reaction rates.  n is the capacity. */
+int lmx_get_table_f32(lmx_solver *s, int which, float *out, size_t n, size_t *written);
+
+/* Evaluate the device's reaction-probability function (float)(1-exp(-(double)a))
+ * and the device exp itself on n inputs (numerics tests). */
+int lmx_device_exp(int device, const double *x, double *y, size_t n);
+int lmx_device_reaction_probability(int device, const float *a, float *prob, size_t n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,182 @@
+// Device-side building blocks of the MPD-RDME step: the counter-based hash
+// RNG, the project double exp, slot packing and the per-launch parameter
+// block.  Reference paths are relative to the Lattice Microbes 2.5.0 tree.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace lmx {
+
+constexpr uint32_t kMaxOverflows = 2048;        // TUNE_MPD_MAX_PARTICLE_OVERFLOWS (options.cmake:88)
+constexpr uint32_t kMaxReplacementDist = 4;     // TUNE_MPD_MAX_OVERFLOW_REPLACEMENT_DIST (:87)
+
+// Reaction order codes of the MPD kernels (byte_reaction_dev.cu:56-59).
+constexpr uint32_t kZeroOrder = 1, kFirstOrder = 2, kSecondOrder = 3, kSecondOrderSelf = 4;
+
+// Sticky device error codes (mirrored by lmx_status).
+constexpr uint32_t kErrOverflowCap = 3, kErrUnplaceable = 4, kErrOutOfRange = 5,
+                   kErrReactionUnderflow = 6;
+
+// Per-solver control block in device memory.  The step counter lives here so
+// that the step sequence does not depend on host-side arguments.
+struct Control {
+    uint32_t timestep;          // advanced by the resolver at the end of each step
+    uint32_t ovf_count;         // entries appended this step (may exceed the cap)
+    uint32_t error;             // first error code, 0 if none
+    uint32_t error_step;        // timestep of the first error
+    unsigned long long ovf_total;
+    unsigned long long ovf_steps;
+    unsigned long long ovf_keys[kMaxOverflows];
+};
+
+// Overflow entry key: canonical order is phase, then site index, then
+// emission order within the site; the particle rides in the low byte.
+__host__ __device__ inline unsigned long long overflow_key(uint32_t phase, uint32_t site,
+                                                           uint32_t emission, uint32_t particle)
+{
+    return ((unsigned long long)phase << 62) | ((unsigned long long)site << 24) |
+           ((unsigned long long)(emission & 0xFFFFu) << 8) | (unsigned long long)(particle & 0xFFu);
+}
+
+// Everything a kernel needs about one replicate.
+struct Params {
+    uint32_t X, Y, Z, XY, N;
+    uint32_t P, W, pcb;         // slots, 32-bit words per site, RNG key shift
+    uint32_t nS, nST, R;
+    uint32_t boundary;          // 0 periodic, 1 reflecting
+    uint32_t seed32;
+    const float *T;             // [nST][nST][nS]
+    const float *qp0, *qp1, *qp2;
+    const float *rate;          // [R]
+    const uint32_t *order, *D1, *D2;
+    const int8_t *S;            // [R][nS]
+    const uint8_t *RL;          // [R][nST]
+    const uint8_t *sites;       // [N]
+    Control *ctl;
+};
+
+// ---------------------------------------------------------------------------
+// RNG: the 64-bit multiply-xorshift hash of xor_random_dev.cu:46-53 keyed by
+// (site, slot) under the per-substep seed of MpdRdmeSolver.cu:987-994.
+// ---------------------------------------------------------------------------
+__host__ __device__ inline uint32_t hash32(unsigned long long v, unsigned long long seed)
+{
+    v ^= seed;
+    v = v * 3935559000370003845ULL + 2691343689449507681ULL;
+    v ^= v >> 21;
+    v ^= v << 37;
+    v ^= v >> 4;
+    v *= 2685821657736338717ULL;
+    return (uint32_t)(v >> 32);
+}
+
+__host__ __device__ inline unsigned long long timestep_hash(uint32_t seed32, uint32_t t, uint32_t substep)
+{
+    unsigned long long h = (((((unsigned long long)seed32) << 30) + t) << 2) + substep;
+    h = h * 3202034522624059733ULL + 4354685564936845319ULL;
+    h ^= h >> 20;
+    h ^= h << 41;
+    h ^= h >> 5;
+    h *= 7664345821815920749ULL;
+    return h;
+}
+
+// (float)hash * 2^-32: conversion rounds to nearest, the scaling is exact.
+__device__ inline float uniform01(uint32_t v1, uint32_t shift, uint32_t v2, unsigned long long seed)
+{
+    unsigned long long v = ((unsigned long long)v1 << shift) | (unsigned long long)v2;
+    return (float)hash32(v, seed) * 0x1p-32f;
+}
+
+// ---------------------------------------------------------------------------
+// Project double exp, identical bit for bit wherever it is compiled with
+// -ffp-contract=off: Cody-Waite reduction x = k ln2 + r (ln2 head with 21
+// trailing zero bits, so k*LN2_HI is exact for |k| < 2^11), degree-13 Taylor
+// polynomial of exp(r)-1-r by Horner with fused multiply-adds, 1 + r_hi
+// summed exactly (Fast2Sum), scaled by 2^k.
+// ---------------------------------------------------------------------------
+__device__ inline double exp_proj(double x)
+{
+    if (x != x) return x + x;
+    if (x > 709.782712893384) return __builtin_huge_val();
+    if (x < -745.1332191019412) return 0.0;
+    const double kd = __builtin_rint(x * 0x1.71547652b82fep+0);
+    const double rh = __builtin_fma(-kd, 0x1.62e42fee00000p-1, x);
+    const double rl = -kd * 0x1.a39ef35793c76p-33;
+    const double r = rh + rl;
+    double q = 0x1.6124613a86d09p-33;
+    q = __builtin_fma(q, r, 0x1.1eed8eff8d898p-29);
+    q = __builtin_fma(q, r, 0x1.ae64567f544e4p-26);
+    q = __builtin_fma(q, r, 0x1.27e4fb7789f5cp-22);
+    q = __builtin_fma(q, r, 0x1.71de3a556c734p-19);
+    q = __builtin_fma(q, r, 0x1.a01a01a01a01ap-16);
+    q = __builtin_fma(q, r, 0x1.a01a01a01a01ap-13);
+    q = __builtin_fma(q, r, 0x1.6c16c16c16c17p-10);
+    q = __builtin_fma(q, r, 0x1.1111111111111p-7);
+    q = __builtin_fma(q, r, 0x1.5555555555555p-5);
+    q = __builtin_fma(q, r, 0x1.5555555555555p-3);
+    q = __builtin_fma(q, r, 0x1.0000000000000p-1);
+    const double r2 = r * r;
+    const double p = q * r2;
+    const double s = 1.0 + rh;
+    const double e = (1.0 - s) + rh;
+    const double tail = e + (rl + p);
+    const double y = s + tail;
+    return __builtin_ldexp(y, (int)kd);
+}
+
+// calculateReactionProbability, byte_reaction_dev.cu:130-137.
+__device__ inline float reaction_probability(float a)
+{
+    return (float)(1.0 - exp_proj(-(double)a));
+}
+
+// lookupTransitionProbability, byte_diffusion_1d_dev.cu:70-79.
+__device__ inline float lookup_T(const Params &p, uint32_t particle, uint32_t src, uint32_t dst)
+{
+    if ((uint32_t)(particle - 1) >= p.nS || dst >= p.nST || src >= p.nST) return 0.0f;
+    return p.T[(src * p.nST + dst) * p.nS + (particle - 1)];
+}
+
+// ---------------------------------------------------------------------------
+// Slot packing.  A site's particles are W 32-bit words (slot 4w+b = byte b of
+// word w).  SlotBuf appends particles in order and keeps the first P.
+// ---------------------------------------------------------------------------
+template <int W>
+struct SlotBuf {
+    uint32_t w[W];
+    uint32_t n;
+    __device__ inline void clear()
+    {
+#pragma unroll
+        for (int k = 0; k < W; k++) w[k] = 0;
+        n = 0;
+    }
+    // Returns true if the particle fits; otherwise the caller records an overflow.
+    __device__ inline bool push(uint32_t particle)
+    {
+        const uint32_t k = n;
+        n = k + 1;
+        if (k >= 4u * W) return false;
+#pragma unroll
+        for (int q = 0; q < W; q++)
+            if ((k >> 2) == (uint32_t)q) w[q] |= particle << (8 * (k & 3));
+        return true;
+    }
+};
+
+__device__ inline uint32_t slot_of(const uint32_t *w, int j)
+{
+    return (w[j >> 2] >> (8 * (j & 3))) & 0xFFu;
+}
+
+// Appends an overflow entry (phase, site, emission, particle) to the step's list.
+__device__ inline void record_overflow(Control *ctl, uint32_t phase, uint32_t site, uint32_t emission,
+                                       uint32_t particle)
+{
+    const uint32_t k = atomicAdd(&ctl->ovf_count, 1u);
+    if (k < kMaxOverflows) ctl->ovf_keys[k] = overflow_key(phase, site, emission, particle);
+}
+
+}  // namespace lmx

@@ -1,0 +1,19 @@
+// Host-side launchers of the MPD-RDME kernels (implemented in mpd_kernels.hip).
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace lmx {
+
+struct Params;
+
+hipError_t launch_sweep(const Params &p, const uint32_t *src, uint32_t *dst, int axis, hipStream_t st);
+hipError_t launch_reaction(const Params &p, uint32_t *lat, hipStream_t st);
+hipError_t launch_resolve(const Params &p, uint32_t *lat, hipStream_t st);
+hipError_t launch_counts(const uint32_t *lat, size_t nwords, uint32_t nS, uint32_t *counts, hipStream_t st);
+hipError_t launch_exp(const double *x, double *y, size_t n, hipStream_t st);
+hipError_t launch_prob(const float *a, float *y, size_t n, hipStream_t st);
+
+}  // namespace lmx

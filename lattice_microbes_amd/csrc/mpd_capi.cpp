@@ -1,0 +1,812 @@
+// C-ABI implementation of the MPD-RDME solver (declared in include/lmx_mpd.h).
+//
+// Host-side responsibilities of lm::rdme::MpdRdmeSolver restated for this
+// build: model ingestion and table construction (MpdRdmeSolver.cu:232-483,
+// :697-774), lattice layout (RDMESolver.cpp:144-191, ByteLattice.cpp:371-435),
+// the step sequence (runTimestep, :996-1193) and lattice readback.  The step
+// itself runs entirely on the device; the host never touches the lattice
+// between steps (overflow resolution is a device kernel).
+#include "lmx_mpd.h"
+
+#include <time.h>
+
+#include <cmath>
+#include <cstddef>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "lmx_device.h"
+#include "lmx_kernels.h"
+
+namespace {
+
+thread_local std::string g_error;
+
+int fail(int code, const std::string &msg)
+{
+    g_error = msg;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char *what)
+{
+    return fail(LMX_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define LMX_HIP(call)                                  \
+    do {                                               \
+        hipError_t e_ = (call);                        \
+        if (e_ != hipSuccess) return hip_fail(e_, #call); \
+    } while (0)
+
+template <typename T>
+int dev_upload(T **dptr, const std::vector<T> &host)
+{
+    if (*dptr) {
+        hipFree(*dptr);
+        *dptr = nullptr;
+    }
+    size_t bytes = host.size() * sizeof(T);
+    if (bytes == 0) bytes = sizeof(T);
+    LMX_HIP(hipMalloc((void **)dptr, bytes));
+    if (!host.empty()) LMX_HIP(hipMemcpy(*dptr, host.data(), host.size() * sizeof(T), hipMemcpyHostToDevice));
+    return LMX_OK;
+}
+
+template <typename T>
+void dev_free(T *&p)
+{
+    if (p) hipFree(p);
+    p = nullptr;
+}
+
+}  // namespace
+
+struct lmx_solver {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool initialized = false, model_built = false, diffusion_built = false;
+    uint32_t replicate = 0, seed32 = 0;
+    double tau = 0.0;
+    int overflow_mode = 0, boundary = 0, path = LMX_PATH_SWEEPS;
+
+    // Reaction model (CMESolver members).
+    uint32_t nS = 0, R = 0;
+    std::vector<uint32_t> initialCounts, types, si, s2i, order, D1, D2;
+    std::vector<double> k;
+    std::vector<int32_t> S_sm;
+    std::vector<int8_t> S_rm;
+
+    // Diffusion model and lattice.
+    uint32_t nST = 0, X = 0, Y = 0, Z = 0, P = 0, W = 0;
+    size_t N = 0;
+    double lambda = 0.0;
+    std::vector<double> DF;
+    std::vector<uint32_t> RL;
+    std::vector<float> T, rate, qp0, qp1, qp2;
+
+    // Device state.
+    uint32_t *d_lat[2] = {nullptr, nullptr};
+    int cur = 0;
+    uint8_t *d_sites = nullptr;
+    float *d_T = nullptr, *d_qp0 = nullptr, *d_qp1 = nullptr, *d_qp2 = nullptr, *d_rate = nullptr;
+    uint32_t *d_order = nullptr, *d_D1 = nullptr, *d_D2 = nullptr;
+    int8_t *d_S = nullptr;
+    uint8_t *d_RL = nullptr;
+    lmx::Control *d_ctl = nullptr;
+    uint32_t *d_counts = nullptr;
+    bool tables_dirty = false;
+    uint32_t timestep = 0;
+};
+
+namespace {
+
+int select_device(lmx_solver *s)
+{
+    LMX_HIP(hipSetDevice(s->device));
+    return LMX_OK;
+}
+
+// MpdRdmeSolver::computePropensities (MpdRdmeSolver.cu:697-774).  The float
+// tables are updated through double arithmetic exactly as the compound
+// assignments there evaluate.
+void compute_propensities(lmx_solver *s)
+{
+    const uint32_t nsites32 = s->X * s->Y * s->Z;  // unsigned product, as the reference
+    s->rate.assign(s->R, 0.0f);
+    for (uint32_t i = 0; i < s->R; i++) {
+        const double k = s->k[i];
+        switch (s->types[i]) {
+        case 0: s->rate[i] = (float)(k * s->tau / (double)nsites32); break;
+        case 1: s->rate[i] = (float)(k * s->tau); break;
+        default: s->rate[i] = (float)(k * s->tau * (double)s->X * (double)s->Y * (double)s->Z); break;
+        }
+    }
+    const uint32_t nS = s->nS, nST = s->nST;
+    s->qp0.assign(nST, 0.0f);
+    s->qp1.assign((size_t)nST * nS, 0.0f);
+    s->qp2.assign((size_t)nST * nS * nS, 0.0f);
+    const float scale = (float)nsites32;
+    for (uint32_t st = 0; st < nST; st++) {
+        const size_t o1 = (size_t)st * nS, o2 = (size_t)st * nS * nS;
+        for (uint32_t i = 0; i < s->R; i++) {
+            if (!s->RL[(size_t)i * nST + st]) continue;
+            const double k = s->k[i];
+            switch (s->types[i]) {
+            case 0: s->qp0[st] = (float)((double)s->qp0[st] + k * s->tau / (double)scale); break;
+            case 1: s->qp1[o1 + s->si[i]] = (float)((double)s->qp1[o1 + s->si[i]] + k * s->tau); break;
+            case 2: {
+                const float v = (float)(k * s->tau * (double)scale);
+                s->qp2[o2 + (size_t)s->si[i] * nS + s->s2i[i]] = v;
+                s->qp2[o2 + (size_t)s->s2i[i] * nS + s->si[i]] = v;
+            } break;
+            case 3: {
+                float &e = s->qp2[o2 + (size_t)s->si[i] * nS + s->si[i]];
+                e = (float)((double)e + k * s->tau * (double)scale * 2);
+            } break;
+            }
+        }
+    }
+    s->tables_dirty = true;
+}
+
+int upload_tables(lmx_solver *s)
+{
+    int rc;
+    if ((rc = dev_upload(&s->d_rate, s->rate))) return rc;
+    if ((rc = dev_upload(&s->d_qp0, s->qp0))) return rc;
+    if ((rc = dev_upload(&s->d_qp1, s->qp1))) return rc;
+    if ((rc = dev_upload(&s->d_qp2, s->qp2))) return rc;
+    s->tables_dirty = false;
+    return LMX_OK;
+}
+
+lmx::Params make_params(const lmx_solver *s)
+{
+    lmx::Params p;
+    p.X = s->X; p.Y = s->Y; p.Z = s->Z;
+    p.XY = s->X * s->Y;
+    p.N = (uint32_t)s->N;
+    p.P = s->P; p.W = s->W;
+    p.pcb = s->P <= 4 ? 2u : (s->P <= 8 ? 3u : 4u);  // MPD_PARTICLE_COUNT_BITS
+    p.nS = s->nS; p.nST = s->nST; p.R = s->R;
+    p.boundary = (uint32_t)s->boundary;
+    p.seed32 = s->seed32;
+    p.T = s->d_T;
+    p.qp0 = s->d_qp0; p.qp1 = s->d_qp1; p.qp2 = s->d_qp2;
+    p.rate = s->d_rate;
+    p.order = s->d_order; p.D1 = s->d_D1; p.D2 = s->d_D2;
+    p.S = s->d_S;
+    p.RL = s->d_RL;
+    p.sites = s->d_sites;
+    p.ctl = s->d_ctl;
+    return p;
+}
+
+int require_ready(lmx_solver *s)
+{
+    if (!s) return fail(LMX_ERR_ARG, "null solver");
+    if (!s->diffusion_built) return fail(LMX_ERR_STATE, "the diffusion model and lattice have not been set");
+    return select_device(s);
+}
+
+// Row-major (x,y,z,p) -> native planes, inserting each nonzero byte at the
+// first empty slot of its site (RDMESolver::setLatticeData + addParticle).
+int row_major_to_native(const lmx_solver *s, const uint8_t *rm, std::vector<uint32_t> &native)
+{
+    const size_t N = s->N;
+    native.assign(N * s->W, 0u);
+    std::vector<uint8_t> fill(N, 0);
+    size_t idx = 0;
+    for (uint32_t x = 0; x < s->X; x++)
+        for (uint32_t y = 0; y < s->Y; y++)
+            for (uint32_t z = 0; z < s->Z; z++) {
+                const size_t i = (size_t)x + (size_t)s->X * y + (size_t)s->X * s->Y * z;
+                for (uint32_t l = 0; l < s->P; l++, idx++) {
+                    const uint8_t v = rm[idx];
+                    if (v == 0) continue;
+                    if (v > s->nS) return fail(LMX_ERR_ARG, "latticeData: an invalid species was found: " + std::to_string(v));
+                    const uint32_t j = fill[i]++;
+                    native[(size_t)(j >> 2) * N + i] |= (uint32_t)v << (8 * (j & 3));
+                }
+            }
+    return LMX_OK;
+}
+
+void native_to_row_major(const lmx_solver *s, const uint32_t *native, uint8_t *rm)
+{
+    const size_t N = s->N;
+    for (uint32_t x = 0; x < s->X; x++)
+        for (uint32_t y = 0; y < s->Y; y++)
+            for (uint32_t z = 0; z < s->Z; z++) {
+                const size_t i = (size_t)x + (size_t)s->X * y + (size_t)s->X * s->Y * z;
+                uint8_t *d = rm + (((size_t)x * s->Y + y) * s->Z + z) * s->P;
+                for (uint32_t j = 0; j < s->P; j++) d[j] = (uint8_t)(native[(size_t)(j >> 2) * N + i] >> (8 * (j & 3)));
+            }
+}
+
+int sites_row_major_to_native(const lmx_solver *s, const uint8_t *rm, std::vector<uint8_t> &native)
+{
+    native.assign(s->N, 0);
+    size_t idx = 0;
+    for (uint32_t x = 0; x < s->X; x++)
+        for (uint32_t y = 0; y < s->Y; y++)
+            for (uint32_t z = 0; z < s->Z; z++, idx++) {
+                const uint8_t v = rm[idx];
+                if (v == 0) continue;
+                if (v >= s->nST) return fail(LMX_ERR_ARG, "latticeSitesData: an invalid site type was found: " + std::to_string(v));
+                native[(size_t)x + (size_t)s->X * y + (size_t)s->X * s->Y * z] = v;
+            }
+    return LMX_OK;
+}
+
+int check_device_error(lmx_solver *s)
+{
+    uint32_t hdr[4];
+    LMX_HIP(hipMemcpy(hdr, s->d_ctl, sizeof(hdr), hipMemcpyDeviceToHost));
+    switch (hdr[2]) {
+    case 0: return LMX_OK;
+    case lmx::kErrOverflowCap:
+        return fail(LMX_ERR_OVERFLOW_CAP, "Too many particle overflows for the available buffer at timestep " + std::to_string(hdr[3]));
+    case lmx::kErrUnplaceable:
+        return fail(LMX_ERR_OVERFLOW_UNPLACEABLE, "Unable to find an available site to handle a particle overflow at timestep " + std::to_string(hdr[3]));
+    case lmx::kErrOutOfRange:
+        return fail(LMX_ERR_OVERFLOW_OUT_OF_RANGE, "Overflow search left the lattice at timestep " + std::to_string(hdr[3]));
+    case lmx::kErrReactionUnderflow:
+        return fail(LMX_ERR_REACTION_UNDERFLOW, "Reaction selected without its reactants at timestep " + std::to_string(hdr[3]));
+    }
+    return fail(LMX_ERR_HIP, "unknown device error " + std::to_string(hdr[2]));
+}
+
+// One timestep (runTimestep, MpdRdmeSolver.cu:996-1193).  ev, if given,
+// receives events around each launch (ev[0] before the first launch, then one
+// after every launch); kinds receives the kernel class of each launch.
+int enqueue_step(lmx_solver *s, const lmx::Params &p, hipEvent_t *ev, int *kinds, int *nlaunch)
+{
+    hipError_t e;
+    int n = 0;
+    auto mark = [&](int kind) -> int {
+        if (ev) {
+            hipError_t r = hipEventRecord(ev[n + 1], s->stream);
+            if (r != hipSuccess) return hip_fail(r, "hipEventRecord");
+            kinds[n] = kind;
+        }
+        n++;
+        return LMX_OK;
+    };
+    if (ev) {
+        e = hipEventRecord(ev[0], s->stream);
+        if (e != hipSuccess) return hip_fail(e, "hipEventRecord");
+    }
+    for (int axis = 0; axis < 3; axis++) {
+        e = lmx::launch_sweep(p, s->d_lat[s->cur], s->d_lat[1 - s->cur], axis, s->stream);
+        if (e != hipSuccess) return hip_fail(e, "sweep launch");
+        s->cur = 1 - s->cur;
+        if (int rc = mark(LMX_K_SWEEP_X + axis)) return rc;
+    }
+    if (s->R > 0) {
+        e = lmx::launch_reaction(p, s->d_lat[s->cur], s->stream);
+        if (e != hipSuccess) return hip_fail(e, "reaction launch");
+        if (int rc = mark(LMX_K_REACTION)) return rc;
+    }
+    e = lmx::launch_resolve(p, s->d_lat[s->cur], s->stream);
+    if (e != hipSuccess) return hip_fail(e, "resolve launch");
+    if (int rc = mark(LMX_K_RESOLVE)) return rc;
+    if (nlaunch) *nlaunch = n;
+    s->timestep++;
+    return LMX_OK;
+}
+
+void free_device(lmx_solver *s)
+{
+    dev_free(s->d_lat[0]);
+    dev_free(s->d_lat[1]);
+    dev_free(s->d_sites);
+    dev_free(s->d_T);
+    dev_free(s->d_qp0);
+    dev_free(s->d_qp1);
+    dev_free(s->d_qp2);
+    dev_free(s->d_rate);
+    dev_free(s->d_order);
+    dev_free(s->d_D1);
+    dev_free(s->d_D2);
+    dev_free(s->d_S);
+    dev_free(s->d_RL);
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *lmx_last_error(void) { return g_error.c_str(); }
+
+int lmx_abi_version(void) { return LMX_ABI_VERSION; }
+
+int lmx_device_count(int *count)
+{
+    if (!count) return fail(LMX_ERR_ARG, "null count");
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    *count = (e == hipSuccess) ? n : 0;
+    return LMX_OK;
+}
+
+int lmx_create(int device, lmx_solver **out)
+{
+    if (!out) return fail(LMX_ERR_ARG, "null output handle");
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return fail(LMX_ERR_HIP, "no HIP device is available");
+    if (device < 0 || device >= n) return fail(LMX_ERR_ARG, "device index out of range");
+    lmx_solver *s = new lmx_solver();
+    s->device = device;
+    hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipMalloc((void **)&s->d_ctl, sizeof(lmx::Control));
+    if (e == hipSuccess) e = hipMemset(s->d_ctl, 0, sizeof(lmx::Control));
+    if (e == hipSuccess) e = hipMalloc((void **)&s->d_counts, 256 * sizeof(uint32_t));
+    if (e != hipSuccess) {
+        int rc = hip_fail(e, "lmx_create");
+        lmx_destroy(s);
+        return rc;
+    }
+    *out = s;
+    return LMX_OK;
+}
+
+int lmx_destroy(lmx_solver *s)
+{
+    if (!s) return LMX_OK;
+    hipSetDevice(s->device);
+    if (s->stream) hipStreamSynchronize(s->stream);
+    free_device(s);
+    dev_free(s->d_ctl);
+    dev_free(s->d_counts);
+    if (s->stream) hipStreamDestroy(s->stream);
+    delete s;
+    return LMX_OK;
+}
+
+int lmx_initialize(lmx_solver *s, uint32_t replicate, uint32_t seedTop, double timestep, int overflow_mode,
+                   int boundary)
+{
+    if (!s) return fail(LMX_ERR_ARG, "null solver");
+    if (!(timestep > 0.0)) return fail(LMX_ERR_ARG, "timestep: A positive timestep must be specified for the solver.");
+    if (overflow_mode != 0) return fail(LMX_ERR_ARG, "rdme.mpd.overflowhandler: only the classic handler is supported");
+    if (boundary != LMX_BOUNDARY_PERIODIC && boundary != LMX_BOUNDARY_REFLECTING)
+        return fail(LMX_ERR_ARG, "unknown boundary mode");
+    if (seedTop == 0) {
+        struct timespec ts;
+        clock_gettime(CLOCK_REALTIME, &ts);
+        seedTop = (uint32_t)ts.tv_nsec;
+    }
+    s->replicate = replicate;
+    s->seed32 = (seedTop << 16) | (replicate & 0x0000FFFFu);
+    s->tau = timestep;
+    s->overflow_mode = overflow_mode;
+    s->boundary = boundary;
+    s->initialized = true;
+    return LMX_OK;
+}
+
+int lmx_get_seed(lmx_solver *s, uint32_t *seed32)
+{
+    if (!s || !seed32) return fail(LMX_ERR_ARG, "null argument");
+    *seed32 = s->seed32;
+    return LMX_OK;
+}
+
+int lmx_build_model(lmx_solver *s, uint32_t nS, uint32_t R, const uint32_t *initialCounts, const uint32_t *types,
+                    const double *K, const int32_t *S, const uint32_t *D, uint32_t kCols)
+{
+    if (!s) return fail(LMX_ERR_ARG, "null solver");
+    if (!s->initialized) return fail(LMX_ERR_STATE, "initialize must be called before buildModel");
+    if (R > 0 && kCols == 0) return fail(LMX_ERR_ARG, "K: must have at least 1 column");
+    if (nS > 255) return fail(LMX_ERR_ARG, "a byte lattice holds at most 255 species");
+    if (R > 16384) return fail(LMX_ERR_ARG, "The number of reaction table entries exceeds the maximum supported by the solver (16384).");
+    if ((R > 0 || nS > 0) && (!types || !K || !S || !D) && R > 0) return fail(LMX_ERR_ARG, "null model array");
+    s->nS = nS;
+    s->R = R;
+    s->initialCounts.assign(initialCounts ? initialCounts : nullptr, initialCounts ? initialCounts + nS : nullptr);
+    s->types.assign(types, types + R);
+    s->k.assign(R, 0.0);
+    s->si.assign(R, 0);
+    s->s2i.assign(R, 0);
+    s->order.assign(R, 0);
+    s->D1.assign(R, 0);
+    s->D2.assign(R, 0);
+    s->S_sm.assign(S, S + (size_t)nS * R);
+    s->S_rm.assign((size_t)nS * R, 0);
+    for (uint32_t i = 0; i < R; i++) {
+        uint32_t deps[2] = {0, 0}, nd = 0;
+        for (uint32_t j = 0; j < nS; j++)
+            if (D[(size_t)j * R + i] == 1) {
+                if (nd < 2) deps[nd] = j;
+                nd++;
+            }
+        s->k[i] = K[(size_t)i * kCols];
+        switch (types[i]) {
+        case 0:
+            if (nd != 0) return fail(LMX_ERR_ARG, "D: zeroth order reaction cannot have any dependencies");
+            s->order[i] = lmx::kZeroOrder;
+            break;
+        case 1:
+            if (nd != 1) return fail(LMX_ERR_ARG, "D: first order reaction had invalid number of dependencies");
+            s->order[i] = lmx::kFirstOrder;
+            s->si[i] = deps[0];
+            s->D1[i] = deps[0] + 1;
+            break;
+        case 2:
+            if (nd != 2) return fail(LMX_ERR_ARG, "D: second order reaction had invalid number of dependencies");
+            s->order[i] = lmx::kSecondOrder;
+            s->si[i] = deps[0];
+            s->s2i[i] = deps[1];
+            s->D1[i] = deps[0] + 1;
+            s->D2[i] = deps[1] + 1;
+            break;
+        case 3:
+            if (nd != 1) return fail(LMX_ERR_ARG, "D: second order self reaction had invalid number of dependencies");
+            s->order[i] = lmx::kSecondOrderSelf;
+            s->si[i] = deps[0];
+            s->D1[i] = deps[0] + 1;
+            break;
+        default:
+            return fail(LMX_ERR_ARG, "reactionTypeA: the reaction type was not supported by the solver");
+        }
+    }
+    // S transposed to reaction-major int8 (MpdRdmeSolver.cu:317-325).
+    for (uint32_t rx = 0; rx < R; rx++)
+        for (uint32_t p = 0; p < nS; p++) s->S_rm[(size_t)rx * nS + p] = (int8_t)S[(size_t)R * p + rx];
+    s->model_built = true;
+    s->diffusion_built = false;
+    return LMX_OK;
+}
+
+int lmx_build_diffusion_model(lmx_solver *s, uint32_t nST, const double *DF, const uint32_t *RL, uint32_t X,
+                              uint32_t Y, uint32_t Z, uint32_t P, uint32_t bytesPerParticle, double lambda,
+                              const uint8_t *lattice, const uint8_t *sites, int rowMajor)
+{
+    if (!s) return fail(LMX_ERR_ARG, "null solver");
+    if (!s->model_built) return fail(LMX_ERR_STATE, "buildModel must be called before buildDiffusionModel");
+    if (!rowMajor) return fail(LMX_ERR_ARG, "rowMajorData: columns major lattice data is not currently supported");
+    if (bytesPerParticle != 1) return fail(LMX_ERR_ARG, "bytes_per_particle must be 1 for the byte lattice");
+    if (P != 4 && P != 8 && P != 16) return fail(LMX_ERR_ARG, "particlesPerSite must be 4, 8 or 16");
+    if (nST == 0 || nST > 256) return fail(LMX_ERR_ARG, "numberSiteTypes must be in 1..256");
+    if (X == 0 || Y == 0 || Z == 0) return fail(LMX_ERR_ARG, "incorrect lattice dimensions");
+    const unsigned long long N = (unsigned long long)X * Y * Z;
+    if (N >= (1ull << 32)) return fail(LMX_ERR_ARG, "lattice has 2^32 or more sites");
+    if (!DF || !lattice || !sites || (s->R > 0 && !RL)) return fail(LMX_ERR_ARG, "null model array");
+    int rc = select_device(s);
+    if (rc) return rc;
+    LMX_HIP(hipStreamSynchronize(s->stream));
+
+    s->nST = nST;
+    s->X = X; s->Y = Y; s->Z = Z; s->P = P; s->W = P / 4;
+    s->N = (size_t)N;
+    s->lambda = lambda;
+    const size_t nT = (size_t)s->nS * nST * nST;
+    s->DF.assign(DF, DF + nT);
+    s->RL.assign(RL ? RL : nullptr, RL ? RL + (size_t)s->R * nST : nullptr);
+    // Transition probabilities (MpdRdmeSolver.cu:408-421).
+    s->T.assign(nT, 0.0f);
+    for (size_t i = 0; i < nT; i++) {
+        const float q = (float)(DF[i] * s->tau / pow(lambda, 2));
+        if (q > 0.50f) return fail(LMX_ERR_ARG, "D: The specified diffusion coefficient is too high for the diffusion model.");
+        s->T[i] = q;
+    }
+    std::vector<uint32_t> native;
+    if ((rc = row_major_to_native(s, lattice, native))) return rc;
+    std::vector<uint8_t> nsites;
+    if ((rc = sites_row_major_to_native(s, sites, nsites))) return rc;
+    compute_propensities(s);
+
+    free_device(s);
+    const size_t lat_bytes = s->N * s->W * sizeof(uint32_t);
+    hipError_t e = hipMalloc((void **)&s->d_lat[0], lat_bytes);
+    if (e == hipSuccess) e = hipMalloc((void **)&s->d_lat[1], lat_bytes);
+    if (e == hipSuccess) e = hipMalloc((void **)&s->d_sites, s->N);
+    if (e != hipSuccess) {
+        free_device(s);
+        return fail(LMX_ERR_NOMEM, std::string("Failed to allocate the device lattice: ") + hipGetErrorString(e));
+    }
+    s->cur = 0;
+    LMX_HIP(hipMemcpy(s->d_lat[0], native.data(), lat_bytes, hipMemcpyHostToDevice));
+    LMX_HIP(hipMemset(s->d_lat[1], 0, lat_bytes));
+    LMX_HIP(hipMemcpy(s->d_sites, nsites.data(), s->N, hipMemcpyHostToDevice));
+    if ((rc = dev_upload(&s->d_T, s->T))) return rc;
+    if ((rc = dev_upload(&s->d_order, s->order))) return rc;
+    if ((rc = dev_upload(&s->d_D1, s->D1))) return rc;
+    if ((rc = dev_upload(&s->d_D2, s->D2))) return rc;
+    if ((rc = dev_upload(&s->d_S, s->S_rm))) return rc;
+    std::vector<uint8_t> rl8(s->RL.size());
+    for (size_t i = 0; i < rl8.size(); i++) rl8[i] = (uint8_t)s->RL[i];  // tmpRL (MpdRdmeSolver.cu:424-428)
+    if ((rc = dev_upload(&s->d_RL, rl8))) return rc;
+    if ((rc = upload_tables(s))) return rc;
+    LMX_HIP(hipMemset(s->d_ctl, 0, sizeof(lmx::Control)));
+    s->timestep = 0;
+    s->diffusion_built = true;
+    return LMX_OK;
+}
+
+int lmx_set_reaction_rate(lmx_solver *s, uint32_t rxid, float rate)
+{
+    if (!s) return fail(LMX_ERR_ARG, "null solver");
+    if (!s->model_built || rxid >= s->R) return fail(LMX_ERR_ARG, "reaction index is out of bounds");
+    s->k[rxid] = rate;
+    if (s->diffusion_built) compute_propensities(s);
+    return LMX_OK;
+}
+
+int lmx_set_path(lmx_solver *s, int path)
+{
+    if (!s) return fail(LMX_ERR_ARG, "null solver");
+    if (path != LMX_PATH_SWEEPS) return fail(LMX_ERR_ARG, "unsupported step path");
+    s->path = path;
+    return LMX_OK;
+}
+
+int lmx_run_steps(lmx_solver *s, uint32_t nsteps)
+{
+    int rc = require_ready(s);
+    if (rc) return rc;
+    if (s->tables_dirty) {
+        LMX_HIP(hipStreamSynchronize(s->stream));
+        if ((rc = upload_tables(s))) return rc;
+    }
+    const lmx::Params p = make_params(s);
+    for (uint32_t i = 0; i < nsteps; i++)
+        if ((rc = enqueue_step(s, p, nullptr, nullptr, nullptr))) return rc;
+    return LMX_OK;
+}
+
+int lmx_synchronize(lmx_solver *s)
+{
+    if (!s) return fail(LMX_ERR_ARG, "null solver");
+    int rc = select_device(s);
+    if (rc) return rc;
+    LMX_HIP(hipStreamSynchronize(s->stream));
+    if (s->d_ctl) return check_device_error(s);
+    return LMX_OK;
+}
+
+int lmx_get_timestep(lmx_solver *s, uint32_t *t)
+{
+    if (!s || !t) return fail(LMX_ERR_ARG, "null argument");
+    *t = s->timestep;
+    return LMX_OK;
+}
+
+int lmx_set_timestep(lmx_solver *s, uint32_t t)
+{
+    int rc = require_ready(s);
+    if (rc) return rc;
+    LMX_HIP(hipStreamSynchronize(s->stream));
+    LMX_HIP(hipMemcpy(&s->d_ctl->timestep, &t, sizeof(t), hipMemcpyHostToDevice));
+    s->timestep = t;
+    return LMX_OK;
+}
+
+int lmx_get_lattice_native(lmx_solver *s, uint32_t *native, size_t words)
+{
+    int rc = require_ready(s);
+    if (rc) return rc;
+    if (!native || words != s->N * s->W) return fail(LMX_ERR_ARG, "the buffer size was not equal to the lattice data size");
+    LMX_HIP(hipStreamSynchronize(s->stream));
+    LMX_HIP(hipMemcpy(native, s->d_lat[s->cur], words * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    return LMX_OK;
+}
+
+int lmx_get_lattice(lmx_solver *s, uint8_t *rm, size_t size)
+{
+    int rc = require_ready(s);
+    if (rc) return rc;
+    if (!rm || size != s->N * s->P) return fail(LMX_ERR_ARG, "bufferSize: the buffer size was not equal to the lattice data size");
+    std::vector<uint32_t> native(s->N * s->W);
+    if ((rc = lmx_get_lattice_native(s, native.data(), native.size()))) return rc;
+    native_to_row_major(s, native.data(), rm);
+    return LMX_OK;
+}
+
+int lmx_get_sites(lmx_solver *s, uint8_t *rm, size_t size)
+{
+    int rc = require_ready(s);
+    if (rc) return rc;
+    if (!rm || size != s->N) return fail(LMX_ERR_ARG, "bufferSize: the buffer size was not equal to the lattice site size");
+    std::vector<uint8_t> native(s->N);
+    LMX_HIP(hipStreamSynchronize(s->stream));
+    LMX_HIP(hipMemcpy(native.data(), s->d_sites, s->N, hipMemcpyDeviceToHost));
+    size_t idx = 0;
+    for (uint32_t x = 0; x < s->X; x++)
+        for (uint32_t y = 0; y < s->Y; y++)
+            for (uint32_t z = 0; z < s->Z; z++, idx++)
+                rm[idx] = native[(size_t)x + (size_t)s->X * y + (size_t)s->X * s->Y * z];
+    return LMX_OK;
+}
+
+int lmx_set_lattice_native(lmx_solver *s, const uint32_t *native, size_t words)
+{
+    int rc = require_ready(s);
+    if (rc) return rc;
+    if (!native || words != s->N * s->W) return fail(LMX_ERR_ARG, "the buffer size was not equal to the lattice data size");
+    for (size_t i = 0; i < words; i++)
+        for (int b = 0; b < 4; b++)
+            if (((native[i] >> (8 * b)) & 0xFFu) > s->nS)
+                return fail(LMX_ERR_ARG, "latticeData: an invalid species was found");
+    LMX_HIP(hipStreamSynchronize(s->stream));
+    LMX_HIP(hipMemcpy(s->d_lat[s->cur], native, words * sizeof(uint32_t), hipMemcpyHostToDevice));
+    return LMX_OK;
+}
+
+int lmx_set_lattice(lmx_solver *s, const uint8_t *rm, size_t size)
+{
+    int rc = require_ready(s);
+    if (rc) return rc;
+    if (!rm || size != s->N * s->P) return fail(LMX_ERR_ARG, "bufferSize: the buffer size was not equal to the lattice data size");
+    std::vector<uint32_t> native;
+    if ((rc = row_major_to_native(s, rm, native))) return rc;
+    LMX_HIP(hipStreamSynchronize(s->stream));
+    LMX_HIP(hipMemcpy(s->d_lat[s->cur], native.data(), native.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    return LMX_OK;
+}
+
+int lmx_set_sites(lmx_solver *s, const uint8_t *rm, size_t size)
+{
+    int rc = require_ready(s);
+    if (rc) return rc;
+    if (!rm || size != s->N) return fail(LMX_ERR_ARG, "bufferSize: the buffer size was not equal to the lattice site size");
+    std::vector<uint8_t> native;
+    if ((rc = sites_row_major_to_native(s, rm, native))) return rc;
+    LMX_HIP(hipStreamSynchronize(s->stream));
+    LMX_HIP(hipMemcpy(s->d_sites, native.data(), s->N, hipMemcpyHostToDevice));
+    return LMX_OK;
+}
+
+int lmx_get_species_counts(lmx_solver *s, uint32_t *counts, uint32_t n)
+{
+    int rc = require_ready(s);
+    if (rc) return rc;
+    if (!counts || n < s->nS) return fail(LMX_ERR_ARG, "counts buffer smaller than the number of species");
+    LMX_HIP(hipMemsetAsync(s->d_counts, 0, 256 * sizeof(uint32_t), s->stream));
+    hipError_t e = lmx::launch_counts(s->d_lat[s->cur], s->N * s->W, s->nS, s->d_counts, s->stream);
+    if (e != hipSuccess) return hip_fail(e, "count launch");
+    LMX_HIP(hipStreamSynchronize(s->stream));
+    std::vector<uint32_t> h(256);
+    LMX_HIP(hipMemcpy(h.data(), s->d_counts, 256 * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    for (uint32_t i = 0; i < n; i++) counts[i] = i < s->nS ? h[i] : 0u;
+    return LMX_OK;
+}
+
+int lmx_get_overflow_stats(lmx_solver *s, uint64_t *entries, uint64_t *steps)
+{
+    if (!s) return fail(LMX_ERR_ARG, "null solver");
+    int rc = select_device(s);
+    if (rc) return rc;
+    LMX_HIP(hipStreamSynchronize(s->stream));
+    lmx::Control c;
+    LMX_HIP(hipMemcpy(&c, s->d_ctl, offsetof(lmx::Control, ovf_keys), hipMemcpyDeviceToHost));
+    if (entries) *entries = c.ovf_total;
+    if (steps) *steps = c.ovf_steps;
+    return LMX_OK;
+}
+
+int lmx_time_steps(lmx_solver *s, uint32_t nsteps, double *total_ms, double *kernel_ms, uint64_t *kernel_launches,
+                   uint32_t nkernels)
+{
+    int rc = require_ready(s);
+    if (rc) return rc;
+    if (s->tables_dirty) {
+        LMX_HIP(hipStreamSynchronize(s->stream));
+        if ((rc = upload_tables(s))) return rc;
+    }
+    const lmx::Params p = make_params(s);
+    if (kernel_ms)
+        for (uint32_t k = 0; k < nkernels; k++) kernel_ms[k] = 0.0;
+    if (kernel_launches)
+        for (uint32_t k = 0; k < nkernels; k++) kernel_launches[k] = 0;
+    hipEvent_t t0, t1;
+    LMX_HIP(hipEventCreate(&t0));
+    LMX_HIP(hipEventCreate(&t1));
+    const int kMaxLaunches = 8;
+    const uint32_t chunk = 256;
+    std::vector<hipEvent_t> ev;
+    std::vector<int> kinds;
+    if (kernel_ms) {
+        ev.resize((size_t)chunk * (kMaxLaunches + 1));
+        for (auto &e : ev) LMX_HIP(hipEventCreate(&e));
+        kinds.resize((size_t)chunk * kMaxLaunches);
+    }
+    LMX_HIP(hipStreamSynchronize(s->stream));
+    LMX_HIP(hipEventRecord(t0, s->stream));
+    for (uint32_t base = 0; base < nsteps; base += chunk) {
+        const uint32_t m = (nsteps - base) < chunk ? (nsteps - base) : chunk;
+        std::vector<int> nl(m, 0);
+        for (uint32_t i = 0; i < m; i++) {
+            if (kernel_ms)
+                rc = enqueue_step(s, p, &ev[(size_t)i * (kMaxLaunches + 1)], &kinds[(size_t)i * kMaxLaunches], &nl[i]);
+            else
+                rc = enqueue_step(s, p, nullptr, nullptr, nullptr);
+            if (rc) return rc;
+        }
+        if (kernel_ms) {
+            LMX_HIP(hipEventSynchronize(ev[(size_t)(m - 1) * (kMaxLaunches + 1) + nl[m - 1]]));
+            for (uint32_t i = 0; i < m; i++) {
+                hipEvent_t *e = &ev[(size_t)i * (kMaxLaunches + 1)];
+                for (int j = 0; j < nl[i]; j++) {
+                    float ms = 0.0f;
+                    LMX_HIP(hipEventElapsedTime(&ms, e[j], e[j + 1]));
+                    const int kind = kinds[(size_t)i * kMaxLaunches + j];
+                    if ((uint32_t)kind < nkernels) {
+                        kernel_ms[kind] += ms;
+                        if (kernel_launches) kernel_launches[kind]++;
+                    }
+                }
+            }
+        }
+    }
+    LMX_HIP(hipEventRecord(t1, s->stream));
+    LMX_HIP(hipEventSynchronize(t1));
+    float ms = 0.0f;
+    LMX_HIP(hipEventElapsedTime(&ms, t0, t1));
+    if (total_ms) *total_ms = ms;
+    for (auto &e : ev) hipEventDestroy(e);
+    hipEventDestroy(t0);
+    hipEventDestroy(t1);
+    return check_device_error(s);
+}
+
+int lmx_get_table_f32(lmx_solver *s, int which, float *out, size_t n, size_t *written)
+{
+    if (!s) return fail(LMX_ERR_ARG, "null solver");
+    const std::vector<float> *v = nullptr;
+    switch (which) {
+    case 0: v = &s->T; break;
+    case 1: v = &s->qp0; break;
+    case 2: v = &s->qp1; break;
+    case 3: v = &s->qp2; break;
+    case 4: v = &s->rate; break;
+    default: return fail(LMX_ERR_ARG, "unknown table");
+    }
+    if (written) *written = v->size();
+    if (out) {
+        if (n < v->size()) return fail(LMX_ERR_ARG, "table buffer too small");
+        memcpy(out, v->data(), v->size() * sizeof(float));
+    }
+    return LMX_OK;
+}
+
+int lmx_device_exp(int device, const double *x, double *y, size_t n)
+{
+    if (!x || !y) return fail(LMX_ERR_ARG, "null argument");
+    LMX_HIP(hipSetDevice(device));
+    double *dx = nullptr, *dy = nullptr;
+    LMX_HIP(hipMalloc((void **)&dx, (n ? n : 1) * sizeof(double)));
+    LMX_HIP(hipMalloc((void **)&dy, (n ? n : 1) * sizeof(double)));
+    LMX_HIP(hipMemcpy(dx, x, n * sizeof(double), hipMemcpyHostToDevice));
+    hipError_t e = lmx::launch_exp(dx, dy, n, nullptr);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipMemcpy(y, dy, n * sizeof(double), hipMemcpyDeviceToHost);
+    hipFree(dx);
+    hipFree(dy);
+    if (e != hipSuccess) return hip_fail(e, "lmx_device_exp");
+    return LMX_OK;
+}
+
+int lmx_device_reaction_probability(int device, const float *a, float *prob, size_t n)
+{
+    if (!a || !prob) return fail(LMX_ERR_ARG, "null argument");
+    LMX_HIP(hipSetDevice(device));
+    float *da = nullptr, *dp = nullptr;
+    LMX_HIP(hipMalloc((void **)&da, (n ? n : 1) * sizeof(float)));
+    LMX_HIP(hipMalloc((void **)&dp, (n ? n : 1) * sizeof(float)));
+    LMX_HIP(hipMemcpy(da, a, n * sizeof(float), hipMemcpyHostToDevice));
+    hipError_t e = lmx::launch_prob(da, dp, n, nullptr);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipMemcpy(prob, dp, n * sizeof(float), hipMemcpyDeviceToHost);
+    hipFree(da);
+    hipFree(dp);
+    if (e != hipSuccess) return hip_fail(e, "lmx_device_reaction_probability");
+    return LMX_OK;
+}
+
+}  // extern "C"

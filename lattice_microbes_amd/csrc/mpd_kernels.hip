@@ -1,0 +1,473 @@
+// MPD-RDME step kernels for gfx950 (CDNA4, wave64).
+//
+//  sweep_kernel     one diffusion sweep along x, y or z (mpd_{x,y,z}_kernel,
+//                   MpdRdmeSolver.cu:1342-1490 + byte_diffusion_1d_dev.cu)
+//  reaction_kernel  per-site reaction from the precomputed propensity tables
+//                   (precomp_reaction_kernel, MpdRdmeSolver.cu:1592-1675 +
+//                   byte_reaction_dev.cu:62-257)
+//  resolve_kernel   canonical-order overflow placement on the device (the
+//                   classic host handler, MpdRdmeSolver.cu:1103-1189)
+//  count_kernel     species counts (ByteLattice::getParticleCounts)
+//
+// Every per-particle decision depends only on (site index, slot, timestep,
+// substep) through the counter-based hash, so the results do not depend on
+// how sites are distributed over threads.
+#include "lmx_device.h"
+#include "lmx_kernels.h"
+
+namespace lmx {
+
+// ---------------------------------------------------------------------------
+// Diffusion sweep.  Site i's new content is its own stayers, then the minus
+// neighbour's PLUS movers, then the plus neighbour's MINUS movers, each in
+// slot order (performPropagation, byte_diffusion_1d_dev.cu:199-270).  A
+// particle moves minus iff u < qm and plus iff 0.5 <= u < qp + 0.5
+// (makeDiffusionChoices, :86-121); q <= 0.5 makes the two exclusive, so a
+// site only evaluates the decisions that can move a particle into it.
+// ---------------------------------------------------------------------------
+template <int W>
+__global__ __launch_bounds__(256) void sweep_kernel(Params p, const uint32_t *__restrict__ src,
+                                                    uint32_t *__restrict__ dst, uint32_t stride,
+                                                    uint32_t len, uint32_t substep)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= p.N) return;
+    const unsigned long long h = timestep_hash(p.seed32, p.ctl->timestep, substep);
+    const uint32_t c = (i / stride) % len;
+    const bool has_m = (c > 0) || (p.boundary == 0);
+    const bool has_p = (c + 1 < len) || (p.boundary == 0);
+    const uint32_t im = (c > 0) ? i - stride : i + (len - 1) * stride;
+    const uint32_t ip = (c + 1 < len) ? i + stride : i - (len - 1) * stride;
+
+    uint32_t wi[W], wm[W], wp[W];
+    uint32_t any = 0;
+#pragma unroll
+    for (int k = 0; k < W; k++) {
+        wi[k] = src[(size_t)k * p.N + i];
+        wm[k] = has_m ? src[(size_t)k * p.N + im] : 0u;
+        wp[k] = has_p ? src[(size_t)k * p.N + ip] : 0u;
+        any |= wi[k] | wm[k] | wp[k];
+    }
+    if (any == 0) {
+#pragma unroll
+        for (int k = 0; k < W; k++) dst[(size_t)k * p.N + i] = 0u;
+        return;
+    }
+    const uint32_t sti = p.sites[i];
+    const uint32_t stm = has_m ? p.sites[im] : 0u;
+    const uint32_t stp = has_p ? p.sites[ip] : 0u;
+
+    SlotBuf<W> out;
+    out.clear();
+#pragma unroll
+    for (int j = 0; j < 4 * W; j++) {
+        const uint32_t part = slot_of(wi, j);
+        if (part == 0) continue;
+        const float u = uniform01(i, p.pcb, j, h);
+        const float qm = has_m ? lookup_T(p, part, sti, stm) : 0.0f;
+        const float qp = has_p ? lookup_T(p, part, sti, stp) : 0.0f;
+        const bool leaves = (u < qm) || (u >= 0.5f && u < (qp + 0.5f));
+        if (!leaves && !out.push(part)) record_overflow(p.ctl, substep, i, out.n - 1 - 4 * W, part);
+    }
+    if (has_m) {
+#pragma unroll
+        for (int j = 0; j < 4 * W; j++) {
+            const uint32_t part = slot_of(wm, j);
+            if (part == 0) continue;
+            const float u = uniform01(im, p.pcb, j, h);
+            const float qp = lookup_T(p, part, stm, sti);
+            if (u >= 0.5f && u < (qp + 0.5f) && !out.push(part))
+                record_overflow(p.ctl, substep, i, out.n - 1 - 4 * W, part);
+        }
+    }
+    if (has_p) {
+#pragma unroll
+        for (int j = 0; j < 4 * W; j++) {
+            const uint32_t part = slot_of(wp, j);
+            if (part == 0) continue;
+            const float u = uniform01(ip, p.pcb, j, h);
+            const float qm = lookup_T(p, part, stp, sti);
+            if (u < qm && !out.push(part)) record_overflow(p.ctl, substep, i, out.n - 1 - 4 * W, part);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < W; k++) dst[(size_t)k * p.N + i] = out.w[k];
+}
+
+// ---------------------------------------------------------------------------
+// Reaction phase, in place.
+// ---------------------------------------------------------------------------
+
+// calculateReactionPropensity, global-R branch (byte_reaction_dev.cu:77-100).
+template <int W>
+__device__ inline float reaction_propensity(const Params &p, uint32_t st, const uint32_t *w, uint32_t r)
+{
+    if (!p.RL[r * p.nST + st]) return 0.0f;
+    const uint32_t d1 = p.D1[r], d2 = p.D2[r];
+    float n1 = 0.0f, n2 = 0.0f;
+#pragma unroll
+    for (int j = 0; j < 4 * W; j++) {
+        const uint32_t s = slot_of(w, j);
+        if (s > 0) {
+            n1 += (s == d1) ? 1.0f : 0.0f;
+            n2 += (s == d2) ? 1.0f : 0.0f;
+        }
+    }
+    const float k = p.rate[r];
+    switch (p.order[r]) {
+    case kZeroOrder: return k;
+    case kFirstOrder: return k * n1;
+    case kSecondOrder: return k * n1 * n2;
+    case kSecondOrderSelf: return k * n1 * (n1 - 1.0f);
+    }
+    return 0.0f;
+}
+
+template <int W>
+__global__ __launch_bounds__(256) void reaction_kernel(Params p, uint32_t *__restrict__ lat)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= p.N) return;
+    const uint32_t st = p.sites[i];
+    uint32_t w[W];
+    uint32_t any = 0;
+#pragma unroll
+    for (int k = 0; k < W; k++) {
+        w[k] = lat[(size_t)k * p.N + i];
+        any |= w[k];
+    }
+    const uint32_t nS = p.nS;
+    // Total propensity in the reference's summation order
+    // (MpdRdmeSolver.cu:1617-1639).
+    float a = p.qp0[st];
+    if (any != 0) {
+#pragma unroll
+        for (int m = 0; m < 4 * W; m++) {
+            const uint32_t p1 = slot_of(w, m);
+            if (p1 > 0) {
+                a += p.qp1[st * nS + (p1 - 1)];
+#pragma unroll
+                for (int n = m + 1; n < 4 * W; n++) {
+                    const uint32_t p2 = slot_of(w, n);
+                    if (p2 > 0) a += p.qp2[(st * nS + (p1 - 1)) * nS + (p2 - 1)];
+                }
+            }
+        }
+    }
+    if (a == 0.0f) return;
+    const unsigned long long h = timestep_hash(p.seed32, p.ctl->timestep, 3);
+    const float prob = reaction_probability(a);
+    // checkForReaction (byte_reaction_dev.cu:139-144).
+    if (!(prob > 0.0f && uniform01(i, 1, 0, h) <= prob)) return;
+    // determineReactionIndex (:146-177).
+    float rp = uniform01(i, 1, 1, h) * a;
+    uint32_t idx = 0;
+    for (uint32_t r = 0; r < p.R; r++) {
+        const float pr = reaction_propensity<W>(p, st, w, r);
+        if (pr > 0.0f) {
+            if (rp > 0.0f) idx = r;
+            rp -= pr;
+        }
+    }
+    // evaluateReaction (:179-257) without the device-heap copy of the S row:
+    // the m-th particle of species s is consumed iff fewer than -S[s] earlier
+    // particles of s were seen.
+    const int8_t *Srow = p.S + (size_t)idx * nS;
+    SlotBuf<W> out;
+    out.clear();
+#pragma unroll
+    for (int m = 0; m < 4 * W; m++) {
+        const uint32_t s = slot_of(w, m);
+        if (s == 0) continue;
+        const int sv = Srow[s - 1];
+        if (sv >= 0) {
+            out.push(s);
+        } else {
+            uint32_t earlier = 0;
+#pragma unroll
+            for (int q = 0; q < m; q++) earlier += (slot_of(w, q) == s) ? 1u : 0u;
+            if ((int)earlier >= -sv) out.push(s);
+        }
+    }
+    // Products in species order; a reactant that was not present leaves a
+    // negative residue, which the reference turns into an unbounded loop.
+    bool underflow = false;
+    for (uint32_t s = 0; s < nS; s++) {
+        int sv = Srow[s];
+        if (sv == 0) continue;
+        if (sv < 0) {
+            uint32_t present = 0;
+#pragma unroll
+            for (int q = 0; q < 4 * W; q++) present += (slot_of(w, q) == s + 1) ? 1u : 0u;
+            if ((int)present < -sv) underflow = true;
+            continue;
+        }
+        for (int j = 0; j < sv; j++) {
+            if (!out.push(s + 1)) record_overflow(p.ctl, 3, i, out.n - 1 - 4 * W, s + 1);
+        }
+    }
+    if (underflow) {
+        if (atomicCAS(&p.ctl->error, 0u, kErrReactionUnderflow) == 0u) p.ctl->error_step = p.ctl->timestep;
+    }
+#pragma unroll
+    for (int k = 0; k < W; k++) lat[(size_t)k * p.N + i] = out.w[k];
+}
+
+// ---------------------------------------------------------------------------
+// Overflow resolution: one workgroup.  Entries are sorted into canonical
+// order (phase, site, emission) and placed one after another; for each, the
+// workgroup scans the reference's candidate list (Lattice::getNearbySites,
+// Lattice.cpp:106-128: radius 0..4, shell max(R-1,0)^2 < d^2 <= R^2, x outer,
+// z inner) 256 candidates at a time and takes the first site with room and
+// the origin's site type.  The step counter advances at the end.
+// ---------------------------------------------------------------------------
+__device__ inline uint32_t load_agent(const uint32_t *ptr)
+{
+    return __hip_atomic_load(ptr, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ inline void store_agent(uint32_t *ptr, uint32_t v)
+{
+    __hip_atomic_store(ptr, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ inline uint32_t site_occupancy(const Params &p, const uint32_t *lat, uint32_t s)
+{
+    uint32_t occ = 0;
+    for (uint32_t k = 0; k < p.W; k++) {
+        const uint32_t word = load_agent(lat + (size_t)k * p.N + s);
+        for (uint32_t b = 0; b < 4; b++) {
+            if (((word >> (8 * b)) & 0xFFu) == 0) return occ;
+            occ++;
+        }
+    }
+    return occ;
+}
+
+__device__ inline void set_error(Control *ctl, uint32_t code, uint32_t step)
+{
+    if (atomicCAS(&ctl->error, 0u, code) == 0u) ctl->error_step = step;
+}
+
+constexpr int kResolveThreads = 256;
+
+__global__ __launch_bounds__(kResolveThreads) void resolve_kernel(Params p, uint32_t *lat)
+{
+    __shared__ unsigned long long keys[kMaxOverflows];
+    __shared__ int best;
+    __shared__ uint32_t s_t;
+    Control *ctl = p.ctl;
+    const int tid = threadIdx.x;
+    if (tid == 0) s_t = ctl->timestep;
+    const uint32_t n = ctl->ovf_count;
+    __syncthreads();
+    const uint32_t t = s_t;
+    if (n == 0) {
+        if (tid == 0) ctl->timestep = t + 1;
+        return;
+    }
+    if (n > kMaxOverflows) {
+        if (tid == 0) {
+            set_error(ctl, kErrOverflowCap, t);
+            ctl->ovf_count = 0;
+            ctl->timestep = t + 1;
+        }
+        return;
+    }
+    uint32_t M = 2;
+    while (M < n) M <<= 1;
+    for (uint32_t k = tid; k < M; k += kResolveThreads) keys[k] = (k < n) ? ctl->ovf_keys[k] : ~0ull;
+    __syncthreads();
+    for (uint32_t k = 2; k <= M; k <<= 1) {
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t a = tid; a < M; a += kResolveThreads) {
+                const uint32_t b = a ^ j;
+                if (b > a) {
+                    const unsigned long long ka = keys[a], kb = keys[b];
+                    const bool up = (a & k) == 0;
+                    if ((ka > kb) == up) {
+                        keys[a] = kb;
+                        keys[b] = ka;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+
+    const uint32_t X = p.X, Y = p.Y, Z = p.Z;
+    bool failed = false;
+    for (uint32_t e = 0; e < n && !failed; e++) {
+        const unsigned long long key = keys[e];
+        const uint32_t site = (uint32_t)((key >> 24) & 0xFFFFFFFFull);
+        const uint32_t particle = (uint32_t)(key & 0xFFull);
+        const uint32_t xc = site % X, yc = (site / X) % Y, zc = site / p.XY;
+        const uint32_t st0 = p.sites[site];
+        int found = 0x7fffffff;
+        for (uint32_t R = 0; R <= kMaxReplacementDist && found == 0x7fffffff; R++) {
+            const uint32_t minD = R > 0 ? R - 1 : 0;
+            const uint32_t x0 = xc >= R ? xc - R : 0, x1 = (xc <= X - 1 - R) ? xc + R : X - 1;
+            const uint32_t y0 = yc >= R ? yc - R : 0, y1 = (yc <= Y - 1 - R) ? yc + R : Y - 1;
+            const uint32_t z0 = zc >= R ? zc - R : 0, z1 = (zc <= Z - 1 - R) ? zc + R : Z - 1;
+            const uint32_t ny = y1 - y0 + 1, nz = z1 - z0 + 1;
+            const uint32_t total = (x1 - x0 + 1) * ny * nz;
+            for (uint32_t base = 0; base < total; base += kResolveThreads) {
+                if (tid == 0) best = 0x7fffffff;
+                __syncthreads();
+                const uint32_t c = base + tid;
+                if (c < total) {
+                    const uint32_t x = x0 + c / (ny * nz);
+                    const uint32_t rem = c % (ny * nz);
+                    const uint32_t y = y0 + rem / nz;
+                    const uint32_t z = z0 + rem % nz;
+                    const long long dx = (long long)x - xc, dy = (long long)y - yc, dz = (long long)z - zc;
+                    const long long r2 = dx * dx + dy * dy + dz * dz;
+                    const bool shell = r2 <= (long long)(R * R) && (r2 > (long long)(minD * minD) || R == 0);
+                    if (shell) {
+                        if (x >= X || y >= Y || z >= Z) {
+                            atomicMin(&best, (int)(2 * c + 1));
+                        } else {
+                            const uint32_t s2 = x + X * y + p.XY * z;
+                            if (p.sites[s2] == st0 && site_occupancy(p, lat, s2) < p.P)
+                                atomicMin(&best, (int)(2 * c));
+                        }
+                    }
+                }
+                __syncthreads();
+                const int b = best;
+                __syncthreads();
+                if (b != 0x7fffffff) {
+                    found = b;
+                    break;
+                }
+            }
+            if (found != 0x7fffffff) {
+                if (found & 1) {
+                    if (tid == 0) set_error(ctl, kErrOutOfRange, t);
+                    failed = true;
+                } else if (tid == 0) {
+                    const uint32_t c = (uint32_t)found >> 1;
+                    const uint32_t x = x0 + c / (ny * nz);
+                    const uint32_t rem = c % (ny * nz);
+                    const uint32_t y = y0 + rem / nz;
+                    const uint32_t z = z0 + rem % nz;
+                    const uint32_t s2 = x + X * y + p.XY * z;
+                    const uint32_t occ = site_occupancy(p, lat, s2);
+                    uint32_t *wp = lat + (size_t)(occ >> 2) * p.N + s2;
+                    const uint32_t old = load_agent(wp);
+                    store_agent(wp, old | (particle << (8 * (occ & 3))));
+                }
+            }
+        }
+        if (found == 0x7fffffff) {
+            if (tid == 0) set_error(ctl, kErrUnplaceable, t);
+            failed = true;
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        ctl->ovf_total += n;
+        ctl->ovf_steps += 1;
+        ctl->ovf_count = 0;
+        ctl->timestep = t + 1;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Species counts (ByteLattice::getParticleCounts, ByteLattice.cpp:495-520):
+// each word counts up to its first empty byte.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void count_kernel(const uint32_t *__restrict__ lat, size_t nwords,
+                                                    uint32_t nS, uint32_t *__restrict__ counts)
+{
+    __shared__ uint32_t hist[256];
+    for (uint32_t k = threadIdx.x; k < 256; k += blockDim.x) hist[k] = 0;
+    __syncthreads();
+    for (size_t wi = (size_t)blockIdx.x * blockDim.x + threadIdx.x; wi < nwords;
+         wi += (size_t)gridDim.x * blockDim.x) {
+        const uint32_t word = lat[wi];
+        if (word == 0) continue;
+        for (int b = 0; b < 4; b++) {
+            const uint32_t s = (word >> (8 * b)) & 0xFFu;
+            if (s == 0) break;
+            atomicAdd(&hist[s], 1u);
+        }
+    }
+    __syncthreads();
+    for (uint32_t k = threadIdx.x; k < 256; k += blockDim.x)
+        if (k >= 1 && k <= nS && hist[k]) atomicAdd(&counts[k - 1], hist[k]);
+}
+
+__global__ void exp_kernel(const double *x, double *y, size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = exp_proj(x[i]);
+}
+
+__global__ void prob_kernel(const float *a, float *y, size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = reaction_probability(a[i]);
+}
+
+// ---------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------
+static inline unsigned blocks_for(size_t n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }
+
+hipError_t launch_sweep(const Params &p, const uint32_t *src, uint32_t *dst, int axis, hipStream_t st)
+{
+    const uint32_t stride = axis == 0 ? 1u : (axis == 1 ? p.X : p.XY);
+    const uint32_t len = axis == 0 ? p.X : (axis == 1 ? p.Y : p.Z);
+    const dim3 grid(blocks_for(p.N, 256)), block(256);
+    switch (p.W) {
+    case 1: hipLaunchKernelGGL(sweep_kernel<1>, grid, block, 0, st, p, src, dst, stride, len, (uint32_t)axis); break;
+    case 2: hipLaunchKernelGGL(sweep_kernel<2>, grid, block, 0, st, p, src, dst, stride, len, (uint32_t)axis); break;
+    case 4: hipLaunchKernelGGL(sweep_kernel<4>, grid, block, 0, st, p, src, dst, stride, len, (uint32_t)axis); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_reaction(const Params &p, uint32_t *lat, hipStream_t st)
+{
+    const dim3 grid(blocks_for(p.N, 256)), block(256);
+    switch (p.W) {
+    case 1: hipLaunchKernelGGL(reaction_kernel<1>, grid, block, 0, st, p, lat); break;
+    case 2: hipLaunchKernelGGL(reaction_kernel<2>, grid, block, 0, st, p, lat); break;
+    case 4: hipLaunchKernelGGL(reaction_kernel<4>, grid, block, 0, st, p, lat); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_resolve(const Params &p, uint32_t *lat, hipStream_t st)
+{
+    hipLaunchKernelGGL(resolve_kernel, dim3(1), dim3(kResolveThreads), 0, st, p, lat);
+    return hipGetLastError();
+}
+
+hipError_t launch_counts(const uint32_t *lat, size_t nwords, uint32_t nS, uint32_t *counts, hipStream_t st)
+{
+    unsigned g = blocks_for(nwords, 256);
+    if (g > 4096) g = 4096;
+    if (g == 0) g = 1;
+    hipLaunchKernelGGL(count_kernel, dim3(g), dim3(256), 0, st, lat, nwords, nS, counts);
+    return hipGetLastError();
+}
+
+hipError_t launch_exp(const double *x, double *y, size_t n, hipStream_t st)
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(exp_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, st, x, y, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_prob(const float *a, float *y, size_t n, hipStream_t st)
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(prob_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, st, a, y, n);
+    return hipGetLastError();
+}
+
+}  // namespace lmx

@@ -1,0 +1,136 @@
+/*
+ * Serial CPU restatement of the MPD-RDME timestep (lm::rdme::MpdRdmeSolver).
+ *
+ * TEST INFRASTRUCTURE ONLY.  This oracle is the checker that the HIP product
+ * path is compared against.  Nothing in the product (lattice_microbes_amd/)
+ * may include, link or call it.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py use it.
+ *
+ * Every function cites the reference file:line whose behaviour it restates
+ * (paths relative to the Lattice Microbes 2.5.0 source tree).
+ *
+ * Parity pinning (what fixes each function independently of this code):
+ *   orc_hash / orc_timestep_hash / orc_uniform : Python big-int and numpy
+ *       uint64 restatements plus committed known-answer vectors.
+ *   orc_exp                                     : correctly rounded exp from
+ *       Python's decimal module (<= 1 ulp, and correct rounding rate).
+ *   orc_build_*                                 : closed forms of the tables
+ *       recomputed in tests from the model arrays (T = D*tau/lambda^2 etc).
+ *   sweeps                                      : two formulations (choice
+ *       array as the reference, and per-site pull) must agree bit for bit;
+ *       conservation, compaction, periodic-neighbour known answers, single
+ *       particle hop frequencies vs T.
+ *   reaction                                    : single-site closed forms
+ *       (P(react) = 1-exp(-a)), selection frequencies proportional to
+ *       propensities, stoichiometry.
+ *   overflow                                    : hand-built full-site
+ *       scenarios with placement computed by hand from the search order.
+ */
+#ifndef LMX_ORACLE_H
+#define LMX_ORACLE_H
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    ORC_OK = 0,
+    ORC_ERR_ARG = 1,
+    ORC_ERR_OVERFLOW_CAP = 2,        /* > 2048 overflow entries in a step */
+    ORC_ERR_OVERFLOW_UNPLACEABLE = 3,/* no site within radius 4 */
+    ORC_ERR_OVERFLOW_OUT_OF_RANGE = 4,/* search reached a site outside the lattice */
+    ORC_ERR_REACTION_UNDERFLOW = 5   /* reaction applied without its reactants */
+};
+
+#define ORC_MAX_OVERFLOWS 2048u
+#define ORC_MAX_REPLACEMENT_DIST 4u
+
+/* Reaction order codes used by the MPD kernels (byte_reaction_dev.cu:56-59). */
+#define ORC_ZERO_ORDER 1u
+#define ORC_FIRST_ORDER 2u
+#define ORC_SECOND_ORDER 3u
+#define ORC_SECOND_ORDER_SELF 4u
+
+typedef struct {
+    uint32_t X, Y, Z;   /* lattice dimensions */
+    uint32_t P;         /* particle slots per site: 4, 8 or 16 */
+    uint32_t nS;        /* species */
+    uint32_t nST;       /* site types */
+    uint32_t R;         /* reactions */
+    uint32_t boundary;  /* 0 = periodic (every reference build), 1 = reflecting */
+    uint32_t seed32;    /* (seedTop<<16)|(replicate&0xFFFF) */
+    const float *T;     /* [nST][nST][nS] transition probabilities */
+    const float *qp0;   /* [nST] */
+    const float *qp1;   /* [nST][nS] */
+    const float *qp2;   /* [nST][nS][nS] */
+    const float *rate;  /* [R] model_reactionRates */
+    const uint32_t *order; /* [R] ORC_*_ORDER */
+    const uint32_t *D1; /* [R] first reactant species+1 (0: none) */
+    const uint32_t *D2; /* [R] second reactant species+1 (0: none) */
+    const int8_t *S;    /* [R][nS] reaction-major stoichiometry */
+    const uint8_t *RL;  /* [R][nST] reaction location matrix */
+} orc_model;
+
+typedef struct {
+    uint64_t overflow_entries;   /* total overflow entries resolved */
+    uint64_t overflow_steps;     /* steps with at least one entry */
+    uint64_t reactions;          /* reactions applied */
+    uint32_t last_error_step;
+    uint32_t pad;
+} orc_stats;
+
+/* RNG (xor_random_dev.cu:46-81, MpdRdmeSolver.cu:987-994). */
+uint32_t orc_hash(uint64_t v, uint64_t seed);
+uint64_t orc_timestep_hash(uint32_t seed32, uint32_t timestep, uint32_t substep);
+float orc_uniform(uint32_t v1, uint32_t shift, uint32_t v2, uint64_t seed);
+uint32_t orc_particle_count_bits(uint32_t P);
+
+/* Project double exp (Cody-Waite reduction + degree-13 Taylor, explicit fma). */
+double orc_exp(double x);
+float orc_reaction_probability(float a);
+
+/* Tables (MpdRdmeSolver.cu:232-364, :366-483, :697-774). */
+int orc_build_reaction_tables(uint32_t nS, uint32_t R, const uint32_t *types,
+                              const double *K, uint32_t kCols, const int32_t *S_species_major,
+                              const uint32_t *D_species_major,
+                              uint32_t *order_out, uint32_t *D1_out, uint32_t *D2_out,
+                              int8_t *S_out, uint32_t *si_out, uint32_t *s2i_out, double *k_out);
+int orc_build_T(uint32_t nS, uint32_t nST, const double *DF, double tau, double lambda, float *T_out);
+int orc_compute_propensities(uint32_t nS, uint32_t nST, uint32_t R, const uint32_t *types,
+                             const double *k, const uint32_t *si, const uint32_t *s2i,
+                             const uint32_t *RL, double tau, uint32_t X, uint32_t Y, uint32_t Z,
+                             float *rate_out, float *qp0_out, float *qp1_out, float *qp2_out);
+
+/* Lattice layout (RDMESolver.cpp:144-191, ByteLattice.cpp:208-520). */
+int orc_lattice_from_row_major(uint32_t X, uint32_t Y, uint32_t Z, uint32_t P, uint32_t nS,
+                               const uint8_t *row_major, uint32_t *native_out);
+void orc_lattice_to_row_major(uint32_t X, uint32_t Y, uint32_t Z, uint32_t P,
+                              const uint32_t *native, uint8_t *row_major_out);
+int orc_sites_from_row_major(uint32_t X, uint32_t Y, uint32_t Z, uint32_t nST,
+                             const uint8_t *row_major, uint8_t *native_out);
+void orc_species_counts(uint32_t N, uint32_t P, const uint32_t *native, uint32_t nS, uint32_t *counts);
+
+/* One diffusion sweep along axis (0=x,1=y,2=z): src -> dst, overflow entries appended.
+ * formulation 0: choice array (byte_diffusion_1d_dev.cu:86-197 + performPropagation :199-270)
+ * formulation 1: per-site pull of only the decisions a site depends on. */
+int orc_sweep(const orc_model *m, const uint32_t *src, const uint8_t *sites, uint32_t *dst,
+              uint32_t axis, uint64_t hash, int formulation,
+              uint32_t *ovf_site, uint8_t *ovf_particle, uint32_t *ovf_count);
+/* Reaction phase in place (MpdRdmeSolver.cu:1592-1675, byte_reaction_dev.cu:62-257). */
+int orc_react(const orc_model *m, uint32_t *lat, const uint8_t *sites, uint64_t hash,
+              uint32_t *ovf_site, uint8_t *ovf_particle, uint32_t *ovf_count, uint64_t *reactions);
+/* Classic overflow resolution (MpdRdmeSolver.cu:1103-1189, Lattice.cpp:106-128). */
+int orc_resolve_overflows(const orc_model *m, uint32_t *lat, const uint8_t *sites,
+                          const uint32_t *ovf_site, const uint8_t *ovf_particle, uint32_t count);
+
+/* Full timesteps t0 .. t0+nsteps-1 (MpdRdmeSolver.cu:996-1193).  lat is the
+ * native lattice (W=P/4 planes of N words), updated in place. */
+int orc_run(const orc_model *m, uint32_t *lat, const uint8_t *sites, uint32_t t0,
+            uint32_t nsteps, int formulation, orc_stats *stats);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,168 @@
+"""GPU parity: the HIP step path against the serial oracle, bit for bit.
+
+Every case builds the same model arrays for the product (through the C ABI)
+and for the oracle, runs the same seeded steps and compares the native
+lattices word for word.  Integer state, so the bar is exact equality.
+"""
+import numpy as np
+import pytest
+
+import lattice_microbes_amd as lmx
+from lattice_microbes_amd import models
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def run_pair(spec, steps, seed=5, replicate=0, chunks=1, path=0):
+    solver = lmx.MpdRdmeSolver(0)
+    spec.load_into(solver, replicate=replicate, seed=seed)
+    if path:
+        solver.setPath(path)
+    orc = oracle.Oracle(spec, seed_top=seed, replicate=replicate)
+    assert solver.seed == orc.seed32
+    per = steps // chunks
+    for _ in range(chunks):
+        solver.runTimesteps(per)
+        rc = orc.run(per)
+        assert rc == 0, f"oracle error {rc}"
+        gpu = solver.getLatticeNative()
+        ref = orc.native()
+        if not np.array_equal(gpu, ref):
+            bad = np.argwhere(gpu != ref)
+            raise AssertionError(f"{spec.name} {spec.shape} P={spec.P}: {len(bad)} words differ, first {bad[:5].tolist()}")
+    return solver, orc
+
+
+CASES = [
+    # shape, P, nS, nST, R, density, boundary
+    ((8, 6, 5), 8, 4, 3, 6, 0.3, "periodic"),
+    ((8, 6, 5), 8, 4, 3, 6, 0.3, "reflecting"),
+    ((7, 5, 3), 4, 3, 2, 4, 0.4, "periodic"),
+    ((7, 5, 3), 16, 5, 2, 5, 0.25, "reflecting"),
+    ((1, 4, 6), 8, 3, 2, 3, 0.2, "periodic"),
+    ((2, 2, 2), 8, 3, 1, 3, 0.3, "periodic"),
+    ((33, 17, 9), 8, 6, 4, 10, 0.1, "periodic"),
+    ((64, 8, 8), 16, 6, 3, 8, 0.2, "periodic"),
+    ((20, 20, 20), 8, 8, 5, 12, 0.05, "reflecting"),
+    ((31, 7, 13), 4, 4, 3, 6, 0.5, "periodic"),
+]
+
+
+@pytest.mark.parametrize("case", CASES, ids=[f"{c[0]}-P{c[1]}-{c[6]}" for c in CASES])
+def test_random_models_bit_exact(gpu, case):
+    shape, P, nS, nST, R, density, boundary = case
+    for seed in (1, 2):
+        spec = models.random_model(shape=shape, P=P, nS=nS, nST=nST, R=R, density=density, seed=seed,
+                                   boundary=boundary)
+        run_pair(spec, steps=8, seed=seed * 7 + 1, replicate=seed, chunks=2)
+
+
+def test_tables_match_oracle(gpu):
+    spec = models.random_model(shape=(9, 7, 5), P=8, nS=6, nST=4, R=12, seed=3)
+    solver = lmx.MpdRdmeSolver(0)
+    spec.load_into(solver, seed=3)
+    orc = oracle.Oracle(spec, seed_top=3)
+    for name, ref in (("T", orc.T), ("qp0", orc.qp0), ("qp1", orc.qp1), ("qp2", orc.qp2), ("rate", orc.rate)):
+        got = solver.getTable(name)
+        assert got.view(np.uint32).tolist() == ref[: got.size].view(np.uint32).tolist(), name
+
+
+def test_bimolecular_64(gpu):
+    spec = models.bimolecular(shape=(64, 64, 64), nA=3000, nB=3000, kf=1e9, seed=2)
+    solver, orc = run_pair(spec, steps=40, seed=9, chunks=4)
+    np.testing.assert_array_equal(solver.getSpeciesCounts(), orc.species_counts())
+
+
+def test_capsule_reduced(gpu):
+    spec = models.capsule_minde(shape=(32, 32, 64), counts=(800, 800, 400), seed=4)
+    run_pair(spec, steps=30, seed=4, chunks=3)
+
+
+def test_dense_overflow_heavy(gpu):
+    # Nearly half full: many overflows per step, resolved in canonical order.
+    spec = models.random_model(shape=(12, 10, 8), P=8, nS=3, nST=1, R=0, density=0.45, seed=11,
+                               qmax=0.5)
+    solver, orc = run_pair(spec, steps=10, seed=13, chunks=5)
+    e_gpu, s_gpu = solver.getOverflowStats()
+    assert e_gpu == orc.stats.overflow_entries and e_gpu > 0
+    assert s_gpu == orc.stats.overflow_steps
+
+
+def test_species_counts_and_readback(gpu):
+    spec = models.random_model(shape=(10, 9, 8), P=16, nS=7, nST=2, R=5, density=0.2, seed=21)
+    solver, orc = run_pair(spec, steps=6, seed=2)
+    np.testing.assert_array_equal(solver.getSpeciesCounts(), orc.species_counts())
+    np.testing.assert_array_equal(solver.getLattice(), orc.row_major())
+    np.testing.assert_array_equal(solver.getSites(), spec.sites)
+
+
+def test_device_exp_matches_oracle(gpu):
+    rng = np.random.default_rng(0)
+    x = np.concatenate([-rng.uniform(0, 3, 20000), -10.0 ** rng.uniform(-9, 2.8, 20000),
+                        rng.uniform(-750, 710, 5000), [0.0, -0.0, -745.2, 709.8, -1e-300]])
+    y = np.zeros_like(x)
+    lib = lmx.load_library()
+    import ctypes
+    assert lib.lmx_device_exp(0, x.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                              y.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), x.size) == 0
+    ref = np.array([oracle.exp(float(v)) for v in x])
+    assert np.array_equal(y.view(np.uint64), ref.view(np.uint64))
+    a = np.concatenate([10.0 ** rng.uniform(-8, 1.5, 20000), [1e-30, 0.5, 20.0]]).astype(np.float32)
+    pr = np.zeros_like(a)
+    assert lib.lmx_device_reaction_probability(0, a.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+                                               pr.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), a.size) == 0
+    ref = np.array([oracle.reaction_probability(float(v)) for v in a], dtype=np.float32)
+    assert np.array_equal(pr.view(np.uint32), ref.view(np.uint32))
+
+
+def test_set_reaction_rate_mid_run(gpu):
+    spec = models.random_model(shape=(10, 8, 6), P=8, nS=4, nST=2, R=6, density=0.3, seed=5)
+    solver = lmx.MpdRdmeSolver(0)
+    spec.load_into(solver, seed=6)
+    orc = oracle.Oracle(spec, seed_top=6)
+    solver.runTimesteps(3)
+    assert orc.run(3) == 0
+    # change reaction 1's rate on both sides
+    new_k = float(np.float32(spec.K[1] * 3.0))
+    solver.setReactionRate(1, new_k)
+    spec2 = models.ModelSpec(**{**spec.__dict__})
+    spec2.K = spec.K.copy()
+    spec2.K[1] = new_k
+    orc2 = oracle.Oracle(spec2, seed_top=6)
+    orc2.lattice[:] = orc.lattice
+    orc2.timestep = orc.timestep
+    solver.runTimesteps(4)
+    assert orc2.run(4) == 0
+    np.testing.assert_array_equal(solver.getLatticeNative(), orc2.native())
+
+
+def test_overflow_cap_error_matches(gpu):
+    # Every site full and every particle mobile: far more than 2048 overflow
+    # entries in the first step; both sides report the too-many-overflows error.
+    spec = models.random_model(shape=(16, 16, 16), P=4, nS=2, nST=1, R=0, density=0.9, seed=3, qmax=0.5)
+    spec.DF[:] = 0.5 * spec.spacing ** 2 / spec.tau * 0.999
+    solver = lmx.MpdRdmeSolver(0)
+    spec.load_into(solver, seed=3)
+    orc = oracle.Oracle(spec, seed_top=3)
+    rc = orc.run(1)
+    assert rc == oracle.ORC_ERR_OVERFLOW_CAP
+    solver.runTimesteps(1, synchronize=False)
+    with pytest.raises(lmx.OverflowException) as ei:
+        solver.synchronize()
+    assert ei.value.code == oracle.TO_LMX[rc]
+
+
+def test_resume_from_timestep(gpu):
+    spec = models.random_model(shape=(9, 9, 9), P=8, nS=3, nST=2, R=4, density=0.2, seed=8)
+    a = lmx.MpdRdmeSolver(0)
+    spec.load_into(a, seed=4)
+    a.runTimesteps(7)
+    mid = a.getLatticeNative()
+    a.runTimesteps(5)
+    b = lmx.MpdRdmeSolver(0)
+    spec.load_into(b, seed=4)
+    b.setLatticeNative(mid)
+    b.setTimestep(7)
+    b.runTimesteps(5)
+    np.testing.assert_array_equal(a.getLatticeNative(), b.getLatticeNative())
