@@ -108,6 +108,14 @@ int lmx_set_reaction_rate(lmx_solver *s, uint32_t rxid, float rate);
 /* Select the step implementation (enum lmx_path); results do not depend on it. */
 int lmx_set_path(lmx_solver *s, int path);
 
+/* Launch-shape options (no reference counterpart: the reference fixes its
+ * block shapes at compile time, src/cmake/options.cmake:87-95).  Results do
+ * not depend on them.
+ *   LMX_OPT_MARCH_SEGMENT  rows per thread in the y/z sweeps (default 32)
+ *   LMX_OPT_FORCE_GENERIC  1 = one-site-per-thread kernels even when X%4==0 */
+enum lmx_option { LMX_OPT_MARCH_SEGMENT = 1, LMX_OPT_FORCE_GENERIC = 2 };
+int lmx_set_launch_option(lmx_solver *s, int option, int value);
+
 /* runTimestep (MpdRdmeSolver.cu:996-1193) nsteps times, enqueued on the
  * solver's stream without a host synchronization per step (the reference
  * synchronizes every step, :1091).  Device-side errors are sticky and are

@@ -26,6 +26,9 @@ ERR_NOMEM = 8
 BOUNDARY_PERIODIC = 0
 BOUNDARY_REFLECTING = 1
 
+OPT_MARCH_SEGMENT = 1
+OPT_FORCE_GENERIC = 2
+
 PATH_SWEEPS = 0
 PATH_FUSED = 1
 
@@ -53,6 +56,7 @@ SIGNATURES = {
                                           c_uint32, c_uint32, c_double, _u8p, _u8p, c_int]),
     "lmx_set_reaction_rate": (c_int, [c_void_p, c_uint32, c_float]),
     "lmx_set_path": (c_int, [c_void_p, c_int]),
+    "lmx_set_launch_option": (c_int, [c_void_p, c_int, c_int]),
     "lmx_run_steps": (c_int, [c_void_p, c_uint32]),
     "lmx_synchronize": (c_int, [c_void_p]),
     "lmx_get_timestep": (c_int, [c_void_p, _u32p]),

@@ -19,8 +19,8 @@ CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_build")
 LIB = os.path.join(OUT_DIR, "liblmx_mpd.so")
 
-SOURCES = ["mpd_kernels.hip", "mpd_capi.cpp"]
-HEADERS = ["lmx_device.h", "lmx_kernels.h"]
+SOURCES = ["mpd_kernels.hip", "mpd_fast.hip", "mpd_capi.cpp"]
+HEADERS = ["lmx_device.h", "lmx_kernels.h", "lmx_site.h"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"

@@ -46,6 +46,7 @@ struct Params {
     uint32_t nS, nST, R;
     uint32_t boundary;          // 0 periodic, 1 reflecting
     uint32_t seed32;
+    uint32_t any_qp0;           // some site type has a zero-order propensity
     const float *T;             // [nST][nST][nS]
     const float *qp0, *qp1, *qp2;
     const float *rate;          // [R]

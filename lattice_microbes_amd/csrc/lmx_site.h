@@ -1,0 +1,187 @@
+// Per-site MPD-RDME update rules shared by every kernel variant, so that all
+// launch shapes and fusions evaluate exactly the same operations.
+// Reference paths are relative to the Lattice Microbes 2.5.0 tree.
+#pragma once
+
+#include "lmx_device.h"
+
+namespace lmx {
+
+// ---------------------------------------------------------------------------
+// One site of a diffusion sweep.  The new content of site i is its own
+// stayers, then the minus neighbour's PLUS movers, then the plus neighbour's
+// MINUS movers, each in slot order (performPropagation,
+// byte_diffusion_1d_dev.cu:199-270).  A particle moves minus iff u < qm and
+// plus iff 0.5 <= u < qp + 0.5 (makeDiffusionChoices, :86-121); q <= 0.5 makes
+// the two exclusive, so a site only evaluates the decisions that can move a
+// particle into it.  wo/wm/wp are the W particle words of the site and its
+// neighbours; has_m/has_p are false at a reflecting edge.  Slots beyond P are
+// recorded as overflow entries of this site and phase.
+// ---------------------------------------------------------------------------
+template <int W>
+__device__ inline void pull_site(const Params &p, uint32_t i, uint32_t im, uint32_t ip, const uint32_t *wo,
+                                 const uint32_t *wm, const uint32_t *wp, uint32_t st, uint32_t stm, uint32_t stp,
+                                 bool has_m, bool has_p, unsigned long long h, uint32_t phase, uint32_t *out)
+{
+    SlotBuf<W> b;
+    b.clear();
+#pragma unroll
+    for (int w = 0; w < W; w++) {
+        if (wo[w] == 0) continue;
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const uint32_t part = (wo[w] >> (8 * q)) & 0xFFu;
+            if (part == 0) continue;
+            const float u = uniform01(i, p.pcb, 4 * w + q, h);
+            const float qm = has_m ? lookup_T(p, part, st, stm) : 0.0f;
+            const float qp = has_p ? lookup_T(p, part, st, stp) : 0.0f;
+            const bool leaves = (u < qm) || (u >= 0.5f && u < (qp + 0.5f));
+            if (!leaves && !b.push(part)) record_overflow(p.ctl, phase, i, b.n - 1 - 4 * W, part);
+        }
+    }
+    if (has_m) {
+#pragma unroll
+        for (int w = 0; w < W; w++) {
+            if (wm[w] == 0) continue;
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const uint32_t part = (wm[w] >> (8 * q)) & 0xFFu;
+                if (part == 0) continue;
+                const float u = uniform01(im, p.pcb, 4 * w + q, h);
+                const float qp = lookup_T(p, part, stm, st);
+                if (u >= 0.5f && u < (qp + 0.5f) && !b.push(part))
+                    record_overflow(p.ctl, phase, i, b.n - 1 - 4 * W, part);
+            }
+        }
+    }
+    if (has_p) {
+#pragma unroll
+        for (int w = 0; w < W; w++) {
+            if (wp[w] == 0) continue;
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const uint32_t part = (wp[w] >> (8 * q)) & 0xFFu;
+                if (part == 0) continue;
+                const float u = uniform01(ip, p.pcb, 4 * w + q, h);
+                const float qm = lookup_T(p, part, stp, st);
+                if (u < qm && !b.push(part)) record_overflow(p.ctl, phase, i, b.n - 1 - 4 * W, part);
+            }
+        }
+    }
+#pragma unroll
+    for (int w = 0; w < W; w++) out[w] = b.w[w];
+}
+
+// calculateReactionPropensity, global-R branch (byte_reaction_dev.cu:77-100).
+template <int W>
+__device__ inline float reaction_propensity(const Params &p, uint32_t st, const uint32_t *w, uint32_t r)
+{
+    if (!p.RL[r * p.nST + st]) return 0.0f;
+    const uint32_t d1 = p.D1[r], d2 = p.D2[r];
+    float n1 = 0.0f, n2 = 0.0f;
+#pragma unroll
+    for (int j = 0; j < 4 * W; j++) {
+        const uint32_t s = slot_of(w, j);
+        if (s > 0) {
+            n1 += (s == d1) ? 1.0f : 0.0f;
+            n2 += (s == d2) ? 1.0f : 0.0f;
+        }
+    }
+    const float k = p.rate[r];
+    switch (p.order[r]) {
+    case kZeroOrder: return k;
+    case kFirstOrder: return k * n1;
+    case kSecondOrder: return k * n1 * n2;
+    case kSecondOrderSelf: return k * n1 * (n1 - 1.0f);
+    }
+    return 0.0f;
+}
+
+// ---------------------------------------------------------------------------
+// Reaction phase at one site, in place on w[W] (precomp_reaction_kernel,
+// MpdRdmeSolver.cu:1592-1675, and byte_reaction_dev.cu:130-257).  Returns
+// true if the site's particles changed.  Needs h3 = timestep hash of substep 3.
+// ---------------------------------------------------------------------------
+template <int W>
+__device__ inline bool react_site(const Params &p, uint32_t i, uint32_t st, uint32_t *w, unsigned long long h3)
+{
+    const uint32_t nS = p.nS;
+    // Total propensity in the reference's summation order (:1617-1639).
+    float a = p.qp0[st];
+    uint32_t any = 0;
+#pragma unroll
+    for (int k = 0; k < W; k++) any |= w[k];
+    if (any != 0) {
+#pragma unroll
+        for (int m = 0; m < 4 * W; m++) {
+            const uint32_t p1 = slot_of(w, m);
+            if (p1 > 0) {
+                a += p.qp1[st * nS + (p1 - 1)];
+#pragma unroll
+                for (int n = m + 1; n < 4 * W; n++) {
+                    const uint32_t p2 = slot_of(w, n);
+                    if (p2 > 0) a += p.qp2[(st * nS + (p1 - 1)) * nS + (p2 - 1)];
+                }
+            }
+        }
+    }
+    if (a == 0.0f) return false;
+    const float prob = reaction_probability(a);
+    // checkForReaction (byte_reaction_dev.cu:139-144).
+    if (!(prob > 0.0f && uniform01(i, 1, 0, h3) <= prob)) return false;
+    // determineReactionIndex (:146-177).
+    float rp = uniform01(i, 1, 1, h3) * a;
+    uint32_t idx = 0;
+    for (uint32_t r = 0; r < p.R; r++) {
+        const float pr = reaction_propensity<W>(p, st, w, r);
+        if (pr > 0.0f) {
+            if (rp > 0.0f) idx = r;
+            rp -= pr;
+        }
+    }
+    // evaluateReaction (:179-257) without the device-heap copy of the S row:
+    // the m-th particle of species s is consumed iff fewer than -S[s] earlier
+    // particles of s were seen.
+    const int8_t *Srow = p.S + (size_t)idx * nS;
+    SlotBuf<W> out;
+    out.clear();
+#pragma unroll
+    for (int m = 0; m < 4 * W; m++) {
+        const uint32_t s = slot_of(w, m);
+        if (s == 0) continue;
+        const int sv = Srow[s - 1];
+        if (sv >= 0) {
+            out.push(s);
+        } else {
+            uint32_t earlier = 0;
+#pragma unroll
+            for (int q = 0; q < m; q++) earlier += (slot_of(w, q) == s) ? 1u : 0u;
+            if ((int)earlier >= -sv) out.push(s);
+        }
+    }
+    // Products in species order; a reactant that was not present leaves a
+    // negative residue, which the reference turns into an unbounded loop.
+    bool underflow = false;
+    for (uint32_t s = 0; s < nS; s++) {
+        const int sv = Srow[s];
+        if (sv == 0) continue;
+        if (sv < 0) {
+            uint32_t present = 0;
+#pragma unroll
+            for (int q = 0; q < 4 * W; q++) present += (slot_of(w, q) == s + 1) ? 1u : 0u;
+            if ((int)present < -sv) underflow = true;
+            continue;
+        }
+        for (int j = 0; j < sv; j++) {
+            if (!out.push(s + 1)) record_overflow(p.ctl, 3, i, out.n - 1 - 4 * W, s + 1);
+        }
+    }
+    if (underflow) {
+        if (atomicCAS(&p.ctl->error, 0u, kErrReactionUnderflow) == 0u) p.ctl->error_step = p.ctl->timestep;
+    }
+#pragma unroll
+    for (int k = 0; k < W; k++) w[k] = out.w[k];
+    return true;
+}
+
+}  // namespace lmx

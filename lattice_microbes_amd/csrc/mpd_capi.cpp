@@ -98,6 +98,10 @@ struct lmx_solver {
     uint32_t *d_counts = nullptr;
     bool tables_dirty = false;
     uint32_t timestep = 0;
+    bool any_qp0 = false;
+    // launch options
+    uint32_t march_seg = 32;
+    bool force_generic = false;
 };
 
 namespace {
@@ -148,6 +152,8 @@ void compute_propensities(lmx_solver *s)
             }
         }
     }
+    s->any_qp0 = false;
+    for (float v : s->qp0) s->any_qp0 |= (v != 0.0f);
     s->tables_dirty = true;
 }
 
@@ -173,6 +179,7 @@ lmx::Params make_params(const lmx_solver *s)
     p.nS = s->nS; p.nST = s->nST; p.R = s->R;
     p.boundary = (uint32_t)s->boundary;
     p.seed32 = s->seed32;
+    p.any_qp0 = s->any_qp0 ? 1u : 0u;
     p.T = s->d_T;
     p.qp0 = s->d_qp0; p.qp1 = s->d_qp1; p.qp2 = s->d_qp2;
     p.rate = s->d_rate;
@@ -279,14 +286,17 @@ int enqueue_step(lmx_solver *s, const lmx::Params &p, hipEvent_t *ev, int *kinds
         e = hipEventRecord(ev[0], s->stream);
         if (e != hipSuccess) return hip_fail(e, "hipEventRecord");
     }
+    const bool fast = !s->force_generic && lmx::fast_path_ok(p);
     for (int axis = 0; axis < 3; axis++) {
-        e = lmx::launch_sweep(p, s->d_lat[s->cur], s->d_lat[1 - s->cur], axis, s->stream);
+        e = fast ? lmx::launch_sweep_fast(p, s->d_lat[s->cur], s->d_lat[1 - s->cur], axis, s->march_seg, s->stream)
+                 : lmx::launch_sweep(p, s->d_lat[s->cur], s->d_lat[1 - s->cur], axis, s->stream);
         if (e != hipSuccess) return hip_fail(e, "sweep launch");
         s->cur = 1 - s->cur;
         if (int rc = mark(LMX_K_SWEEP_X + axis)) return rc;
     }
     if (s->R > 0) {
-        e = lmx::launch_reaction(p, s->d_lat[s->cur], s->stream);
+        e = fast ? lmx::launch_reaction_fast(p, s->d_lat[s->cur], s->stream)
+                 : lmx::launch_reaction(p, s->d_lat[s->cur], s->stream);
         if (e != hipSuccess) return hip_fail(e, "reaction launch");
         if (int rc = mark(LMX_K_REACTION)) return rc;
     }
@@ -544,6 +554,21 @@ int lmx_set_path(lmx_solver *s, int path)
     if (path != LMX_PATH_SWEEPS) return fail(LMX_ERR_ARG, "unsupported step path");
     s->path = path;
     return LMX_OK;
+}
+
+int lmx_set_launch_option(lmx_solver *s, int option, int value)
+{
+    if (!s) return fail(LMX_ERR_ARG, "null solver");
+    switch (option) {
+    case LMX_OPT_MARCH_SEGMENT:
+        if (value < 1 || value > 65536) return fail(LMX_ERR_ARG, "march segment must be in 1..65536");
+        s->march_seg = (uint32_t)value;
+        return LMX_OK;
+    case LMX_OPT_FORCE_GENERIC:
+        s->force_generic = value != 0;
+        return LMX_OK;
+    }
+    return fail(LMX_ERR_ARG, "unknown launch option");
 }
 
 int lmx_run_steps(lmx_solver *s, uint32_t nsteps)

@@ -1,10 +1,9 @@
 // MPD-RDME step kernels for gfx950 (CDNA4, wave64).
 //
-//  sweep_kernel     one diffusion sweep along x, y or z (mpd_{x,y,z}_kernel,
-//                   MpdRdmeSolver.cu:1342-1490 + byte_diffusion_1d_dev.cu)
-//  reaction_kernel  per-site reaction from the precomputed propensity tables
-//                   (precomp_reaction_kernel, MpdRdmeSolver.cu:1592-1675 +
-//                   byte_reaction_dev.cu:62-257)
+//  sweep_kernel     one diffusion sweep along x, y or z, one site per thread,
+//                   any shape (mpd_{x,y,z}_kernel, MpdRdmeSolver.cu:1342-1490)
+//  reaction_kernel  per-site reaction, one site per thread, any shape
+//                   (precomp_reaction_kernel, MpdRdmeSolver.cu:1592-1675)
 //  resolve_kernel   canonical-order overflow placement on the device (the
 //                   classic host handler, MpdRdmeSolver.cu:1103-1189)
 //  count_kernel     species counts (ByteLattice::getParticleCounts)
@@ -14,16 +13,13 @@
 // how sites are distributed over threads.
 #include "lmx_device.h"
 #include "lmx_kernels.h"
+#include "lmx_site.h"
 
 namespace lmx {
 
 // ---------------------------------------------------------------------------
-// Diffusion sweep.  Site i's new content is its own stayers, then the minus
-// neighbour's PLUS movers, then the plus neighbour's MINUS movers, each in
-// slot order (performPropagation, byte_diffusion_1d_dev.cu:199-270).  A
-// particle moves minus iff u < qm and plus iff 0.5 <= u < qp + 0.5
-// (makeDiffusionChoices, :86-121); q <= 0.5 makes the two exclusive, so a
-// site only evaluates the decisions that can move a particle into it.
+// Generic sweep: one site per thread, any lattice shape (the vector kernels in
+// mpd_fast.hip need X % 4 == 0).
 // ---------------------------------------------------------------------------
 template <int W>
 __global__ __launch_bounds__(256) void sweep_kernel(Params p, const uint32_t *__restrict__ src,
@@ -39,7 +35,7 @@ __global__ __launch_bounds__(256) void sweep_kernel(Params p, const uint32_t *__
     const uint32_t im = (c > 0) ? i - stride : i + (len - 1) * stride;
     const uint32_t ip = (c + 1 < len) ? i + stride : i - (len - 1) * stride;
 
-    uint32_t wi[W], wm[W], wp[W];
+    uint32_t wi[W], wm[W], wp[W], out[W];
     uint32_t any = 0;
 #pragma unroll
     for (int k = 0; k < W; k++) {
@@ -56,73 +52,12 @@ __global__ __launch_bounds__(256) void sweep_kernel(Params p, const uint32_t *__
     const uint32_t sti = p.sites[i];
     const uint32_t stm = has_m ? p.sites[im] : 0u;
     const uint32_t stp = has_p ? p.sites[ip] : 0u;
-
-    SlotBuf<W> out;
-    out.clear();
+    pull_site<W>(p, i, im, ip, wi, wm, wp, sti, stm, stp, has_m, has_p, h, substep, out);
 #pragma unroll
-    for (int j = 0; j < 4 * W; j++) {
-        const uint32_t part = slot_of(wi, j);
-        if (part == 0) continue;
-        const float u = uniform01(i, p.pcb, j, h);
-        const float qm = has_m ? lookup_T(p, part, sti, stm) : 0.0f;
-        const float qp = has_p ? lookup_T(p, part, sti, stp) : 0.0f;
-        const bool leaves = (u < qm) || (u >= 0.5f && u < (qp + 0.5f));
-        if (!leaves && !out.push(part)) record_overflow(p.ctl, substep, i, out.n - 1 - 4 * W, part);
-    }
-    if (has_m) {
-#pragma unroll
-        for (int j = 0; j < 4 * W; j++) {
-            const uint32_t part = slot_of(wm, j);
-            if (part == 0) continue;
-            const float u = uniform01(im, p.pcb, j, h);
-            const float qp = lookup_T(p, part, stm, sti);
-            if (u >= 0.5f && u < (qp + 0.5f) && !out.push(part))
-                record_overflow(p.ctl, substep, i, out.n - 1 - 4 * W, part);
-        }
-    }
-    if (has_p) {
-#pragma unroll
-        for (int j = 0; j < 4 * W; j++) {
-            const uint32_t part = slot_of(wp, j);
-            if (part == 0) continue;
-            const float u = uniform01(ip, p.pcb, j, h);
-            const float qm = lookup_T(p, part, stp, sti);
-            if (u < qm && !out.push(part)) record_overflow(p.ctl, substep, i, out.n - 1 - 4 * W, part);
-        }
-    }
-#pragma unroll
-    for (int k = 0; k < W; k++) dst[(size_t)k * p.N + i] = out.w[k];
+    for (int k = 0; k < W; k++) dst[(size_t)k * p.N + i] = out[k];
 }
 
-// ---------------------------------------------------------------------------
-// Reaction phase, in place.
-// ---------------------------------------------------------------------------
-
-// calculateReactionPropensity, global-R branch (byte_reaction_dev.cu:77-100).
-template <int W>
-__device__ inline float reaction_propensity(const Params &p, uint32_t st, const uint32_t *w, uint32_t r)
-{
-    if (!p.RL[r * p.nST + st]) return 0.0f;
-    const uint32_t d1 = p.D1[r], d2 = p.D2[r];
-    float n1 = 0.0f, n2 = 0.0f;
-#pragma unroll
-    for (int j = 0; j < 4 * W; j++) {
-        const uint32_t s = slot_of(w, j);
-        if (s > 0) {
-            n1 += (s == d1) ? 1.0f : 0.0f;
-            n2 += (s == d2) ? 1.0f : 0.0f;
-        }
-    }
-    const float k = p.rate[r];
-    switch (p.order[r]) {
-    case kZeroOrder: return k;
-    case kFirstOrder: return k * n1;
-    case kSecondOrder: return k * n1 * n2;
-    case kSecondOrderSelf: return k * n1 * (n1 - 1.0f);
-    }
-    return 0.0f;
-}
-
+// Generic reaction phase: one site per thread, in place.
 template <int W>
 __global__ __launch_bounds__(256) void reaction_kernel(Params p, uint32_t *__restrict__ lat)
 {
@@ -130,87 +65,13 @@ __global__ __launch_bounds__(256) void reaction_kernel(Params p, uint32_t *__res
     if (i >= p.N) return;
     const uint32_t st = p.sites[i];
     uint32_t w[W];
-    uint32_t any = 0;
 #pragma unroll
-    for (int k = 0; k < W; k++) {
-        w[k] = lat[(size_t)k * p.N + i];
-        any |= w[k];
-    }
-    const uint32_t nS = p.nS;
-    // Total propensity in the reference's summation order
-    // (MpdRdmeSolver.cu:1617-1639).
-    float a = p.qp0[st];
-    if (any != 0) {
-#pragma unroll
-        for (int m = 0; m < 4 * W; m++) {
-            const uint32_t p1 = slot_of(w, m);
-            if (p1 > 0) {
-                a += p.qp1[st * nS + (p1 - 1)];
-#pragma unroll
-                for (int n = m + 1; n < 4 * W; n++) {
-                    const uint32_t p2 = slot_of(w, n);
-                    if (p2 > 0) a += p.qp2[(st * nS + (p1 - 1)) * nS + (p2 - 1)];
-                }
-            }
-        }
-    }
-    if (a == 0.0f) return;
+    for (int k = 0; k < W; k++) w[k] = lat[(size_t)k * p.N + i];
     const unsigned long long h = timestep_hash(p.seed32, p.ctl->timestep, 3);
-    const float prob = reaction_probability(a);
-    // checkForReaction (byte_reaction_dev.cu:139-144).
-    if (!(prob > 0.0f && uniform01(i, 1, 0, h) <= prob)) return;
-    // determineReactionIndex (:146-177).
-    float rp = uniform01(i, 1, 1, h) * a;
-    uint32_t idx = 0;
-    for (uint32_t r = 0; r < p.R; r++) {
-        const float pr = reaction_propensity<W>(p, st, w, r);
-        if (pr > 0.0f) {
-            if (rp > 0.0f) idx = r;
-            rp -= pr;
-        }
-    }
-    // evaluateReaction (:179-257) without the device-heap copy of the S row:
-    // the m-th particle of species s is consumed iff fewer than -S[s] earlier
-    // particles of s were seen.
-    const int8_t *Srow = p.S + (size_t)idx * nS;
-    SlotBuf<W> out;
-    out.clear();
+    if (react_site<W>(p, i, st, w, h)) {
 #pragma unroll
-    for (int m = 0; m < 4 * W; m++) {
-        const uint32_t s = slot_of(w, m);
-        if (s == 0) continue;
-        const int sv = Srow[s - 1];
-        if (sv >= 0) {
-            out.push(s);
-        } else {
-            uint32_t earlier = 0;
-#pragma unroll
-            for (int q = 0; q < m; q++) earlier += (slot_of(w, q) == s) ? 1u : 0u;
-            if ((int)earlier >= -sv) out.push(s);
-        }
+        for (int k = 0; k < W; k++) lat[(size_t)k * p.N + i] = w[k];
     }
-    // Products in species order; a reactant that was not present leaves a
-    // negative residue, which the reference turns into an unbounded loop.
-    bool underflow = false;
-    for (uint32_t s = 0; s < nS; s++) {
-        int sv = Srow[s];
-        if (sv == 0) continue;
-        if (sv < 0) {
-            uint32_t present = 0;
-#pragma unroll
-            for (int q = 0; q < 4 * W; q++) present += (slot_of(w, q) == s + 1) ? 1u : 0u;
-            if ((int)present < -sv) underflow = true;
-            continue;
-        }
-        for (int j = 0; j < sv; j++) {
-            if (!out.push(s + 1)) record_overflow(p.ctl, 3, i, out.n - 1 - 4 * W, s + 1);
-        }
-    }
-    if (underflow) {
-        if (atomicCAS(&p.ctl->error, 0u, kErrReactionUnderflow) == 0u) p.ctl->error_step = p.ctl->timestep;
-    }
-#pragma unroll
-    for (int k = 0; k < W; k++) lat[(size_t)k * p.N + i] = out.w[k];
 }
 
 // ---------------------------------------------------------------------------

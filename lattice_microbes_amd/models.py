@@ -197,17 +197,25 @@ def bimolecular(shape=(64, 64, 64), spacing=32e-9, tau=50e-6, P=8, nA=1000, nB=1
                      ["A", "B", "C"], ["cytoplasm"])
 
 
+def _grid(shape):
+    X, Y, Z = shape
+    z = np.arange(Z, dtype=np.float32)[:, None, None]
+    y = np.arange(Y, dtype=np.float32)[None, :, None]
+    x = np.arange(X, dtype=np.float32)[None, None, :]
+    return x, y, z
+
+
 def _capsule_types(shape, radius_sites, half_length_sites, membrane_sites=1.0):
     """Spherocylinder along z centred in the box: 0 outside, 1 membrane shell,
     2 cytoplasm.  Returns native (Z, Y, X) uint8."""
     X, Y, Z = shape
-    z, y, x = np.meshgrid(np.arange(Z), np.arange(Y), np.arange(X), indexing="ij")
+    x, y, z = _grid(shape)
     cx, cy, cz = (X - 1) / 2.0, (Y - 1) / 2.0, (Z - 1) / 2.0
     dzc = np.clip(z - cz, -half_length_sites, half_length_sites)
-    d = np.sqrt((x - cx) ** 2 + (y - cy) ** 2 + (z - cz - dzc) ** 2)
+    d2 = (x - cx) ** 2 + (y - cy) ** 2 + (z - cz - dzc) ** 2
     t = np.zeros((Z, Y, X), dtype=np.uint8)
-    t[d <= radius_sites] = 1
-    t[d <= radius_sites - membrane_sites] = 2
+    t[d2 <= np.float32(radius_sites ** 2)] = 1
+    t[d2 <= np.float32(max(radius_sites - membrane_sites, 0.0) ** 2)] = 2
     return t
 
 
@@ -268,7 +276,7 @@ def whole_cell(shape=(256, 256, 512), spacing=10e-9, tau=4e-6, P=8, nS=100, R=30
     radius = 0.47 * min(X, Y)
     half_len = max(Z / 2.0 - radius - 2.0, 0.0)
     t = _capsule_types(shape, radius, half_len)  # 0 ext, 1 membrane, 2 cytoplasm
-    z, y, x = np.meshgrid(np.arange(Z), np.arange(Y), np.arange(X), indexing="ij")
+    x, y, z = _grid(shape)
     cx, cy, cz = (X - 1) / 2.0, (Y - 1) / 2.0, (Z - 1) / 2.0
     # nucleoid: ellipsoid around the centre
     nuc = ((x - cx) / (0.45 * radius)) ** 2 + ((y - cy) / (0.45 * radius)) ** 2 + \
@@ -282,7 +290,7 @@ def whole_cell(shape=(256, 256, 512), spacing=10e-9, tau=4e-6, P=8, nS=100, R=30
         xc, yc = cx + 0.6 * radius * np.cos(ang), cy + 0.6 * radius * np.sin(ang)
         ball = (x - xc) ** 2 + (y - yc) ** 2 + (z - zc) ** 2 <= rr * rr
         t[(t == 2) & ball] = 5 + k
-    del x, y, z, nuc
+    del nuc
     cyt = np.flatnonzero(t.reshape(-1) == 2)
     nobs = int(obstacle_fraction * cyt.size)
     obs = rng.choice(cyt, size=nobs, replace=False)

@@ -159,6 +159,10 @@ class MpdRdmeSolver:
     def setPath(self, path: int) -> None:
         _lib.check(self._lib.lmx_set_path(self._h, int(path)))
 
+    def setLaunchOption(self, option: int, value: int) -> None:
+        """Launch-shape options (lmx_option); results do not depend on them."""
+        _lib.check(self._lib.lmx_set_launch_option(self._h, int(option), int(value)))
+
     # ------------------------------------------------------------------ stepping
     def runTimesteps(self, nsteps: int, synchronize: bool = True) -> None:
         """runTimestep (MpdRdmeSolver.cu:996-1193) nsteps times."""

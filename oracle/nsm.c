@@ -137,6 +137,7 @@ struct orc_nsm {
     uint32_t *ndep;  /* per reaction: number of changed species */
     uint32_t *dep;   /* [R][nS] changed species */
     int32_t *dchg;   /* [R][nS] change */
+    int has_zero_order; /* an empty subvolume has nonzero propensity only then */
     heap_t heap;
     xs_rng rng;
     double expv[RNG_CACHE], univ[RNG_CACHE];
@@ -183,7 +184,9 @@ static double reaction_prop(const orc_nsm *s, uint32_t r)
 
 static double subvolume_propensity(orc_nsm *s, size_t sv)
 {
-    /* calculateSubvolumePropensity, NextSubvolumeSolver.cpp:421-452. */
+    /* calculateSubvolumePropensity, NextSubvolumeSolver.cpp:421-452.  An
+     * empty subvolume without zero-order reactions sums only zeros: 0.0. */
+    if (!s->has_zero_order && s->slots[sv * s->P] == 0) return 0.0;
     count_subvolume(s, sv);
     uint32_t src = s->sites[sv];
     double a = 0.0;
@@ -271,6 +274,7 @@ orc_nsm *orc_nsm_create(uint32_t X, uint32_t Y, uint32_t Z, uint32_t P, uint32_t
     memcpy(s->RL, RL, sizeof(uint32_t) * R * nST);
     for (uint32_t r = 0; r < R; r++) {
         s->k[r] = k[r];
+        if (types[r] == 0) s->has_zero_order = 1;
         if (types[r] == 2 || types[r] == 3) s->k[r] *= (double)s->N;
         uint32_t n = 0;
         for (uint32_t j = 0; j < nS; j++) {

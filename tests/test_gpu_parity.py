@@ -166,3 +166,35 @@ def test_resume_from_timestep(gpu):
     b.setTimestep(7)
     b.runTimesteps(5)
     np.testing.assert_array_equal(a.getLatticeNative(), b.getLatticeNative())
+
+
+ALIGNED = [
+    ((8, 6, 5), 8, "periodic", 0.3),
+    ((12, 7, 9), 8, "reflecting", 0.3),
+    ((4, 4, 4), 8, "periodic", 0.2),
+    ((16, 1, 3), 16, "periodic", 0.25),
+    ((32, 12, 20), 4, "reflecting", 0.3),
+    ((64, 16, 16), 8, "periodic", 0.05),
+]
+
+
+@pytest.mark.parametrize("case", ALIGNED, ids=[f"{c[0]}-P{c[1]}-{c[2]}" for c in ALIGNED])
+def test_vector_kernels_match_generic_and_oracle(gpu, case):
+    """X % 4 == 0 selects the 4-sites-per-thread kernels; they must agree with
+    the one-site-per-thread kernels and the oracle for every march segment."""
+    shape, P, boundary, density = case
+    spec = models.random_model(shape=shape, P=P, nS=5, nST=3, R=7, density=density, seed=17, boundary=boundary)
+    orc = oracle.Oracle(spec, seed_top=5, replicate=2)
+    assert orc.run(6) == 0
+    ref = orc.native()
+    for opt in ({"generic": 1}, {"seg": 1}, {"seg": 3}, {"seg": 7}, {"seg": 32}, {"seg": 100000}):
+        s = lmx.MpdRdmeSolver(0)
+        spec.load_into(s, replicate=2, seed=5)
+        if "generic" in opt:
+            s.setLaunchOption(lmx._lib.OPT_FORCE_GENERIC, 1)
+        else:
+            s.setLaunchOption(lmx._lib.OPT_MARCH_SEGMENT, min(opt["seg"], 65536))
+        s.runTimesteps(6)
+        got = s.getLatticeNative()
+        assert np.array_equal(got, ref), f"{opt}: {(got != ref).sum()} words differ"
+        s.close()
