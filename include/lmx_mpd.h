@@ -111,9 +111,11 @@ int lmx_set_path(lmx_solver *s, int path);
 /* Launch-shape options (no reference counterpart: the reference fixes its
  * block shapes at compile time, src/cmake/options.cmake:87-95).  Results do
  * not depend on them.
- *   LMX_OPT_MARCH_SEGMENT  rows per thread in the y/z sweeps (default 32)
- *   LMX_OPT_FORCE_GENERIC  1 = one-site-per-thread kernels even when X%4==0 */
-enum lmx_option { LMX_OPT_MARCH_SEGMENT = 1, LMX_OPT_FORCE_GENERIC = 2 };
+ *   LMX_OPT_MARCH_SEGMENT  rows per thread of the marching y/z sweep (default 8)
+ *   LMX_OPT_FORCE_GENERIC  1 = one-site-per-thread kernels even when X%4==0
+ *   LMX_OPT_YZ_KERNEL      y/z sweep kernel: 0 = LDS row tiles (default),
+ *                          1 = register march */
+enum lmx_option { LMX_OPT_MARCH_SEGMENT = 1, LMX_OPT_FORCE_GENERIC = 2, LMX_OPT_YZ_KERNEL = 3 };
 int lmx_set_launch_option(lmx_solver *s, int option, int value);
 
 /* runTimestep (MpdRdmeSolver.cu:996-1193) nsteps times, enqueued on the

@@ -28,6 +28,7 @@ BOUNDARY_REFLECTING = 1
 
 OPT_MARCH_SEGMENT = 1
 OPT_FORCE_GENERIC = 2
+OPT_YZ_KERNEL = 3
 
 PATH_SWEEPS = 0
 PATH_FUSED = 1

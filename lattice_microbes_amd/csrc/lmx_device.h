@@ -50,8 +50,18 @@ struct Params {
     const float *T;             // [nST][nST][nS]
     const float *qp0, *qp1, *qp2;
     const float *rate;          // [R]
+    const float *prob1;         // [nST][nS] float2 (total propensity, probability) of a one-particle site
+    const float *prob0;         // [nST] probability of an empty site (zero-order only)
     const uint32_t *order, *D1, *D2;
     const int8_t *S;            // [R][nS]
+    // Reactions by first reactant: rx_list[rx_off[s]..rx_off[s+1]) lists, in
+    // ascending order, the reactions whose first reactant is species s; list
+    // nS holds the zero-order reactions.
+    const uint32_t *rx_off, *rx_list;
+    // Nonzero stoichiometry of reaction r, ascending species:
+    // (sto_sp[k], sto_val[k]) for k in [sto_off[r], sto_off[r+1]).
+    const uint32_t *sto_off, *sto_sp;
+    const int32_t *sto_val;
     const uint8_t *RL;          // [R][nST]
     const uint8_t *sites;       // [N]
     Control *ctl;

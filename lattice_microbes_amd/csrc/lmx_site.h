@@ -102,11 +102,11 @@ __device__ inline float reaction_propensity(const Params &p, uint32_t st, const 
 // MpdRdmeSolver.cu:1592-1675, and byte_reaction_dev.cu:130-257).  Returns
 // true if the site's particles changed.  Needs h3 = timestep hash of substep 3.
 // ---------------------------------------------------------------------------
+// Total propensity in the reference's summation order (:1617-1639).
 template <int W>
-__device__ inline bool react_site(const Params &p, uint32_t i, uint32_t st, uint32_t *w, unsigned long long h3)
+__device__ inline float total_propensity(const Params &p, uint32_t st, const uint32_t *w)
 {
     const uint32_t nS = p.nS;
-    // Total propensity in the reference's summation order (:1617-1639).
     float a = p.qp0[st];
     uint32_t any = 0;
 #pragma unroll
@@ -125,31 +125,125 @@ __device__ inline bool react_site(const Params &p, uint32_t i, uint32_t st, uint
             }
         }
     }
+    return a;
+}
+
+template <int W>
+__device__ inline bool react_apply(const Params &p, uint32_t i, uint32_t st, uint32_t *w, float a, float prob,
+                                   unsigned long long h3);
+
+template <int W>
+__device__ inline bool react_site(const Params &p, uint32_t i, uint32_t st, uint32_t *w, unsigned long long h3)
+{
+    const float a = total_propensity<W>(p, st, w);
     if (a == 0.0f) return false;
-    const float prob = reaction_probability(a);
+    return react_apply<W>(p, i, st, w, a, reaction_probability(a), h3);
+}
+
+// Same result as react_site.  A site holding at most one particle takes its
+// total propensity as the same float sum and its probability from the table
+// precomputed on the device with the same exp (prob1[st][species],
+// prob0[st]), so the double-precision exp runs only for multi-particle sites.
+template <int W>
+__device__ inline bool react_site_table(const Params &p, uint32_t i, uint32_t st, uint32_t *w,
+                                        unsigned long long h3, const float2 *tab)
+{
+    uint32_t n = 0, first = 0;
+#pragma unroll
+    for (int j = 4 * W - 1; j >= 0; j--) {
+        const uint32_t s = slot_of(w, j);
+        if (s) {
+            n++;
+            first = s;
+        }
+    }
+    if (n > 1) return react_site<W>(p, i, st, w, h3);
+    float a, prob;
+    if (n == 1) {
+        // tab (prob1, or its copy in LDS) holds (qp0[st] + qp1[st][s], its
+        // probability) as one float2.
+        const float2 ap = tab[st * p.nS + (first - 1)];
+        a = ap.x;
+        prob = ap.y;
+    } else {
+        a = p.qp0[st];
+        prob = p.prob0[st];
+    }
+    if (a == 0.0f) return false;
+    return react_apply<W>(p, i, st, w, a, prob, h3);
+}
+
+// checkForReaction, determineReactionIndex and evaluateReaction
+// (byte_reaction_dev.cu:139-257) for a site with total propensity a and
+// reaction probability prob.
+template <int W>
+__device__ inline bool react_apply(const Params &p, uint32_t i, uint32_t st, uint32_t *w, float a, float prob,
+                                   unsigned long long h3)
+{
+    const uint32_t nS = p.nS;
     // checkForReaction (byte_reaction_dev.cu:139-144).
     if (!(prob > 0.0f && uniform01(i, 1, 0, h3) <= prob)) return false;
-    // determineReactionIndex (:146-177).
+    // determineReactionIndex (:146-177): reactions in ascending index, each
+    // with a positive propensity moving rp.  A reaction whose first reactant
+    // is absent has propensity 0 (or -0) and changes nothing, so only the
+    // reactions listed under a species present at the site, plus the
+    // zero-order ones, are visited: a merge of those ascending lists.
+    constexpr int P = 4 * W;
+    constexpr uint32_t kNone = 0xFFFFFFFFu;
     float rp = uniform01(i, 1, 1, h3) * a;
     uint32_t idx = 0;
-    for (uint32_t r = 0; r < p.R; r++) {
-        const float pr = reaction_propensity<W>(p, st, w, r);
-        if (pr > 0.0f) {
-            if (rp > 0.0f) idx = r;
-            rp -= pr;
+    {
+        uint32_t cur[P + 1], end[P + 1], head[P + 1];
+#pragma unroll
+        for (int j = 0; j < P; j++) {
+            const uint32_t s = slot_of(w, j);
+            bool first = s != 0;
+#pragma unroll
+            for (int q = 0; q < j; q++) first = first && (slot_of(w, q) != s);
+            const uint32_t sl = first ? s - 1 : 0u;
+            cur[j] = first ? p.rx_off[sl] : 0u;
+            end[j] = first ? p.rx_off[sl + 1] : 0u;
+        }
+        cur[P] = p.rx_off[nS];
+        end[P] = p.rx_off[nS + 1];
+#pragma unroll
+        for (int j = 0; j <= P; j++) head[j] = cur[j] < end[j] ? p.rx_list[cur[j]] : kNone;
+        for (;;) {
+            uint32_t best = kNone;
+            int bj = -1;
+#pragma unroll
+            for (int j = 0; j <= P; j++)
+                if (head[j] < best) {
+                    best = head[j];
+                    bj = j;
+                }
+            if (best == kNone) break;
+#pragma unroll
+            for (int j = 0; j <= P; j++)
+                if (j == bj) {
+                    cur[j]++;
+                    head[j] = cur[j] < end[j] ? p.rx_list[cur[j]] : kNone;
+                }
+            const float pr = reaction_propensity<W>(p, st, w, best);
+            if (pr > 0.0f) {
+                if (rp > 0.0f) idx = best;
+                rp -= pr;
+            }
         }
     }
     // evaluateReaction (:179-257) without the device-heap copy of the S row:
     // the m-th particle of species s is consumed iff fewer than -S[s] earlier
-    // particles of s were seen.
-    const int8_t *Srow = p.S + (size_t)idx * nS;
+    // particles of s were seen.  S[idx] comes from its nonzero entries.
+    const uint32_t k0 = p.sto_off[idx], k1 = p.sto_off[idx + 1];
     SlotBuf<W> out;
     out.clear();
 #pragma unroll
-    for (int m = 0; m < 4 * W; m++) {
+    for (int m = 0; m < P; m++) {
         const uint32_t s = slot_of(w, m);
         if (s == 0) continue;
-        const int sv = Srow[s - 1];
+        int sv = 0;
+        for (uint32_t k = k0; k < k1; k++)
+            if (p.sto_sp[k] == s - 1) sv = p.sto_val[k];
         if (sv >= 0) {
             out.push(s);
         } else {
@@ -162,18 +256,18 @@ __device__ inline bool react_site(const Params &p, uint32_t i, uint32_t st, uint
     // Products in species order; a reactant that was not present leaves a
     // negative residue, which the reference turns into an unbounded loop.
     bool underflow = false;
-    for (uint32_t s = 0; s < nS; s++) {
-        const int sv = Srow[s];
-        if (sv == 0) continue;
+    for (uint32_t k = k0; k < k1; k++) {
+        const uint32_t s = p.sto_sp[k];
+        const int sv = p.sto_val[k];
         if (sv < 0) {
             uint32_t present = 0;
 #pragma unroll
-            for (int q = 0; q < 4 * W; q++) present += (slot_of(w, q) == s + 1) ? 1u : 0u;
+            for (int q = 0; q < P; q++) present += (slot_of(w, q) == s + 1) ? 1u : 0u;
             if ((int)present < -sv) underflow = true;
             continue;
         }
         for (int j = 0; j < sv; j++) {
-            if (!out.push(s + 1)) record_overflow(p.ctl, 3, i, out.n - 1 - 4 * W, s + 1);
+            if (!out.push(s + 1)) record_overflow(p.ctl, 3, i, out.n - 1 - P, s + 1);
         }
     }
     if (underflow) {

@@ -91,17 +91,21 @@ struct lmx_solver {
     int cur = 0;
     uint8_t *d_sites = nullptr;
     float *d_T = nullptr, *d_qp0 = nullptr, *d_qp1 = nullptr, *d_qp2 = nullptr, *d_rate = nullptr;
+    float *d_prob1 = nullptr, *d_prob0 = nullptr;
     uint32_t *d_order = nullptr, *d_D1 = nullptr, *d_D2 = nullptr;
     int8_t *d_S = nullptr;
     uint8_t *d_RL = nullptr;
+    uint32_t *d_rx_off = nullptr, *d_rx_list = nullptr, *d_sto_off = nullptr, *d_sto_sp = nullptr;
+    int32_t *d_sto_val = nullptr;
     lmx::Control *d_ctl = nullptr;
     uint32_t *d_counts = nullptr;
     bool tables_dirty = false;
     uint32_t timestep = 0;
     bool any_qp0 = false;
     // launch options
-    uint32_t march_seg = 32;
+    uint32_t march_seg = 8;
     bool force_generic = false;
+    int yz_kernel = 0;  // 0: LDS tiles, 1: register march
 };
 
 namespace {
@@ -157,6 +161,8 @@ void compute_propensities(lmx_solver *s)
     s->tables_dirty = true;
 }
 
+lmx::Params make_params(const lmx_solver *s);
+
 int upload_tables(lmx_solver *s)
 {
     int rc;
@@ -164,6 +170,15 @@ int upload_tables(lmx_solver *s)
     if ((rc = dev_upload(&s->d_qp0, s->qp0))) return rc;
     if ((rc = dev_upload(&s->d_qp1, s->qp1))) return rc;
     if ((rc = dev_upload(&s->d_qp2, s->qp2))) return rc;
+    // Single-particle reaction probabilities, built on the device with the
+    // kernels' own exp.
+    std::vector<float> zeros1((size_t)s->nST * s->nS * 2, 0.0f), zeros0(s->nST, 0.0f);
+    if ((rc = dev_upload(&s->d_prob1, zeros1))) return rc;
+    if ((rc = dev_upload(&s->d_prob0, zeros0))) return rc;
+    const lmx::Params p = make_params(s);
+    hipError_t e = lmx::launch_prob_tables(p, s->d_prob1, s->d_prob0, s->stream);
+    if (e != hipSuccess) return hip_fail(e, "probability table launch");
+    LMX_HIP(hipStreamSynchronize(s->stream));
     s->tables_dirty = false;
     return LMX_OK;
 }
@@ -183,9 +198,16 @@ lmx::Params make_params(const lmx_solver *s)
     p.T = s->d_T;
     p.qp0 = s->d_qp0; p.qp1 = s->d_qp1; p.qp2 = s->d_qp2;
     p.rate = s->d_rate;
+    p.prob1 = s->d_prob1;
+    p.prob0 = s->d_prob0;
     p.order = s->d_order; p.D1 = s->d_D1; p.D2 = s->d_D2;
     p.S = s->d_S;
     p.RL = s->d_RL;
+    p.rx_off = s->d_rx_off;
+    p.rx_list = s->d_rx_list;
+    p.sto_off = s->d_sto_off;
+    p.sto_sp = s->d_sto_sp;
+    p.sto_val = s->d_sto_val;
     p.sites = s->d_sites;
     p.ctl = s->d_ctl;
     return p;
@@ -288,8 +310,12 @@ int enqueue_step(lmx_solver *s, const lmx::Params &p, hipEvent_t *ev, int *kinds
     }
     const bool fast = !s->force_generic && lmx::fast_path_ok(p);
     for (int axis = 0; axis < 3; axis++) {
-        e = fast ? lmx::launch_sweep_fast(p, s->d_lat[s->cur], s->d_lat[1 - s->cur], axis, s->march_seg, s->stream)
-                 : lmx::launch_sweep(p, s->d_lat[s->cur], s->d_lat[1 - s->cur], axis, s->stream);
+        if (!fast)
+            e = lmx::launch_sweep(p, s->d_lat[s->cur], s->d_lat[1 - s->cur], axis, s->stream);
+        else if (axis > 0 && s->yz_kernel == 0)
+            e = lmx::launch_sweep_tile(p, s->d_lat[s->cur], s->d_lat[1 - s->cur], axis, s->stream);
+        else
+            e = lmx::launch_sweep_fast(p, s->d_lat[s->cur], s->d_lat[1 - s->cur], axis, s->march_seg, s->stream);
         if (e != hipSuccess) return hip_fail(e, "sweep launch");
         s->cur = 1 - s->cur;
         if (int rc = mark(LMX_K_SWEEP_X + axis)) return rc;
@@ -318,11 +344,18 @@ void free_device(lmx_solver *s)
     dev_free(s->d_qp1);
     dev_free(s->d_qp2);
     dev_free(s->d_rate);
+    dev_free(s->d_prob1);
+    dev_free(s->d_prob0);
     dev_free(s->d_order);
     dev_free(s->d_D1);
     dev_free(s->d_D2);
     dev_free(s->d_S);
     dev_free(s->d_RL);
+    dev_free(s->d_rx_off);
+    dev_free(s->d_rx_list);
+    dev_free(s->d_sto_off);
+    dev_free(s->d_sto_sp);
+    dev_free(s->d_sto_val);
 }
 
 }  // namespace
@@ -532,6 +565,32 @@ int lmx_build_diffusion_model(lmx_solver *s, uint32_t nST, const double *DF, con
     std::vector<uint8_t> rl8(s->RL.size());
     for (size_t i = 0; i < rl8.size(); i++) rl8[i] = (uint8_t)s->RL[i];  // tmpRL (MpdRdmeSolver.cu:424-428)
     if ((rc = dev_upload(&s->d_RL, rl8))) return rc;
+    // Reactions listed by first reactant (zero-order ones under nS), and the
+    // nonzero stoichiometry of each reaction, both in ascending order.
+    {
+        std::vector<uint32_t> rx_off(s->nS + 2, 0), rx_list(s->R), sto_off(s->R + 1, 0), sto_sp;
+        std::vector<int32_t> sto_val;
+        auto bucket = [&](uint32_t r) { return s->order[r] == lmx::kZeroOrder ? s->nS : s->D1[r] - 1; };
+        for (uint32_t r = 0; r < s->R; r++) rx_off[bucket(r) + 1]++;
+        for (uint32_t b = 0; b <= s->nS; b++) rx_off[b + 1] += rx_off[b];
+        std::vector<uint32_t> fill(rx_off.begin(), rx_off.end() - 1);
+        for (uint32_t r = 0; r < s->R; r++) rx_list[fill[bucket(r)]++] = r;
+        for (uint32_t r = 0; r < s->R; r++) {
+            for (uint32_t sp = 0; sp < s->nS; sp++) {
+                const int8_t v = s->S_rm[(size_t)r * s->nS + sp];
+                if (v != 0) {
+                    sto_sp.push_back(sp);
+                    sto_val.push_back(v);
+                }
+            }
+            sto_off[r + 1] = (uint32_t)sto_sp.size();
+        }
+        if ((rc = dev_upload(&s->d_rx_off, rx_off))) return rc;
+        if ((rc = dev_upload(&s->d_rx_list, rx_list))) return rc;
+        if ((rc = dev_upload(&s->d_sto_off, sto_off))) return rc;
+        if ((rc = dev_upload(&s->d_sto_sp, sto_sp))) return rc;
+        if ((rc = dev_upload(&s->d_sto_val, sto_val))) return rc;
+    }
     if ((rc = upload_tables(s))) return rc;
     LMX_HIP(hipMemset(s->d_ctl, 0, sizeof(lmx::Control)));
     s->timestep = 0;
@@ -566,6 +625,10 @@ int lmx_set_launch_option(lmx_solver *s, int option, int value)
         return LMX_OK;
     case LMX_OPT_FORCE_GENERIC:
         s->force_generic = value != 0;
+        return LMX_OK;
+    case LMX_OPT_YZ_KERNEL:
+        if (value != 0 && value != 1) return fail(LMX_ERR_ARG, "y/z kernel must be 0 (tiles) or 1 (march)");
+        s->yz_kernel = value;
         return LMX_OK;
     }
     return fail(LMX_ERR_ARG, "unknown launch option");

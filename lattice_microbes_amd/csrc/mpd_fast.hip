@@ -3,20 +3,27 @@
 // and the site types as one 32-bit word.  Requires X % 4 == 0 (then a group
 // never straddles a row, and every plane base stays 16-byte aligned).
 //
-//  sweep_x4_kernel     x sweep; the x-neighbours of a group's end sites come
-//                      from the adjacent lanes through cross-lane shuffles
-//                      (a 16-byte-aligned global load when the neighbour
-//                      group is outside the wave)
-//  sweep_march_kernel  y or z sweep; each thread walks a segment of rows along
-//                      the axis keeping the previous, current and next rows
-//                      (and one prefetched row) in registers, so each row is
-//                      read once plus two apron rows per segment
-//  reaction4_kernel    reaction phase, in place; groups with no particle and no
-//                      zero-order propensity return without touching memory
-//                      beyond their first load
+// Particles are sparse (a few percent of sites), so the per-particle work is
+// organised as per-lane loops over the set bits of an occupancy mask: a wave
+// iterates as often as its busiest lane, not once per slot position any lane
+// occupies.  Every particle's move decision is evaluated once per sweep:
 //
-// The per-site arithmetic is pull_site / react_site (lmx_site.h), shared with
-// the one-site-per-thread kernels.
+//  sweep_x4_kernel     x sweep; a lane decides its own particles, then the
+//                      decisions and words of the two neighbouring sites come
+//                      from the adjacent lanes (cross-lane shuffles; decided
+//                      locally when the neighbour group is outside the wave)
+//  sweep_march_kernel  y or z sweep; each thread walks a segment of rows along
+//                      the axis with rows r-1..r+2 in registers, decides row
+//                      r+2 while writing row r, and carries the decisions
+//  reaction4_kernel    reaction phase in place, looping over occupied sites;
+//                      single-particle sites take their probability from the
+//                      device-built table (react_site_table)
+//
+// Decision rule (makeDiffusionChoices, byte_diffusion_1d_dev.cu:86-121): a
+// particle with uniform u moves minus iff u < qm and plus iff
+// 0.5 <= u < qp + 0.5; assembly (performPropagation, :199-270): stayers, then
+// the minus neighbour's plus movers, then the plus neighbour's minus movers,
+// each in slot order, first P kept, the rest recorded as overflow.
 #include "lmx_device.h"
 #include "lmx_kernels.h"
 #include "lmx_site.h"
@@ -36,10 +43,10 @@ __device__ inline uint32_t lane_of(const uint4 &v, int k)
 
 __device__ inline void set_lane(uint4 &v, int k, uint32_t x)
 {
-    if (k == 0) v.x = x;
-    else if (k == 1) v.y = x;
-    else if (k == 2) v.z = x;
-    else v.w = x;
+    v.x = k == 0 ? x : v.x;
+    v.y = k == 1 ? x : v.y;
+    v.z = k == 2 ? x : v.z;
+    v.w = k == 3 ? x : v.w;
 }
 
 template <int W>
@@ -58,13 +65,34 @@ __device__ inline void zero_group(Group4<W> &G)
     G.t = 0u;
 }
 
-template <int W>
-__device__ inline uint32_t group_any(const Group4<W> &G)
+// Bit 7 of each byte set iff the byte is nonzero, folded to 4 bits.
+__device__ inline uint32_t nonzero_bytes(uint32_t v)
 {
-    uint32_t a = 0;
+    const uint32_t hi = (((v & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | v) & 0x80808080u;
+    return ((hi >> 7) & 1u) | ((hi >> 14) & 2u) | ((hi >> 21) & 4u) | ((hi >> 28) & 8u);
+}
+
+// Occupancy mask of a site's W words: bit j = slot j nonzero.
+template <int W>
+__device__ inline uint32_t site_occ(const uint32_t *w)
+{
+    uint32_t m = 0;
 #pragma unroll
-    for (int k = 0; k < W; k++) a |= G.w[k].x | G.w[k].y | G.w[k].z | G.w[k].w;
-    return a;
+    for (int k = 0; k < W; k++) m |= nonzero_bytes(w[k]) << (4 * k);
+    return m;
+}
+
+// Occupancy of a group: bit s*P + j = slot j of site s nonzero.
+template <int W>
+__device__ inline unsigned long long group_occ(const Group4<W> &G)
+{
+    unsigned long long m = 0;
+#pragma unroll
+    for (int s = 0; s < 4; s++)
+#pragma unroll
+        for (int k = 0; k < W; k++)
+            m |= (unsigned long long)nonzero_bytes(lane_of(G.w[k], s)) << (s * 4 * W + 4 * k);
+    return m;
 }
 
 template <int W>
@@ -74,6 +102,120 @@ __device__ inline void site_words(const Group4<W> &G, int s, uint32_t *out)
     for (int k = 0; k < W; k++) out[k] = lane_of(G.w[k], s);
 }
 
+// Slot j of a site (j may be a runtime value).
+template <int W>
+__device__ inline uint32_t slot_dyn(const uint32_t *w, uint32_t j)
+{
+    uint32_t word = w[0];
+#pragma unroll
+    for (int k = 1; k < W; k++) word = ((j >> 2) == (uint32_t)k) ? w[k] : word;
+    return (word >> (8 * (j & 3u))) & 0xFFu;
+}
+
+// Slot j of site s of a group (s, j may be runtime values).
+template <int W>
+__device__ inline uint32_t group_slot(const Group4<W> &G, uint32_t s, uint32_t j)
+{
+    uint32_t word = 0;
+#pragma unroll
+    for (int k = 0; k < W; k++) {
+        const uint32_t v = lane_of(G.w[k], (int)s);
+        word = ((j >> 2) == (uint32_t)k) ? v : word;
+    }
+    return (word >> (8 * (j & 3u))) & 0xFFu;
+}
+
+__device__ inline uint32_t lane_dyn(const uint4 &v, uint32_t s)
+{
+    return s == 0 ? v.x : (s == 1 ? v.y : (s == 2 ? v.z : v.w));
+}
+
+template <int W>
+__device__ inline uint32_t group_slot_dyn(const Group4<W> &G, uint32_t s, uint32_t j)
+{
+    uint32_t word = 0;
+#pragma unroll
+    for (int k = 0; k < W; k++) {
+        const uint32_t v = lane_dyn(G.w[k], s);
+        word = ((j >> 2) == (uint32_t)k) ? v : word;
+    }
+    return (word >> (8 * (j & 3u))) & 0xFFu;
+}
+
+// Decide every particle selected by pending.  Site s of the group has index
+// base + s, type byte s of G.t, minus-neighbour type byte s of tm and
+// plus-neighbour type byte s of tp; hm/hp (bit s) say whether that neighbour
+// exists.  Bit b = s*P + j of minus/plus is set for a particle moving minus /
+// plus.
+template <int W>
+__device__ inline void decide(const Params &p, const Group4<W> &G, uint32_t tm, uint32_t tp, uint32_t hm,
+                              uint32_t hp, uint32_t base, unsigned long long h, unsigned long long pending,
+                              unsigned long long &minus, unsigned long long &plus)
+{
+    constexpr uint32_t P = 4 * W;
+    constexpr uint32_t LOGP = W == 1 ? 2 : (W == 2 ? 3 : 4);
+    unsigned long long mi = 0, pl = 0;
+    while (pending) {
+        const uint32_t b = (uint32_t)__builtin_ctzll(pending);
+        pending &= pending - 1;
+        const uint32_t s = b >> LOGP, j = b & (P - 1);
+        const uint32_t part = group_slot_dyn<W>(G, s, j);
+        const uint32_t st = (G.t >> (8 * s)) & 0xFFu;
+        const uint32_t stm = (tm >> (8 * s)) & 0xFFu;
+        const uint32_t stp = (tp >> (8 * s)) & 0xFFu;
+        const float u = uniform01(base + s, p.pcb, j, h);
+        const float qm = ((hm >> s) & 1u) ? lookup_T(p, part, st, stm) : 0.0f;
+        const float qp = ((hp >> s) & 1u) ? lookup_T(p, part, st, stp) : 0.0f;
+        if (u < qm) mi |= 1ull << b;
+        else if (u >= 0.5f && u < (qp + 0.5f)) pl |= 1ull << b;
+    }
+    minus = mi;
+    plus = pl;
+}
+
+// New content of site i: own stayers, minus-source movers, plus-source movers.
+template <int W>
+__device__ inline void assemble(const Params &p, uint32_t i, uint32_t phase, const uint32_t *own, uint32_t stay,
+                                uint32_t own_occ, const uint32_t *ms, uint32_t msel, const uint32_t *ps,
+                                uint32_t psel, uint32_t *out)
+{
+    // Nothing arrives or leaves and the site is packed: unchanged.
+    if ((msel | psel) == 0 && stay == own_occ && (own_occ & (own_occ + 1)) == 0) {
+#pragma unroll
+        for (int k = 0; k < W; k++) out[k] = own[k];
+        return;
+    }
+    SlotBuf<W> b;
+    b.clear();
+    while (stay) {
+        const uint32_t j = (uint32_t)__builtin_ctz(stay);
+        stay &= stay - 1;
+        const uint32_t part = slot_dyn<W>(own, j);
+        if (!b.push(part)) record_overflow(p.ctl, phase, i, b.n - 1 - 4 * W, part);
+    }
+    while (msel) {
+        const uint32_t j = (uint32_t)__builtin_ctz(msel);
+        msel &= msel - 1;
+        const uint32_t part = slot_dyn<W>(ms, j);
+        if (!b.push(part)) record_overflow(p.ctl, phase, i, b.n - 1 - 4 * W, part);
+    }
+    while (psel) {
+        const uint32_t j = (uint32_t)__builtin_ctz(psel);
+        psel &= psel - 1;
+        const uint32_t part = slot_dyn<W>(ps, j);
+        if (!b.push(part)) record_overflow(p.ctl, phase, i, b.n - 1 - 4 * W, part);
+    }
+#pragma unroll
+    for (int k = 0; k < W; k++) out[k] = b.w[k];
+}
+
+template <int W>
+__device__ inline uint32_t site_bits(unsigned long long m, int s)
+{
+    constexpr uint32_t P = 4 * W;
+    return (uint32_t)(m >> (s * P)) & ((1u << P) - 1u);
+}
+
 // ---------------------------------------------------------------------------
 // x sweep
 // ---------------------------------------------------------------------------
@@ -81,6 +223,7 @@ template <int W>
 __global__ __launch_bounds__(256) void sweep_x4_kernel(Params p, const uint32_t *__restrict__ src,
                                                        uint32_t *__restrict__ dst)
 {
+    constexpr uint32_t P = 4 * W;
     const uint32_t ngroups = p.N >> 2;
     const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
     const bool active = g < ngroups;
@@ -100,24 +243,24 @@ __global__ __launch_bounds__(256) void sweep_x4_kernel(Params p, const uint32_t 
     const int lr = (int)(ir >> 2) - wave_first;
     const bool shl = ll >= 0 && ll < 64;
     const bool shr = lr >= 0 && lr < 64;
+    const int sl = shl ? ll : lane, sr = shr ? lr : lane;
 
-    // Neighbour sites: the left neighbour is site 3 of its group, the right
-    // neighbour site 0 of its group.
-    uint32_t lw[W], rw[W], lt, rt;
+    // Words and types of the neighbour sites (left: site 3 of its group,
+    // right: site 0 of its group).
+    uint32_t lw[W], rw[W];
 #pragma unroll
     for (int k = 0; k < W; k++) {
-        lw[k] = __shfl(G.w[k].w, shl ? ll : lane, 64);
-        rw[k] = __shfl(G.w[k].x, shr ? lr : lane, 64);
+        lw[k] = __shfl(G.w[k].w, sl, 64);
+        rw[k] = __shfl(G.w[k].x, sr, 64);
     }
-    lt = __shfl(G.t >> 24, shl ? ll : lane, 64);
-    rt = __shfl(G.t & 0xFFu, shr ? lr : lane, 64);
-    if (!active) return;
-    if (!shl) {
+    uint32_t lt = __shfl(G.t >> 24, sl, 64);
+    uint32_t rt = __shfl(G.t & 0xFFu, sr, 64);
+    if (active && !shl) {
 #pragma unroll
         for (int k = 0; k < W; k++) lw[k] = src[(size_t)k * p.N + il];
         lt = p.sites[il];
     }
-    if (!shr) {
+    if (active && !shr) {
 #pragma unroll
         for (int k = 0; k < W; k++) rw[k] = src[(size_t)k * p.N + ir];
         rt = p.sites[ir];
@@ -130,49 +273,87 @@ __global__ __launch_bounds__(256) void sweep_x4_kernel(Params p, const uint32_t 
 #pragma unroll
         for (int k = 0; k < W; k++) rw[k] = 0u;
     }
-    uint32_t any = group_any<W>(G);
-#pragma unroll
-    for (int k = 0; k < W; k++) any |= lw[k] | rw[k];
+
+    const unsigned long long h = timestep_hash(p.seed32, p.ctl->timestep, 0);
+    const unsigned long long occ = active ? group_occ<W>(G) : 0ull;
+    unsigned long long minus = 0, plus = 0;
+    if (occ) {
+        const uint32_t tm = (G.t << 8) | lt;
+        const uint32_t tp = (G.t >> 8) | (rt << 24);
+        decide<W>(p, G, tm, tp, has_m ? 0xFu : 0xEu, has_p ? 0xFu : 0x7u, i0, h, occ, minus, plus);
+    }
+    // Plus movers of the left neighbour and minus movers of the right one.
+    uint32_t lplus = __shfl(site_bits<W>(plus, 3), sl, 64);
+    uint32_t rminus = __shfl(site_bits<W>(minus, 0), sr, 64);
+    if (!active) return;
+    const uint32_t st0 = G.t & 0xFFu, st3 = G.t >> 24;
+    if (!shl) {
+        lplus = 0;
+        uint32_t m = site_occ<W>(lw);
+        while (m) {
+            const uint32_t j = (uint32_t)__builtin_ctz(m);
+            m &= m - 1;
+            const uint32_t part = slot_dyn<W>(lw, j);
+            const float u = uniform01(il, p.pcb, j, h);
+            const float qp = lookup_T(p, part, lt, st0);
+            if (u >= 0.5f && u < (qp + 0.5f)) lplus |= 1u << j;
+        }
+    }
+    if (!shr) {
+        rminus = 0;
+        uint32_t m = site_occ<W>(rw);
+        while (m) {
+            const uint32_t j = (uint32_t)__builtin_ctz(m);
+            m &= m - 1;
+            const uint32_t part = slot_dyn<W>(rw, j);
+            const float u = uniform01(ir, p.pcb, j, h);
+            const float qm = lookup_T(p, part, rt, st3);
+            if (u < qm) rminus |= 1u << j;
+        }
+    }
+    if (!has_m) lplus = 0;
+    if (!has_p) rminus = 0;
+
     uint4 *out4[W];
 #pragma unroll
     for (int k = 0; k < W; k++) out4[k] = reinterpret_cast<uint4 *>(dst + (size_t)k * p.N) + g;
+    uint32_t any = (uint32_t)(occ != 0) | lplus | rminus;
     if (any == 0) {
 #pragma unroll
         for (int k = 0; k < W; k++) *out4[k] = make_uint4(0u, 0u, 0u, 0u);
         return;
     }
-    const unsigned long long h = timestep_hash(p.seed32, p.ctl->timestep, 0);
     uint4 res[W];
 #pragma unroll
     for (int s = 0; s < 4; s++) {
         uint32_t wo[W], wm[W], wp[W], o[W];
         site_words<W>(G, s, wo);
+        const uint32_t own_occ = site_bits<W>(occ, s);
+        const uint32_t stay = own_occ & ~(site_bits<W>(minus, s) | site_bits<W>(plus, s));
+        uint32_t msel, psel;
         if (s == 0) {
 #pragma unroll
             for (int k = 0; k < W; k++) wm[k] = lw[k];
+            msel = lplus;
         } else {
             site_words<W>(G, s - 1, wm);
+            msel = site_bits<W>(plus, s - 1);
         }
         if (s == 3) {
 #pragma unroll
             for (int k = 0; k < W; k++) wp[k] = rw[k];
+            psel = rminus;
         } else {
             site_words<W>(G, s + 1, wp);
+            psel = site_bits<W>(minus, s + 1);
         }
-        const uint32_t st = (G.t >> (8 * s)) & 0xFFu;
-        const uint32_t stm = s == 0 ? lt : ((G.t >> (8 * (s - 1))) & 0xFFu);
-        const uint32_t stp = s == 3 ? rt : ((G.t >> (8 * (s + 1))) & 0xFFu);
-        const bool hm = s == 0 ? has_m : true;
-        const bool hp = s == 3 ? has_p : true;
-        const uint32_t i = i0 + s;
-        const uint32_t im = s == 0 ? il : i - 1;
-        const uint32_t ip = s == 3 ? ir : i + 1;
-        pull_site<W>(p, i, im, ip, wo, wm, wp, st, stm, stp, hm, hp, h, 0, o);
+        assemble<W>(p, i0 + s, 0, wo, stay, own_occ, wm, msel, wp, psel, o);
 #pragma unroll
         for (int k = 0; k < W; k++) set_lane(res[k], s, o[k]);
     }
 #pragma unroll
     for (int k = 0; k < W; k++) *out4[k] = res[k];
+    (void)P;
 }
 
 // ---------------------------------------------------------------------------
@@ -183,19 +364,31 @@ __global__ __launch_bounds__(256) void sweep_x4_kernel(Params p, const uint32_t 
 // segment s = t / ((X/4) * nouter) covering rows [s*seg, min((s+1)*seg, len)).
 // ---------------------------------------------------------------------------
 template <int W>
-__device__ inline void load_row(const Params &p, const uint32_t *__restrict__ src, uint32_t base, uint32_t stride,
+__device__ inline bool load_row(const Params &p, const uint32_t *__restrict__ src, uint32_t base, uint32_t stride,
                                 uint32_t len, int r, Group4<W> &G)
 {
-    // r in [-1, len]; wraps periodically, empty beyond a reflecting edge.
+    // r in [-1, len+1]; wraps periodically, empty beyond a reflecting edge.
     int rr = r;
     if (rr < 0) {
-        if (p.boundary) { zero_group<W>(G); return; }
+        if (p.boundary) { zero_group<W>(G); return false; }
         rr += (int)len;
     } else if (rr >= (int)len) {
-        if (p.boundary) { zero_group<W>(G); return; }
+        if (p.boundary) { zero_group<W>(G); return false; }
         rr -= (int)len;
     }
+    rr %= (int)len;
     load_group<W>(p, src, (base + (uint32_t)rr * stride) >> 2, G);
+    return true;
+}
+
+template <int W>
+__device__ inline unsigned long long row_index(const Params &p, uint32_t base, uint32_t stride, uint32_t len, int r)
+{
+    int rr = r;
+    if (rr < 0) rr += (int)len;
+    if (rr >= (int)len) rr -= (int)len;
+    rr %= (int)len;
+    return base + (uint32_t)rr * stride;
 }
 
 template <int W>
@@ -215,24 +408,41 @@ __global__ __launch_bounds__(256) void sweep_march_kernel(Params p, const uint32
     const uint32_t r1 = min(r0 + seg, len);
     const uint32_t base = o * ostride + g * 4u;
     const unsigned long long h = timestep_hash(p.seed32, p.ctl->timestep, substep);
+    const bool periodic = p.boundary == 0;
+    // Neighbour flags of row r, one bit per site (all four sites share them).
+    auto hm_of = [&](int r) -> uint32_t { return (r > 0 || periodic) ? 0xFu : 0u; };
+    auto hp_of = [&](int r) -> uint32_t { return (r + 1 < (int)len || periodic) ? 0xFu : 0u; };
 
-    Group4<W> A, B, C, D;  // rows r-1, r, r+1, r+2
-    zero_group<W>(D);
+    // Rows r-1 (A), r (B), r+1 (C), r+2 (D); E is the row being fetched.
+    Group4<W> A, B, C, D, E;
     load_row<W>(p, src, base, stride, len, (int)r0 - 1, A);
     load_row<W>(p, src, base, stride, len, (int)r0, B);
     load_row<W>(p, src, base, stride, len, (int)r0 + 1, C);
+    load_row<W>(p, src, base, stride, len, (int)r0 + 2, D);
+    zero_group<W>(E);
+    // Decisions: plus movers of A, both directions of B and C.  Rows beyond a
+    // reflecting edge are empty, so they decide nothing.
+    unsigned long long occA = group_occ<W>(A), occB = group_occ<W>(B), occC = group_occ<W>(C);
+    unsigned long long mA, pA, mB, pB, mC, pC;
+    {
+        // A's minus neighbour (row r0-2) matters only for its minus movers,
+        // which never reach row r0: pass "no minus neighbour".
+        const int ra = (int)r0 - 1;
+        decide<W>(p, A, 0u, B.t, 0u, hp_of(ra), (uint32_t)row_index<W>(p, base, stride, len, ra), h, occA, mA, pA);
+        decide<W>(p, B, A.t, C.t, hm_of((int)r0), hp_of((int)r0), (uint32_t)row_index<W>(p, base, stride, len, (int)r0),
+                  h, occB, mB, pB);
+        decide<W>(p, C, B.t, D.t, hm_of((int)r0 + 1), hp_of((int)r0 + 1),
+                  (uint32_t)row_index<W>(p, base, stride, len, (int)r0 + 1), h, occC, mC, pC);
+    }
+    (void)mA;
     for (uint32_t r = r0; r < r1; r++) {
-        if (r + 1 < r1) load_row<W>(p, src, base, stride, len, (int)r + 2, D);
-        const bool has_m = (r > 0) || (p.boundary == 0);
-        const bool has_p = (r + 1 < len) || (p.boundary == 0);
+        const bool more = r + 1 < r1;
+        if (more) load_row<W>(p, src, base, stride, len, (int)r + 3, E);
         const uint32_t rowi = base + r * stride;
-        const uint32_t rowm = (r > 0) ? rowi - stride : rowi + (len - 1) * stride;
-        const uint32_t rowp = (r + 1 < len) ? rowi + stride : rowi - (len - 1) * stride;
         uint4 *out4[W];
 #pragma unroll
         for (int k = 0; k < W; k++) out4[k] = reinterpret_cast<uint4 *>(dst + (size_t)k * p.N + rowi);
-        const uint32_t any = group_any<W>(A) | group_any<W>(B) | group_any<W>(C);
-        if (any == 0) {
+        if ((occA | occB | occC) == 0) {
 #pragma unroll
             for (int k = 0; k < W; k++) *out4[k] = make_uint4(0u, 0u, 0u, 0u);
         } else {
@@ -243,59 +453,289 @@ __global__ __launch_bounds__(256) void sweep_march_kernel(Params p, const uint32
                 site_words<W>(B, s, wo);
                 site_words<W>(A, s, wm);
                 site_words<W>(C, s, wp);
-                const uint32_t st = (B.t >> (8 * s)) & 0xFFu;
-                const uint32_t stm = (A.t >> (8 * s)) & 0xFFu;
-                const uint32_t stp = (C.t >> (8 * s)) & 0xFFu;
-                pull_site<W>(p, rowi + s, rowm + s, rowp + s, wo, wm, wp, st, stm, stp, has_m, has_p, h, substep, ov);
+                const uint32_t own_occ = site_bits<W>(occB, s);
+                const uint32_t stay = own_occ & ~(site_bits<W>(mB, s) | site_bits<W>(pB, s));
+                assemble<W>(p, rowi + s, substep, wo, stay, own_occ, wm, site_bits<W>(pA, s), wp,
+                            site_bits<W>(mC, s), ov);
 #pragma unroll
                 for (int k = 0; k < W; k++) set_lane(res[k], s, ov[k]);
             }
 #pragma unroll
             for (int k = 0; k < W; k++) *out4[k] = res[k];
         }
+        if (!more) break;
+        // Decide row r+2 (D) with its neighbours C and E, then shift.
+        const unsigned long long occD = group_occ<W>(D);
+        unsigned long long mD = 0, pD = 0;
+        if (occD) {
+            const int rd = (int)r + 2;
+            decide<W>(p, D, C.t, E.t, hm_of(rd), hp_of(rd), (uint32_t)row_index<W>(p, base, stride, len, rd), h,
+                      occD, mD, pD);
+        }
         A = B;
         B = C;
         C = D;
+        D = E;
+        occA = occB;
+        occB = occC;
+        occC = occD;
+        pA = pB;
+        mB = mC;
+        pB = pC;
+        mC = mD;
+        pC = pD;
     }
 }
 
 // ---------------------------------------------------------------------------
-// Reaction phase, 4 sites per thread, in place.
+// y / z sweep on LDS tiles.  A block covers Gx groups along x by Ty rows along
+// the axis (one group per thread), stages its rows plus one apron row on each
+// side in LDS, decides its own rows' particles (the apron rows only in the
+// direction that enters the tile), exchanges the decision masks through LDS
+// and assembles its rows.  All row loads are issued at once.  Blocks are
+// numbered so that the tiles a given XCD (blockIdx % 8) runs are adjacent
+// along the axis, keeping shared apron rows in that XCD's L2.
 // ---------------------------------------------------------------------------
+template <int W>
+constexpr int tile_threads() { return W == 4 ? 512 : 1024; }
+
+template <int W>
+__global__ __launch_bounds__(tile_threads<W>()) void sweep_tile_kernel(
+    Params p, const uint32_t *__restrict__ src, uint32_t *__restrict__ dst, uint32_t stride, uint32_t len,
+    uint32_t ostride, uint32_t nouter, uint32_t Gx, uint32_t Ty, uint32_t nxchunk, uint32_t ntiles,
+    uint32_t substep)
+{
+    constexpr int kEntries = tile_threads<W>() + 128;
+    __shared__ uint4 s_w[W][kEntries];
+    __shared__ uint32_t s_t[kEntries];
+    __shared__ unsigned long long s_m[kEntries], s_p[kEntries];
+
+    const uint32_t nblocks = nxchunk * nouter * ntiles;
+    const uint32_t per_xcd = (nblocks + 7) / 8;
+    const uint32_t logical = (blockIdx.x & 7u) * per_xcd + (blockIdx.x >> 3);
+    if (logical >= nblocks) return;  // whole block: no barrier is skipped by part of it
+    const uint32_t tile = logical % ntiles;
+    const uint32_t rest = logical / ntiles;
+    const uint32_t xc = rest % nxchunk;
+    const uint32_t o = rest / nxchunk;
+
+    const uint32_t tid = threadIdx.x;
+    const uint32_t lx = tid % Gx, ly = tid / Gx;
+    const uint32_t gx = xc * Gx + lx;
+    const bool xvalid = gx < (p.X >> 2) && ly < Ty;
+    const uint32_t r0 = tile * Ty;
+    const uint32_t nrows = min(Ty, len - r0);
+    const uint32_t r = r0 + ly;
+    const bool valid = xvalid && ly < nrows;
+    const uint32_t base = o * ostride + gx * 4u;
+    const bool periodic = p.boundary == 0;
+    const unsigned long long h = timestep_hash(p.seed32, p.ctl->timestep, substep);
+
+    Group4<W> G;
+    zero_group<W>(G);
+    if (valid) load_group<W>(p, src, (base + r * stride) >> 2, G);
+    Group4<W> Ap;  // apron row for this thread (top or bottom), if any
+    zero_group<W>(Ap);
+    const bool top = valid && ly == 0;
+    const bool bottom = valid && ly == nrows - 1;
+    if (top) load_row<W>(p, src, base, stride, len, (int)r0 - 1, Ap);
+    Group4<W> Bp;
+    zero_group<W>(Bp);
+    if (bottom) load_row<W>(p, src, base, stride, len, (int)(r0 + nrows), Bp);
+    // Slot (row+1)*Gx + lx; rows at or beyond nrows write nothing, because
+    // slot nrows+1 holds the bottom apron row.
+    const uint32_t e = (ly + 1) * Gx + lx;
+    if (ly < nrows) {
+#pragma unroll
+        for (int k = 0; k < W; k++) s_w[k][e] = G.w[k];
+        s_t[e] = G.t;
+    }
+    if (top) {
+#pragma unroll
+        for (int k = 0; k < W; k++) s_w[k][lx] = Ap.w[k];
+        s_t[lx] = Ap.t;
+    }
+    if (bottom) {
+        const uint32_t eb = (nrows + 1) * Gx + lx;
+#pragma unroll
+        for (int k = 0; k < W; k++) s_w[k][eb] = Bp.w[k];
+        s_t[eb] = Bp.t;
+    }
+    __syncthreads();
+
+    unsigned long long occ = valid ? group_occ<W>(G) : 0ull;
+    unsigned long long mi = 0, pl = 0;
+    if (occ) {
+        const uint32_t hm = (r > 0 || periodic) ? 0xFu : 0u;
+        const uint32_t hp = (r + 1 < len || periodic) ? 0xFu : 0u;
+        decide<W>(p, G, s_t[e - Gx], s_t[e + Gx], hm, hp, base + r * stride, h, occ, mi, pl);
+    }
+    if (ly < nrows) {
+        s_m[e] = mi;
+        s_p[e] = pl;
+    }
+    if (top) {
+        // Row r0-1: only its plus movers enter the tile.
+        unsigned long long am = 0, ap = 0;
+        const unsigned long long ao = group_occ<W>(Ap);
+        if (ao) {
+            const int ra = (int)r0 - 1;
+            decide<W>(p, Ap, 0u, G.t, 0u, 0xFu, (uint32_t)row_index<W>(p, base, stride, len, ra), h, ao, am, ap);
+        }
+        s_p[lx] = ap;
+    }
+    if (bottom) {
+        // Row r0+nrows: only its minus movers enter the tile.
+        unsigned long long bm = 0, bpl = 0;
+        const unsigned long long bo = group_occ<W>(Bp);
+        if (bo) {
+            const int rb = (int)(r0 + nrows);
+            decide<W>(p, Bp, G.t, 0u, 0xFu, 0u, (uint32_t)row_index<W>(p, base, stride, len, rb), h, bo, bm, bpl);
+        }
+        s_m[(nrows + 1) * Gx + lx] = bm;
+    }
+    __syncthreads();
+    if (!valid) return;
+
+    const unsigned long long pa = s_p[e - Gx], mc = s_m[e + Gx];
+    uint4 *out4[W];
+#pragma unroll
+    for (int k = 0; k < W; k++) out4[k] = reinterpret_cast<uint4 *>(dst + (size_t)k * p.N + base + r * stride);
+    if ((occ | pa | mc) == 0) {
+#pragma unroll
+        for (int k = 0; k < W; k++) *out4[k] = make_uint4(0u, 0u, 0u, 0u);
+        return;
+    }
+    Group4<W> A, C;
+#pragma unroll
+    for (int k = 0; k < W; k++) {
+        A.w[k] = s_w[k][e - Gx];
+        C.w[k] = s_w[k][e + Gx];
+    }
+    uint4 res[W];
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+        uint32_t wo[W], wm[W], wp[W], ov[W];
+        site_words<W>(G, s, wo);
+        site_words<W>(A, s, wm);
+        site_words<W>(C, s, wp);
+        const uint32_t own_occ = site_bits<W>(occ, s);
+        const uint32_t stay = own_occ & ~(site_bits<W>(mi, s) | site_bits<W>(pl, s));
+        assemble<W>(p, base + r * stride + s, substep, wo, stay, own_occ, wm, site_bits<W>(pa, s), wp,
+                    site_bits<W>(mc, s), ov);
+#pragma unroll
+        for (int k = 0; k < W; k++) set_lane(res[k], s, ov[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < W; k++) *out4[k] = res[k];
+}
+
+// ---------------------------------------------------------------------------
+// Reaction phase in place.  Each thread owns kRG groups (4*kRG sites, groups
+// 256 apart so every load stays coalesced) and issues all their loads before
+// any per-site work; one loop then visits the occupied sites.  The
+// single-particle table is staged in LDS when it fits.
+// ---------------------------------------------------------------------------
+constexpr int kRG = 4;
+constexpr uint32_t kTabLds = 2048;  // float2 entries (16 KB)
+
 template <int W>
 __global__ __launch_bounds__(256) void reaction4_kernel(Params p, uint32_t *__restrict__ lat)
 {
+    __shared__ float2 s_tab[kTabLds];
+    const uint32_t ntab = p.nST * p.nS;
+    const bool in_lds = ntab <= kTabLds;
+    if (in_lds) {
+        for (uint32_t k = threadIdx.x; k < ntab; k += blockDim.x) s_tab[k] = reinterpret_cast<const float2 *>(p.prob1)[k];
+    }
+    __syncthreads();
+    const float2 *tab = in_lds ? s_tab : reinterpret_cast<const float2 *>(p.prob1);
+
     const uint32_t ngroups = p.N >> 2;
-    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= ngroups) return;
-    uint4 w4[W];
+    const uint32_t g0 = blockIdx.x * (256u * kRG) + threadIdx.x;
+    uint4 w4[kRG][W];
+    uint32_t t4[kRG];
+    uint32_t todo = 0;
 #pragma unroll
-    for (int k = 0; k < W; k++) w4[k] = reinterpret_cast<const uint4 *>(lat + (size_t)k * p.N)[g];
-    uint32_t any = 0;
+    for (int q = 0; q < kRG; q++) {
+        const uint32_t g = g0 + 256u * q;
+        if (g < ngroups) {
 #pragma unroll
-    for (int k = 0; k < W; k++) any |= w4[k].x | w4[k].y | w4[k].z | w4[k].w;
-    if (any == 0 && !p.any_qp0) return;
-    const uint32_t t4 = reinterpret_cast<const uint32_t *>(p.sites)[g];
-    const unsigned long long h3 = timestep_hash(p.seed32, p.ctl->timestep, 3);
-    bool changed = false;
+            for (int k = 0; k < W; k++) w4[q][k] = reinterpret_cast<const uint4 *>(lat + (size_t)k * p.N)[g];
+            t4[q] = reinterpret_cast<const uint32_t *>(p.sites)[g];
+        } else {
 #pragma unroll
-    for (int s = 0; s < 4; s++) {
-        uint32_t w[W];
-#pragma unroll
-        for (int k = 0; k < W; k++) w[k] = lane_of(w4[k], s);
-        uint32_t sany = 0;
-#pragma unroll
-        for (int k = 0; k < W; k++) sany |= w[k];
-        if (sany == 0 && !p.any_qp0) continue;
-        if (react_site<W>(p, g * 4u + s, (t4 >> (8 * s)) & 0xFFu, w, h3)) {
-            changed = true;
-#pragma unroll
-            for (int k = 0; k < W; k++) set_lane(w4[k], s, w[k]);
+            for (int k = 0; k < W; k++) w4[q][k] = make_uint4(0u, 0u, 0u, 0u);
+            t4[q] = 0u;
         }
     }
-    if (changed) {
 #pragma unroll
-        for (int k = 0; k < W; k++) reinterpret_cast<uint4 *>(lat + (size_t)k * p.N)[g] = w4[k];
+    for (int q = 0; q < kRG; q++) {
+        const bool inside = g0 + 256u * q < ngroups;
+#pragma unroll
+        for (int s = 0; s < 4; s++) {
+            uint32_t a = 0;
+#pragma unroll
+            for (int k = 0; k < W; k++) a |= lane_of(w4[q][k], s);
+            const bool need = inside && (p.any_qp0 || a != 0);
+            todo |= (need ? 1u : 0u) << (4 * q + s);
+        }
+    }
+    if (todo == 0) return;
+    const unsigned long long h3 = timestep_hash(p.seed32, p.ctl->timestep, 3);
+    uint32_t changed = 0;
+    while (todo) {
+        const uint32_t b = (uint32_t)__builtin_ctz(todo);
+        todo &= todo - 1;
+        const uint32_t q = b >> 2, s = b & 3u;
+        uint32_t w[W], tq = 0;
+#pragma unroll
+        for (int k = 0; k < W; k++) w[k] = 0;
+#pragma unroll
+        for (int qq = 0; qq < kRG; qq++) {
+            if (qq == (int)q) {
+#pragma unroll
+                for (int k = 0; k < W; k++) w[k] = lane_dyn(w4[qq][k], s);
+                tq = t4[qq];
+            }
+        }
+        const uint32_t g = g0 + 256u * q;
+        if (react_site_table<W>(p, g * 4u + s, (tq >> (8 * s)) & 0xFFu, w, h3, tab)) {
+            changed |= 1u << q;
+#pragma unroll
+            for (int qq = 0; qq < kRG; qq++)
+                if (qq == (int)q) {
+#pragma unroll
+                    for (int k = 0; k < W; k++) set_lane(w4[qq][k], (int)s, w[k]);
+                }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < kRG; q++) {
+        if (changed & (1u << q)) {
+            const uint32_t g = g0 + 256u * q;
+#pragma unroll
+            for (int k = 0; k < W; k++) reinterpret_cast<uint4 *>(lat + (size_t)k * p.N)[g] = w4[q][k];
+        }
+    }
+}
+
+// Probability tables for sites with at most one particle, built with the
+// same float sum and device exp as the reaction kernels.
+__global__ void prob_table_kernel(Params p, float *prob1, float *prob0)
+{
+    const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t n1 = p.nST * p.nS;
+    if (idx < n1) {
+        const uint32_t st = idx / p.nS;
+        float a = p.qp0[st];
+        a += p.qp1[idx];
+        prob1[2 * idx] = a;
+        prob1[2 * idx + 1] = a == 0.0f ? 0.0f : reaction_probability(a);
+    }
+    if (idx < p.nST) {
+        const float a = p.qp0[idx];
+        prob0[idx] = a == 0.0f ? 0.0f : reaction_probability(a);
     }
 }
 
@@ -338,15 +778,52 @@ hipError_t launch_sweep_fast(const Params &p, const uint32_t *src, uint32_t *dst
     return hipGetLastError();
 }
 
+template <int W>
+static hipError_t launch_tile(const Params &p, const uint32_t *src, uint32_t *dst, int axis, hipStream_t st)
+{
+    const uint32_t stride = axis == 1 ? p.X : p.XY;
+    const uint32_t len = axis == 1 ? p.Y : p.Z;
+    const uint32_t ostride = axis == 1 ? p.XY : p.X;
+    const uint32_t nouter = axis == 1 ? p.Z : p.Y;
+    const uint32_t gxn = p.X >> 2;
+    const uint32_t Gx = gxn < 64u ? gxn : 64u;
+    const uint32_t Ty = (uint32_t)tile_threads<W>() / Gx;
+    const uint32_t nxchunk = (gxn + Gx - 1) / Gx;
+    const uint32_t ntiles = (len + Ty - 1) / Ty;
+    const uint32_t nblocks = nxchunk * nouter * ntiles;
+    const uint32_t per_xcd = (nblocks + 7) / 8;
+    hipLaunchKernelGGL(sweep_tile_kernel<W>, dim3(per_xcd * 8), dim3(Gx * Ty), 0, st, p, src, dst, stride, len,
+                       ostride, nouter, Gx, Ty, nxchunk, ntiles, (uint32_t)axis);
+    return hipGetLastError();
+}
+
+hipError_t launch_sweep_tile(const Params &p, const uint32_t *src, uint32_t *dst, int axis, hipStream_t st)
+{
+    switch (p.W) {
+    case 1: return launch_tile<1>(p, src, dst, axis, st);
+    case 2: return launch_tile<2>(p, src, dst, axis, st);
+    case 4: return launch_tile<4>(p, src, dst, axis, st);
+    }
+    return hipErrorInvalidValue;
+}
+
 hipError_t launch_reaction_fast(const Params &p, uint32_t *lat, hipStream_t st)
 {
-    const dim3 grid(blocks_for(p.N >> 2, 256)), block(256);
+    const dim3 grid(blocks_for(p.N >> 2, 256 * kRG)), block(256);
     switch (p.W) {
     case 1: hipLaunchKernelGGL(reaction4_kernel<1>, grid, block, 0, st, p, lat); break;
     case 2: hipLaunchKernelGGL(reaction4_kernel<2>, grid, block, 0, st, p, lat); break;
     case 4: hipLaunchKernelGGL(reaction4_kernel<4>, grid, block, 0, st, p, lat); break;
     default: return hipErrorInvalidValue;
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_prob_tables(const Params &p, float *prob1, float *prob0, hipStream_t st)
+{
+    const size_t n = (size_t)p.nST * p.nS > p.nST ? (size_t)p.nST * p.nS : p.nST;
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(prob_table_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, st, p, prob1, prob0);
     return hipGetLastError();
 }
 

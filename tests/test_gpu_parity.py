@@ -187,12 +187,14 @@ def test_vector_kernels_match_generic_and_oracle(gpu, case):
     orc = oracle.Oracle(spec, seed_top=5, replicate=2)
     assert orc.run(6) == 0
     ref = orc.native()
-    for opt in ({"generic": 1}, {"seg": 1}, {"seg": 3}, {"seg": 7}, {"seg": 32}, {"seg": 100000}):
+    for opt in ({"generic": 1}, {"tile": 1}, {"seg": 1}, {"seg": 3}, {"seg": 7}, {"seg": 32},
+                {"seg": 100000}):
         s = lmx.MpdRdmeSolver(0)
         spec.load_into(s, replicate=2, seed=5)
         if "generic" in opt:
             s.setLaunchOption(lmx._lib.OPT_FORCE_GENERIC, 1)
-        else:
+        elif "seg" in opt:
+            s.setLaunchOption(lmx._lib.OPT_YZ_KERNEL, 1)
             s.setLaunchOption(lmx._lib.OPT_MARCH_SEGMENT, min(opt["seg"], 65536))
         s.runTimesteps(6)
         got = s.getLatticeNative()
