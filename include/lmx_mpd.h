@@ -111,7 +111,8 @@ int lmx_set_path(lmx_solver *s, int path);
 /* Launch-shape options (no reference counterpart: the reference fixes its
  * block shapes at compile time, src/cmake/options.cmake:87-95).  Results do
  * not depend on them.
- *   LMX_OPT_MARCH_SEGMENT  rows per thread of the marching y/z sweep (default 8)
+ *   LMX_OPT_MARCH_SEGMENT  rows per thread of the marching y/z sweep: 4, or 8
+ *                          (default; any other value selects 8)
  *   LMX_OPT_FORCE_GENERIC  1 = one-site-per-thread kernels even when X%4==0
  *   LMX_OPT_YZ_KERNEL      y/z sweep kernel: 0 = LDS row tiles (default),
  *                          1 = register march */

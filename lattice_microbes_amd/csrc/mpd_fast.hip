@@ -391,18 +391,24 @@ __device__ inline unsigned long long row_index(const Params &p, uint32_t base, u
     return base + (uint32_t)rr * stride;
 }
 
-template <int W>
+template <int W, int SEG>
 __global__ __launch_bounds__(256) void sweep_march_kernel(Params p, const uint32_t *__restrict__ src,
                                                           uint32_t *__restrict__ dst, uint32_t stride,
                                                           uint32_t len, uint32_t ostride, uint32_t nouter,
-                                                          uint32_t seg, uint32_t substep)
+                                                          uint32_t substep, uint32_t seg_first)
 {
+    constexpr uint32_t seg = SEG;
     const uint32_t gx = p.X >> 2;
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t g = t % gx;
     const uint32_t rest = t / gx;
-    const uint32_t o = rest % nouter;
-    const uint32_t sgi = rest / nouter;
+    const uint32_t nseg = (len + seg - 1) / seg;
+    if (rest >= nouter * nseg) return;
+    // Neighbouring waves take the work that is nearest in memory: the next
+    // segment when segments are closer than outer rows (y sweep), else the
+    // next outer row (z sweep).  Spreads concurrent accesses over channels.
+    const uint32_t o = seg_first ? rest / nseg : rest % nouter;
+    const uint32_t sgi = seg_first ? rest % nseg : rest / nouter;
     const uint32_t r0 = sgi * seg;
     if (r0 >= len) return;
     const uint32_t r1 = min(r0 + seg, len);
@@ -435,7 +441,12 @@ __global__ __launch_bounds__(256) void sweep_march_kernel(Params p, const uint32
                   (uint32_t)row_index<W>(p, base, stride, len, (int)r0 + 1), h, occC, mC, pC);
     }
     (void)mA;
-    for (uint32_t r = r0; r < r1; r++) {
+    // Fully unrolled: the row shift below is register renaming, and loads can
+    // be issued ahead of the rows that need them.
+#pragma unroll
+    for (uint32_t rr = 0; rr < seg; rr++) {
+        const uint32_t r = r0 + rr;
+        if (r >= r1) break;
         const bool more = r + 1 < r1;
         if (more) load_row<W>(p, src, base, stride, len, (int)r + 3, E);
         const uint32_t rowi = base + r * stride;
@@ -503,7 +514,7 @@ template <int W>
 __global__ __launch_bounds__(tile_threads<W>()) void sweep_tile_kernel(
     Params p, const uint32_t *__restrict__ src, uint32_t *__restrict__ dst, uint32_t stride, uint32_t len,
     uint32_t ostride, uint32_t nouter, uint32_t Gx, uint32_t Ty, uint32_t nxchunk, uint32_t ntiles,
-    uint32_t substep)
+    uint32_t substep, uint32_t tile_first)
 {
     constexpr int kEntries = tile_threads<W>() + 128;
     __shared__ uint4 s_w[W][kEntries];
@@ -514,10 +525,21 @@ __global__ __launch_bounds__(tile_threads<W>()) void sweep_tile_kernel(
     const uint32_t per_xcd = (nblocks + 7) / 8;
     const uint32_t logical = (blockIdx.x & 7u) * per_xcd + (blockIdx.x >> 3);
     if (logical >= nblocks) return;  // whole block: no barrier is skipped by part of it
-    const uint32_t tile = logical % ntiles;
-    const uint32_t rest = logical / ntiles;
-    const uint32_t xc = rest % nxchunk;
-    const uint32_t o = rest / nxchunk;
+    // Adjacent logical blocks take the work nearest in memory: the next row
+    // tile when tiles are closer than outer rows (y sweep), else the next
+    // outer row (z sweep).
+    uint32_t tile, xc, o;
+    if (tile_first) {
+        tile = logical % ntiles;
+        const uint32_t rest = logical / ntiles;
+        xc = rest % nxchunk;
+        o = rest / nxchunk;
+    } else {
+        o = logical % nouter;
+        const uint32_t rest = logical / nouter;
+        xc = rest % nxchunk;
+        tile = rest / nxchunk;
+    }
 
     const uint32_t tid = threadIdx.x;
     const uint32_t lx = tid % Gx, ly = tid / Gx;
@@ -764,17 +786,31 @@ hipError_t launch_sweep_fast(const Params &p, const uint32_t *src, uint32_t *dst
     const uint32_t len = axis == 1 ? p.Y : p.Z;
     const uint32_t ostride = axis == 1 ? p.XY : p.X;
     const uint32_t nouter = axis == 1 ? p.Z : p.Y;
-    if (seg == 0) seg = 1;
+    seg = seg == 4 ? 4u : 8u;  // compiled segment lengths
     const uint32_t nseg = (len + seg - 1) / seg;
     const size_t threads = (size_t)(p.X >> 2) * nouter * nseg;
     const dim3 grid(blocks_for(threads, 256));
     const uint32_t sub = (uint32_t)axis;
-    switch (p.W) {
-    case 1: hipLaunchKernelGGL(sweep_march_kernel<1>, grid, block, 0, st, p, src, dst, stride, len, ostride, nouter, seg, sub); break;
-    case 2: hipLaunchKernelGGL(sweep_march_kernel<2>, grid, block, 0, st, p, src, dst, stride, len, ostride, nouter, seg, sub); break;
-    case 4: hipLaunchKernelGGL(sweep_march_kernel<4>, grid, block, 0, st, p, src, dst, stride, len, ostride, nouter, seg, sub); break;
-    default: return hipErrorInvalidValue;
+    // y sweep: the next segment is seg*X sites away, the next outer (z) plane
+    // X*Y away; z sweep: the next outer (y) row is X away.
+    const uint32_t seg_first = (size_t)seg * stride < (size_t)ostride ? 1u : 0u;
+#define LMX_MARCH(WW, SS) hipLaunchKernelGGL((sweep_march_kernel<WW, SS>), grid, block, 0, st, p, src, dst, stride, len, ostride, nouter, sub, seg_first)
+    if (seg == 4) {
+        switch (p.W) {
+        case 1: LMX_MARCH(1, 4); break;
+        case 2: LMX_MARCH(2, 4); break;
+        case 4: LMX_MARCH(4, 4); break;
+        default: return hipErrorInvalidValue;
+        }
+    } else {
+        switch (p.W) {
+        case 1: LMX_MARCH(1, 8); break;
+        case 2: LMX_MARCH(2, 8); break;
+        case 4: LMX_MARCH(4, 8); break;
+        default: return hipErrorInvalidValue;
+        }
     }
+#undef LMX_MARCH
     return hipGetLastError();
 }
 
@@ -793,7 +829,8 @@ static hipError_t launch_tile(const Params &p, const uint32_t *src, uint32_t *ds
     const uint32_t nblocks = nxchunk * nouter * ntiles;
     const uint32_t per_xcd = (nblocks + 7) / 8;
     hipLaunchKernelGGL(sweep_tile_kernel<W>, dim3(per_xcd * 8), dim3(Gx * Ty), 0, st, p, src, dst, stride, len,
-                       ostride, nouter, Gx, Ty, nxchunk, ntiles, (uint32_t)axis);
+                       ostride, nouter, Gx, Ty, nxchunk, ntiles, (uint32_t)axis,
+                       (size_t)Ty * stride < (size_t)ostride ? 1u : 0u);
     return hipGetLastError();
 }
 
