@@ -113,7 +113,7 @@ def load() -> ctypes.CDLL:
     lib.orc_nsm_lattice.restype = None
     lib.orc_nsm_lattice.argtypes = [c_void_p, u32p]
     lib.orc_heap_sequence.restype = c_int
-    lib.orc_heap_sequence.argtypes = [c_uint32, c_uint32, u32p, f64p, u32p]
+    lib.orc_heap_sequence.argtypes = [c_uint32, c_uint32, u32p, f64p, u32p, u32p]
     _lib = lib
     return lib
 

@@ -107,7 +107,7 @@ static void heap_update(heap_t *h, uint32_t ri, double newTime, double newProp)
 }
 
 int orc_heap_sequence(uint32_t n, uint32_t nupd, const uint32_t *idx, const double *times,
-                      uint32_t *top_out)
+                      uint32_t *top_out, uint32_t *pos_out)
 {
     heap_t h;
     if (!heap_init(&h, n)) { heap_free(&h); return ORC_ERR_ARG; }
@@ -115,6 +115,8 @@ int orc_heap_sequence(uint32_t n, uint32_t nupd, const uint32_t *idx, const doub
         if (idx[u] >= n) { heap_free(&h); return ORC_ERR_ARG; }
         heap_update(&h, idx[u], times[u], 1.0);
         top_out[u] = h.queue[1];
+        if (pos_out)
+            for (uint32_t i = 0; i < n; i++) pos_out[(size_t)u * n + i] = h.pos[i];
     }
     heap_free(&h);
     return ORC_OK;

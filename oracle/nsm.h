@@ -44,10 +44,11 @@ uint64_t orc_nsm_reaction_events(const orc_nsm *s);
 void orc_nsm_counts(const orc_nsm *s, uint32_t *counts);
 void orc_nsm_lattice(const orc_nsm *s, uint32_t *native_out);
 
-/* The reference heap (ReactionQueue.h:60-170) exposed for known-answer tests:
- * run a sequence of updates and report the top after each. */
+/* The binary heap (ReactionQueue.h:60-170) exposed for known-answer tests:
+ * run a sequence of updates and report the top and, if pos_out is given,
+ * every index's heap position (nupd x n) after each update. */
 int orc_heap_sequence(uint32_t n, uint32_t nupd, const uint32_t *idx, const double *times,
-                      uint32_t *top_out);
+                      uint32_t *top_out, uint32_t *pos_out);
 
 #ifdef __cplusplus
 }
