@@ -105,7 +105,9 @@ int lmx_build_diffusion_model(lmx_solver *s, uint32_t numberSiteTypes, const dou
  * recomputed (computePropensities, :697-774) and uploaded before the next step. */
 int lmx_set_reaction_rate(lmx_solver *s, uint32_t rxid, float rate);
 
-/* Select the step implementation (enum lmx_path); results do not depend on it. */
+/* Select the step implementation (enum lmx_path); results do not depend on it.
+ * LMX_PATH_FUSED needs X % 4 == 0 and X <= 256 (a lattice row fits one
+ * wavefront); other shapes, and LMX_OPT_FORCE_GENERIC, run the sweeps path. */
 int lmx_set_path(lmx_solver *s, int path);
 
 /* Launch-shape options (no reference counterpart: the reference fixes its
@@ -115,8 +117,12 @@ int lmx_set_path(lmx_solver *s, int path);
  *                          (default; any other value selects 8)
  *   LMX_OPT_FORCE_GENERIC  1 = one-site-per-thread kernels even when X%4==0
  *   LMX_OPT_YZ_KERNEL      y/z sweep kernel: 0 = LDS row tiles (default),
- *                          1 = register march */
-enum lmx_option { LMX_OPT_MARCH_SEGMENT = 1, LMX_OPT_FORCE_GENERIC = 2, LMX_OPT_YZ_KERNEL = 3 };
+ *                          1 = register march
+ *   LMX_OPT_FUSED_ZSEG     z planes per block of the fused step; 0 (default)
+ *                          picks one block per compute unit */
+enum lmx_option {
+    LMX_OPT_MARCH_SEGMENT = 1, LMX_OPT_FORCE_GENERIC = 2, LMX_OPT_YZ_KERNEL = 3, LMX_OPT_FUSED_ZSEG = 4
+};
 int lmx_set_launch_option(lmx_solver *s, int option, int value);
 
 /* runTimestep (MpdRdmeSolver.cu:996-1193) nsteps times, enqueued on the

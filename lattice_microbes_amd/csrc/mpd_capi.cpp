@@ -106,6 +106,8 @@ struct lmx_solver {
     uint32_t march_seg = 8;
     bool force_generic = false;
     int yz_kernel = 0;  // 0: LDS tiles, 1: register march
+    uint32_t fused_zseg = 0;  // 0: automatic
+    uint32_t num_cu = 256;
 };
 
 namespace {
@@ -288,6 +290,19 @@ int check_device_error(lmx_solver *s)
     return fail(LMX_ERR_HIP, "unknown device error " + std::to_string(hdr[2]));
 }
 
+// Planes per block of the fused step: the option's value, else enough
+// z segments that the grid covers every compute unit once (one block per CU
+// fits the kernel's LDS).
+uint32_t fused_zseg(lmx_solver *s)
+{
+    if (s->fused_zseg) return s->fused_zseg;
+    const uint32_t ty = lmx::fused_tile_rows(s->X, s->W);
+    const uint32_t nty = (s->Y + ty - 1) / ty;
+    uint32_t nzs = s->num_cu > nty ? s->num_cu / nty : 1u;
+    if (nzs > s->Z) nzs = s->Z;
+    return (s->Z + nzs - 1) / nzs;
+}
+
 // One timestep (runTimestep, MpdRdmeSolver.cu:996-1193).  ev, if given,
 // receives events around each launch (ev[0] before the first launch, then one
 // after every launch); kinds receives the kernel class of each launch.
@@ -309,6 +324,18 @@ int enqueue_step(lmx_solver *s, const lmx::Params &p, hipEvent_t *ev, int *kinds
         if (e != hipSuccess) return hip_fail(e, "hipEventRecord");
     }
     const bool fast = !s->force_generic && lmx::fast_path_ok(p);
+    if (fast && s->path == LMX_PATH_FUSED && lmx::fused_ok(p)) {
+        e = lmx::launch_fused(p, s->d_lat[s->cur], s->d_lat[1 - s->cur], fused_zseg(s), s->stream);
+        if (e != hipSuccess) return hip_fail(e, "fused step launch");
+        s->cur = 1 - s->cur;
+        if (int rc = mark(LMX_K_FUSED)) return rc;
+        e = lmx::launch_resolve(p, s->d_lat[s->cur], s->stream);
+        if (e != hipSuccess) return hip_fail(e, "resolve launch");
+        if (int rc = mark(LMX_K_RESOLVE)) return rc;
+        if (nlaunch) *nlaunch = n;
+        s->timestep++;
+        return LMX_OK;
+    }
     for (int axis = 0; axis < 3; axis++) {
         if (!fast)
             e = lmx::launch_sweep(p, s->d_lat[s->cur], s->d_lat[1 - s->cur], axis, s->stream);
@@ -389,6 +416,9 @@ int lmx_create(int device, lmx_solver **out)
     if (e == hipSuccess) e = hipMalloc((void **)&s->d_ctl, sizeof(lmx::Control));
     if (e == hipSuccess) e = hipMemset(s->d_ctl, 0, sizeof(lmx::Control));
     if (e == hipSuccess) e = hipMalloc((void **)&s->d_counts, 256 * sizeof(uint32_t));
+    int ncu = 0;
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device);
+    if (ncu > 0) s->num_cu = (uint32_t)ncu;
     if (e != hipSuccess) {
         int rc = hip_fail(e, "lmx_create");
         lmx_destroy(s);
@@ -610,7 +640,7 @@ int lmx_set_reaction_rate(lmx_solver *s, uint32_t rxid, float rate)
 int lmx_set_path(lmx_solver *s, int path)
 {
     if (!s) return fail(LMX_ERR_ARG, "null solver");
-    if (path != LMX_PATH_SWEEPS) return fail(LMX_ERR_ARG, "unsupported step path");
+    if (path != LMX_PATH_SWEEPS && path != LMX_PATH_FUSED) return fail(LMX_ERR_ARG, "unsupported step path");
     s->path = path;
     return LMX_OK;
 }
@@ -629,6 +659,10 @@ int lmx_set_launch_option(lmx_solver *s, int option, int value)
     case LMX_OPT_YZ_KERNEL:
         if (value != 0 && value != 1) return fail(LMX_ERR_ARG, "y/z kernel must be 0 (tiles) or 1 (march)");
         s->yz_kernel = value;
+        return LMX_OK;
+    case LMX_OPT_FUSED_ZSEG:
+        if (value < 0 || value > 65536) return fail(LMX_ERR_ARG, "fused z segment must be in 0..65536");
+        s->fused_zseg = (uint32_t)value;
         return LMX_OK;
     }
     return fail(LMX_ERR_ARG, "unknown launch option");

@@ -174,10 +174,12 @@ __device__ inline void decide(const Params &p, const Group4<W> &G, uint32_t tm, 
 }
 
 // New content of site i: own stayers, minus-source movers, plus-source movers.
+// Particles beyond P are recorded as overflow when rec is set (a redundant
+// copy of a site owned by another block drops them silently).
 template <int W>
-__device__ inline void assemble(const Params &p, uint32_t i, uint32_t phase, const uint32_t *own, uint32_t stay,
-                                uint32_t own_occ, const uint32_t *ms, uint32_t msel, const uint32_t *ps,
-                                uint32_t psel, uint32_t *out)
+__device__ inline void assemble_rec(const Params &p, uint32_t i, uint32_t phase, bool rec, const uint32_t *own,
+                                    uint32_t stay, uint32_t own_occ, const uint32_t *ms, uint32_t msel,
+                                    const uint32_t *ps, uint32_t psel, uint32_t *out)
 {
     // Nothing arrives or leaves and the site is packed: unchanged.
     if ((msel | psel) == 0 && stay == own_occ && (own_occ & (own_occ + 1)) == 0) {
@@ -191,22 +193,30 @@ __device__ inline void assemble(const Params &p, uint32_t i, uint32_t phase, con
         const uint32_t j = (uint32_t)__builtin_ctz(stay);
         stay &= stay - 1;
         const uint32_t part = slot_dyn<W>(own, j);
-        if (!b.push(part)) record_overflow(p.ctl, phase, i, b.n - 1 - 4 * W, part);
+        if (!b.push(part) && rec) record_overflow(p.ctl, phase, i, b.n - 1 - 4 * W, part);
     }
     while (msel) {
         const uint32_t j = (uint32_t)__builtin_ctz(msel);
         msel &= msel - 1;
         const uint32_t part = slot_dyn<W>(ms, j);
-        if (!b.push(part)) record_overflow(p.ctl, phase, i, b.n - 1 - 4 * W, part);
+        if (!b.push(part) && rec) record_overflow(p.ctl, phase, i, b.n - 1 - 4 * W, part);
     }
     while (psel) {
         const uint32_t j = (uint32_t)__builtin_ctz(psel);
         psel &= psel - 1;
         const uint32_t part = slot_dyn<W>(ps, j);
-        if (!b.push(part)) record_overflow(p.ctl, phase, i, b.n - 1 - 4 * W, part);
+        if (!b.push(part) && rec) record_overflow(p.ctl, phase, i, b.n - 1 - 4 * W, part);
     }
 #pragma unroll
     for (int k = 0; k < W; k++) out[k] = b.w[k];
+}
+
+template <int W>
+__device__ inline void assemble(const Params &p, uint32_t i, uint32_t phase, const uint32_t *own, uint32_t stay,
+                                uint32_t own_occ, const uint32_t *ms, uint32_t msel, const uint32_t *ps,
+                                uint32_t psel, uint32_t *out)
+{
+    assemble_rec<W>(p, i, phase, true, own, stay, own_occ, ms, msel, ps, psel, out);
 }
 
 template <int W>
@@ -762,9 +772,318 @@ __global__ void prob_table_kernel(Params p, float *prob1, float *prob0)
 }
 
 // ---------------------------------------------------------------------------
+// Fused step: x, y and z sweeps and the reaction phase of one timestep in a
+// single pass, reading each site's particles and type once and writing once.
+//
+// A block owns a tile of full x-rows: "work rows" k = 0..nrows+1 cover
+// y0-1 .. y0+nrows (rows 0 and nrows+1 are halo rows owned by the
+// neighbouring tiles).  A wave holds rpw rows of gx = X/4 lanes each, so
+// x-neighbours are lanes of the same row (shuffles, periodic wrap inside
+// the row).  The block marches along z over planes z0-1 .. z1 of its
+// segment; for each input plane zi it
+//   1. x-sweeps every work row of plane zi,
+//   2. exchanges the x-swept rows through LDS (double buffered by plane
+//      parity) and y-sweeps its owned rows -> YO(zi),
+//   3. decides the z moves of YO(zi) (types of zi-1, zi, zi+1),
+//   4. z-assembles plane zo = zi-1 from YO(zi-2), YO(zi-1), YO(zi), runs the
+//      reaction phase on it and writes it.
+// Halo rows and halo planes are computed redundantly and never recorded as
+// overflow, so every overflow entry (phase, site, emission) is recorded once,
+// by the owner, exactly as the separate kernels record it.
+// ---------------------------------------------------------------------------
+template <int W>
+constexpr int fused_threads() { return W == 4 ? 512 : 1024; }
+
+template <int W>
+__global__ __launch_bounds__(fused_threads<W>()) void fused_step_kernel(Params p, const uint32_t *__restrict__ src,
+                                                                         uint32_t *__restrict__ dst, uint32_t gx,
+                                                                         uint32_t rpw, uint32_t Ty, uint32_t nty,
+                                                                         uint32_t Zs, uint32_t nzs)
+{
+    constexpr int NT = fused_threads<W>();
+    __shared__ uint4 s_w[2][W][NT];
+    __shared__ uint32_t s_t[2][NT];
+    __shared__ unsigned long long s_m[2][NT], s_p[2][NT];
+    __shared__ float2 s_tab[kTabLds];
+
+    const uint32_t ntab = p.nST * p.nS;
+    const bool tab_lds = ntab <= kTabLds;
+    if (tab_lds)
+        for (uint32_t k = threadIdx.x; k < ntab; k += NT) s_tab[k] = reinterpret_cast<const float2 *>(p.prob1)[k];
+    const float2 *tab = tab_lds ? s_tab : reinterpret_cast<const float2 *>(p.prob1);
+
+    const uint32_t X = p.X, Y = p.Y, Z = p.Z, XY = p.XY;
+    const bool periodic = p.boundary == 0;
+    const uint32_t ty = blockIdx.x % nty, zsg = blockIdx.x / nty;
+    const uint32_t y0 = ty * Ty, z0 = zsg * Zs;
+    const uint32_t nrows = min(Ty, Y - y0);
+    const uint32_t z1 = min(z0 + Zs, Z);
+    const int lane = (int)(threadIdx.x & 63u);
+    const uint32_t wv = threadIdx.x >> 6;
+    const uint32_t r = (uint32_t)lane / gx, g = (uint32_t)lane % gx;
+    const bool lane_ok = r < rpw;
+    const uint32_t k = wv * rpw + r;                  // work row
+    const bool work = lane_ok && k <= nrows + 1;
+    const bool owned = work && k >= 1 && k <= nrows;
+    const int yl = (int)y0 - 1 + (int)k;              // unwrapped y of the work row
+    const bool row_exists = work && (periodic || (yl >= 0 && yl < (int)Y));
+    const uint32_t y = (uint32_t)((yl + (int)Y) % (int)Y);
+    const uint32_t e = k * gx + g;                    // LDS slot of this thread's group
+    // x-neighbour lanes inside the row
+    const int rb = (int)(r * gx);
+    const int ll = lane_ok ? rb + (int)((g + gx - 1) % gx) : lane;
+    const int lr = lane_ok ? rb + (int)((g + 1) % gx) : lane;
+    const bool hmx = periodic || g > 0, hpx = periodic || g + 1 < gx;
+    const uint32_t t = p.ctl->timestep;
+    const unsigned long long h0 = timestep_hash(p.seed32, t, 0), h1 = timestep_hash(p.seed32, t, 1),
+                             h2 = timestep_hash(p.seed32, t, 2), h3 = timestep_hash(p.seed32, t, 3);
+
+    auto plane_exists = [&](int z) { return periodic || (z >= 0 && z < (int)Z); };
+    auto wrapz = [&](int z) { return (uint32_t)((z + (int)Z) % (int)Z); };
+    auto row_base = [&](int z) { return wrapz(z) * XY + y * X + 4u * g; };
+
+    Group4<W> Gn;  // prefetched input row of the next plane
+    zero_group<W>(Gn);
+    if (row_exists && plane_exists((int)z0 - 1)) load_group<W>(p, src, row_base((int)z0 - 1) >> 2, Gn);
+
+    // z window: Ya = YO(zi-2), Yb = YO(zi-1) (+ their z decisions)
+    Group4<W> Ya, Yb;
+    zero_group<W>(Ya);
+    zero_group<W>(Yb);
+    unsigned long long occA = 0, pA = 0, occB = 0, mB = 0, pB = 0;
+    __syncthreads();  // s_tab ready
+
+    for (int zi = (int)z0 - 1; zi <= (int)z1; zi++) {
+        const int par = (zi - (int)z0 + 1) & 1;
+        const bool pz = plane_exists(zi);
+        Group4<W> G = Gn;
+        if (zi < (int)z1) {
+            zero_group<W>(Gn);
+            if (row_exists && plane_exists(zi + 1)) load_group<W>(p, src, row_base(zi + 1) >> 2, Gn);
+        }
+        const bool own_plane = zi >= (int)z0 && zi < (int)z1;
+        const uint32_t base = row_base(zi);
+
+        // ---- 1. x sweep of this work row -------------------------------------
+        uint32_t lw[W], rw[W];
+#pragma unroll
+        for (int q = 0; q < W; q++) {
+            lw[q] = __shfl(G.w[q].w, ll, 64);
+            rw[q] = __shfl(G.w[q].x, lr, 64);
+        }
+        const uint32_t lt = __shfl(G.t >> 24, ll, 64), rt = __shfl(G.t & 0xFFu, lr, 64);
+        if (!hmx) {
+#pragma unroll
+            for (int q = 0; q < W; q++) lw[q] = 0u;
+        }
+        if (!hpx) {
+#pragma unroll
+            for (int q = 0; q < W; q++) rw[q] = 0u;
+        }
+        const unsigned long long occ = (row_exists && pz) ? group_occ<W>(G) : 0ull;
+        unsigned long long xm = 0, xp = 0;
+        if (occ) {
+            const uint32_t tm = (G.t << 8) | lt, tp = (G.t >> 8) | (rt << 24);
+            decide<W>(p, G, tm, tp, hmx ? 0xFu : 0xEu, hpx ? 0xFu : 0x7u, base, h0, occ, xm, xp);
+        }
+        uint32_t lplus = __shfl(site_bits<W>(xp, 3), ll, 64);
+        uint32_t rminus = __shfl(site_bits<W>(xm, 0), lr, 64);
+        if (!hmx) lplus = 0;
+        if (!hpx) rminus = 0;
+        Group4<W> XO;
+        XO.t = G.t;
+        if ((occ | lplus | rminus) == 0) {
+#pragma unroll
+            for (int q = 0; q < W; q++) XO.w[q] = make_uint4(0u, 0u, 0u, 0u);
+        } else {
+            // Overflow of a halo row or halo plane belongs to another tile.
+            const bool rec = owned && own_plane;
+#pragma unroll
+            for (int s = 0; s < 4; s++) {
+                uint32_t wo[W], wm[W], wp[W], o[W];
+                site_words<W>(G, s, wo);
+                const uint32_t own_occ = site_bits<W>(occ, s);
+                const uint32_t stay = own_occ & ~(site_bits<W>(xm, s) | site_bits<W>(xp, s));
+                uint32_t msel, psel;
+                if (s == 0) {
+#pragma unroll
+                    for (int q = 0; q < W; q++) wm[q] = lw[q];
+                    msel = lplus;
+                } else {
+                    site_words<W>(G, s - 1, wm);
+                    msel = site_bits<W>(xp, s - 1);
+                }
+                if (s == 3) {
+#pragma unroll
+                    for (int q = 0; q < W; q++) wp[q] = rw[q];
+                    psel = rminus;
+                } else {
+                    site_words<W>(G, s + 1, wp);
+                    psel = site_bits<W>(xm, s + 1);
+                }
+                assemble_rec<W>(p, base + s, 0, rec, wo, stay, own_occ, wm, msel, wp, psel, o);
+#pragma unroll
+                for (int q = 0; q < W; q++) set_lane(XO.w[q], s, o[q]);
+            }
+        }
+
+        // ---- 2. y sweep through LDS ------------------------------------------
+        if (work) {
+#pragma unroll
+            for (int q = 0; q < W; q++) s_w[par][q][e] = XO.w[q];
+            s_t[par][e] = XO.t;
+        }
+        __syncthreads();
+        const unsigned long long yocc = (row_exists && pz) ? group_occ<W>(XO) : 0ull;
+        unsigned long long ym = 0, yp = 0;
+        if (yocc) {
+            // Row k's y neighbours are work rows k-1 and k+1; the halo rows
+            // only need the direction that enters the tile.
+            const bool hm = k > 0 && (periodic || yl > 0);
+            const bool hp = k <= nrows && (periodic || yl + 1 < (int)Y);
+            const uint32_t tmy = hm ? s_t[par][e - gx] : 0u;
+            const uint32_t tpy = hp ? s_t[par][e + gx] : 0u;
+            decide<W>(p, XO, tmy, tpy, hm ? 0xFu : 0u, hp ? 0xFu : 0u, base, h1, yocc, ym, yp);
+        }
+        if (work) {
+            s_m[par][e] = ym;
+            s_p[par][e] = yp;
+        }
+        __syncthreads();
+
+        if (!owned) continue;  // halo rows and idle lanes have no z state
+        Group4<W> Yc;
+        Yc.t = XO.t;
+        {
+            const bool hm = periodic || yl > 0, hp = periodic || yl + 1 < (int)Y;
+            const unsigned long long pa = hm ? s_p[par][e - gx] : 0ull;
+            const unsigned long long mc = hp ? s_m[par][e + gx] : 0ull;
+            if ((yocc | pa | mc) == 0) {
+#pragma unroll
+                for (int q = 0; q < W; q++) Yc.w[q] = make_uint4(0u, 0u, 0u, 0u);
+            } else {
+                Group4<W> A, C;
+#pragma unroll
+                for (int q = 0; q < W; q++) {
+                    A.w[q] = hm ? s_w[par][q][e - gx] : make_uint4(0u, 0u, 0u, 0u);
+                    C.w[q] = hp ? s_w[par][q][e + gx] : make_uint4(0u, 0u, 0u, 0u);
+                }
+#pragma unroll
+                for (int s = 0; s < 4; s++) {
+                    uint32_t wo[W], wm[W], wp[W], o[W];
+                    site_words<W>(XO, s, wo);
+                    site_words<W>(A, s, wm);
+                    site_words<W>(C, s, wp);
+                    const uint32_t own_occ = site_bits<W>(yocc, s);
+                    const uint32_t stay = own_occ & ~(site_bits<W>(ym, s) | site_bits<W>(yp, s));
+                    assemble_rec<W>(p, base + s, 1, own_plane, wo, stay, own_occ, wm, site_bits<W>(pa, s), wp,
+                                    site_bits<W>(mc, s), o);
+#pragma unroll
+                    for (int q = 0; q < W; q++) set_lane(Yc.w[q], s, o[q]);
+                }
+            }
+        }
+
+        // ---- 3. z decisions of YO(zi) -----------------------------------------
+        const unsigned long long occC = pz ? group_occ<W>(Yc) : 0ull;
+        unsigned long long mC = 0, pC = 0;
+        if (occC) {
+            // YO(z0-1) only sends plus movers in, YO(z1) only minus movers.
+            const bool hm = zi > (int)z0 - 1 && (periodic || zi > 0);
+            const bool hp = zi < (int)z1 && (periodic || zi + 1 < (int)Z);
+            decide<W>(p, Yc, hm ? Yb.t : 0u, hp ? Gn.t : 0u, hm ? 0xFu : 0u, hp ? 0xFu : 0u, base, h2, occC, mC, pC);
+        }
+
+        // ---- 4. z assembly and reaction of plane zi-1 ------------------------
+        const int zo = zi - 1;
+        if (zo >= (int)z0 && zo < (int)z1) {
+            const uint32_t obase = row_base(zo);
+            uint4 res[W];
+            if ((occA | occB | occC) == 0) {
+#pragma unroll
+                for (int q = 0; q < W; q++) res[q] = make_uint4(0u, 0u, 0u, 0u);
+            } else {
+#pragma unroll
+                for (int s = 0; s < 4; s++) {
+                    uint32_t wo[W], wm[W], wp[W], o[W];
+                    site_words<W>(Yb, s, wo);
+                    site_words<W>(Ya, s, wm);
+                    site_words<W>(Yc, s, wp);
+                    const uint32_t own_occ = site_bits<W>(occB, s);
+                    const uint32_t stay = own_occ & ~(site_bits<W>(mB, s) | site_bits<W>(pB, s));
+                    assemble<W>(p, obase + s, 2, wo, stay, own_occ, wm, site_bits<W>(pA, s), wp,
+                                site_bits<W>(mC, s), o);
+#pragma unroll
+                    for (int q = 0; q < W; q++) set_lane(res[q], s, o[q]);
+                }
+            }
+            if (p.R > 0) {
+                uint32_t todo = 0;
+#pragma unroll
+                for (int s = 0; s < 4; s++) {
+                    uint32_t a = 0;
+#pragma unroll
+                    for (int q = 0; q < W; q++) a |= lane_of(res[q], s);
+                    todo |= ((p.any_qp0 || a != 0) ? 1u : 0u) << s;
+                }
+                while (todo) {
+                    const uint32_t s = (uint32_t)__builtin_ctz(todo);
+                    todo &= todo - 1;
+                    uint32_t w[W];
+#pragma unroll
+                    for (int q = 0; q < W; q++) w[q] = lane_dyn(res[q], s);
+                    if (react_site_table<W>(p, obase + s, (Yb.t >> (8 * s)) & 0xFFu, w, h3, tab)) {
+#pragma unroll
+                        for (int q = 0; q < W; q++) set_lane(res[q], (int)s, w[q]);
+                    }
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < W; q++) reinterpret_cast<uint4 *>(dst + (size_t)q * p.N)[obase >> 2] = res[q];
+        }
+        Ya = Yb;
+        occA = occB;
+        pA = pB;
+        Yb = Yc;
+        occB = occC;
+        mB = mC;
+        pB = pC;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Launchers
 // ---------------------------------------------------------------------------
 static inline unsigned blocks_for(size_t n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }
+
+// The fused step needs whole rows in a wave: X % 4 == 0 and X <= 256.
+bool fused_ok(const Params &p) { return (p.X & 3u) == 0 && p.X >= 4 && p.X <= 256; }
+
+// Owned rows per block: (waves x rows per wave) minus the two halo rows.
+uint32_t fused_tile_rows(uint32_t X, uint32_t W)
+{
+    const uint32_t waves = (uint32_t)(W == 4 ? fused_threads<4>() : fused_threads<2>()) / 64u;
+    return waves * (64u / (X >> 2)) - 2u;
+}
+
+hipError_t launch_fused(const Params &p, const uint32_t *src, uint32_t *dst, uint32_t zseg, hipStream_t st)
+{
+    if (!fused_ok(p)) return hipErrorInvalidValue;
+    const uint32_t gx = p.X >> 2;
+    const uint32_t rpw = 64u / gx;
+    const uint32_t Ty = fused_tile_rows(p.X, p.W);
+    const uint32_t nty = (p.Y + Ty - 1) / Ty;
+    if (zseg == 0) zseg = 1;
+    const uint32_t nzs = (p.Z + zseg - 1) / zseg;
+    const dim3 grid(nty * nzs);
+    switch (p.W) {
+    case 1: hipLaunchKernelGGL(fused_step_kernel<1>, grid, dim3(fused_threads<1>()), 0, st, p, src, dst, gx, rpw, Ty, nty, zseg, nzs); break;
+    case 2: hipLaunchKernelGGL(fused_step_kernel<2>, grid, dim3(fused_threads<2>()), 0, st, p, src, dst, gx, rpw, Ty, nty, zseg, nzs); break;
+    case 4: hipLaunchKernelGGL(fused_step_kernel<4>, grid, dim3(fused_threads<4>()), 0, st, p, src, dst, gx, rpw, Ty, nty, zseg, nzs); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
 
 bool fast_path_ok(const Params &p) { return (p.X & 3u) == 0 && p.N >= 4; }
 
@@ -793,7 +1112,7 @@ hipError_t launch_sweep_fast(const Params &p, const uint32_t *src, uint32_t *dst
     const uint32_t sub = (uint32_t)axis;
     // y sweep: the next segment is seg*X sites away, the next outer (z) plane
     // X*Y away; z sweep: the next outer (y) row is X away.
-    const uint32_t seg_first = (size_t)seg * stride < (size_t)ostride ? 1u : 0u;
+    const uint32_t seg_first = 0u;  // measured: outer rows first is faster for both axes
 #define LMX_MARCH(WW, SS) hipLaunchKernelGGL((sweep_march_kernel<WW, SS>), grid, block, 0, st, p, src, dst, stride, len, ostride, nouter, sub, seg_first)
     if (seg == 4) {
         switch (p.W) {

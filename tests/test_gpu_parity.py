@@ -200,3 +200,60 @@ def test_vector_kernels_match_generic_and_oracle(gpu, case):
         got = s.getLatticeNative()
         assert np.array_equal(got, ref), f"{opt}: {(got != ref).sum()} words differ"
         s.close()
+
+
+FUSED = [
+    # shape, P, boundary, density, R
+    ((8, 6, 5), 8, "periodic", 0.3, 7),
+    ((12, 7, 9), 8, "reflecting", 0.3, 7),
+    ((4, 4, 4), 8, "periodic", 0.2, 7),
+    ((4, 33, 3), 4, "reflecting", 0.3, 5),     # one row per lane, several y tiles
+    ((16, 1, 3), 16, "periodic", 0.25, 7),    # Y = 1: both y neighbours are the row itself
+    ((12, 5, 1), 16, "periodic", 0.1, 7),     # Z = 1
+    ((12, 5, 1), 16, "reflecting", 0.1, 7),
+    ((12, 5, 1), 8, "periodic", 0.3, 7),      # overflow search leaves a one-plane lattice: error parity
+    ((32, 40, 20), 4, "reflecting", 0.3, 6),  # rows straddle several tiles
+    ((64, 70, 9), 8, "periodic", 0.05, 9),
+    ((256, 20, 6), 8, "periodic", 0.05, 9),   # one row per wave
+    ((256, 9, 5), 16, "reflecting", 0.1, 9),
+    ((12, 10, 8), 8, "periodic", 0.45, 0),    # overflow heavy
+]
+
+
+@pytest.mark.parametrize("case", FUSED, ids=[f"{c[0]}-P{c[1]}-{c[2]}" for c in FUSED])
+def test_fused_step_matches_oracle(gpu, case):
+    """The fused diffuse+react kernel must give the same lattices, overflow
+    statistics and errors as the separate kernels, for every z segmenting."""
+    shape, P, boundary, density, R = case
+    spec = models.random_model(shape=shape, P=P, nS=5, nST=3, R=R, density=density, seed=23, boundary=boundary,
+                               qmax=0.5)
+    orc = oracle.Oracle(spec, seed_top=6, replicate=1)
+    refs, err = [], 0
+    for _ in range(3):
+        err = orc.run(3)
+        if err:
+            break
+        refs.append(orc.native())
+    for zseg in (0, 1, 2, 3, 1000):
+        s = lmx.MpdRdmeSolver(0)
+        spec.load_into(s, replicate=1, seed=6)
+        s.setPath(lmx._lib.PATH_FUSED)
+        s.setLaunchOption(lmx._lib.OPT_FUSED_ZSEG, zseg)
+        for c, ref in enumerate(refs):
+            s.runTimesteps(3)
+            got = s.getLatticeNative()
+            assert np.array_equal(got, ref), f"zseg={zseg} chunk {c}: {(got != ref).sum()} words differ"
+        if err:
+            s.runTimesteps(3, synchronize=False)
+            with pytest.raises(lmx.LmxError) as ei:
+                s.synchronize()
+            assert ei.value.code == oracle.TO_LMX[err], f"zseg={zseg}"
+        else:
+            e, st = s.getOverflowStats()
+            assert (e, st) == (orc.stats.overflow_entries, orc.stats.overflow_steps), f"zseg={zseg}"
+        s.close()
+
+
+def test_fused_whole_cell_reduced(gpu):
+    spec = models.whole_cell(shape=(64, 64, 32), n_particles=30000, seed=2)
+    run_pair(spec, steps=6, seed=3, chunks=2, path=lmx._lib.PATH_FUSED)
