@@ -112,6 +112,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="whole_cell")
     ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--path", choices=("sweeps", "fused"), default="sweeps",
+                    help="step implementation (results are bit-identical)")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -146,6 +148,8 @@ def main() -> int:
     log(f"workload {spec.name} {spec.shape} N={spec.N} nS={spec.nS} R={spec.R} built in {time.perf_counter() - t0:.1f}s")
     solver = lmx.MpdRdmeSolver(0)
     spec.load_into(solver, replicate=0, seed=args.seed)
+    if args.path == "fused":
+        solver.setPath(lmx._lib.PATH_FUSED)
     log("model uploaded")
     if args.warmup:
         solver.runTimesteps(args.warmup)
@@ -160,7 +164,7 @@ def main() -> int:
     # per-kernel averages and algorithmic bytes per launch
     kernels = {}
     for name, (ms, launches) in per.items():
-        if name.startswith("sweep"):
+        if name.startswith("sweep") or name == "fused":
             b = sweep_bytes_per_site(P) * N
         elif name == "reaction":
             b = reaction_bytes_per_site(P) * N
@@ -175,12 +179,13 @@ def main() -> int:
     tfile = os.path.join(ROOT, "profiles", "traffic.json")
     if os.path.exists(tfile):
         try:
-            traffic = json.load(open(tfile)).get(f"{spec.name}:{dom}")
+            traffic = json.load(open(tfile)).get(f"{spec.name}_{spec.shape[0]}x{spec.shape[1]}x{spec.shape[2]}:{dom}")
         except Exception:
             traffic = None
     roofline = {"bound": "hbm", "kernel": dom, "achieved": dk["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": dk["GBps"] / HBM_PEAK_GBS, "traffic": traffic,
                 "alg_bytes_per_launch": dk["alg_bytes"], "avg_launch_us": dk["avg_us"]}
+    # Unfused-equivalent bytes: three sweeps plus the reaction read.
     step_alg = (3 * sweep_bytes_per_site(P) + (reaction_bytes_per_site(P) if spec.R else 0)) * N
     step_ms = total_ms / K
     out = {
@@ -199,7 +204,7 @@ def main() -> int:
         "config": {"workload": f"{spec.name} {spec.shape[0]}x{spec.shape[1]}x{spec.shape[2]}",
                    "sites": N, "slots_per_site": P, "species": spec.nS, "reactions": spec.R,
                    "site_types": spec.nST, "particles": int(spec.initial_counts().sum()),
-                   "lambda_m": spec.spacing, "tau_s": spec.tau, "path": "sweeps"},
+                   "lambda_m": spec.spacing, "tau_s": spec.tau, "path": args.path},
         "roofline": roofline,
         "step_roofline": {"alg_bytes_per_step": step_alg, "achieved_GBps": step_alg / (step_ms / 1e3) / 1e9,
                           "frac": step_alg / (step_ms / 1e3) / 1e9 / HBM_PEAK_GBS},

@@ -150,6 +150,13 @@ __device__ inline float lookup_T(const Params &p, uint32_t particle, uint32_t sr
     return p.T[(src * p.nST + dst) * p.nS + (particle - 1)];
 }
 
+// The same lookup in a copy of the table (e.g. in LDS).
+__device__ inline float lookup_T_in(const Params &p, const float *Tt, uint32_t particle, uint32_t src, uint32_t dst)
+{
+    if ((uint32_t)(particle - 1) >= p.nS || dst >= p.nST || src >= p.nST) return 0.0f;
+    return Tt[(src * p.nST + dst) * p.nS + (particle - 1)];
+}
+
 // ---------------------------------------------------------------------------
 // Slot packing.  A site's particles are W 32-bit words (slot 4w+b = byte b of
 // word w).  SlotBuf appends particles in order and keeps the first P.

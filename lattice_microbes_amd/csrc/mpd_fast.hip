@@ -24,6 +24,8 @@
 // 0.5 <= u < qp + 0.5; assembly (performPropagation, :199-270): stayers, then
 // the minus neighbour's plus movers, then the plus neighbour's minus movers,
 // each in slot order, first P kept, the rest recorded as overflow.
+#include <type_traits>
+
 #include "lmx_device.h"
 #include "lmx_kernels.h"
 #include "lmx_site.h"
@@ -147,10 +149,11 @@ __device__ inline uint32_t group_slot_dyn(const Group4<W> &G, uint32_t s, uint32
 // plus-neighbour type byte s of tp; hm/hp (bit s) say whether that neighbour
 // exists.  Bit b = s*P + j of minus/plus is set for a particle moving minus /
 // plus.
+// Tt is the transition table (p.T, or its copy in LDS).
 template <int W>
-__device__ inline void decide(const Params &p, const Group4<W> &G, uint32_t tm, uint32_t tp, uint32_t hm,
-                              uint32_t hp, uint32_t base, unsigned long long h, unsigned long long pending,
-                              unsigned long long &minus, unsigned long long &plus)
+__device__ inline void decide_t(const Params &p, const float *Tt, const Group4<W> &G, uint32_t tm, uint32_t tp,
+                                uint32_t hm, uint32_t hp, uint32_t base, unsigned long long h,
+                                unsigned long long pending, unsigned long long &minus, unsigned long long &plus)
 {
     constexpr uint32_t P = 4 * W;
     constexpr uint32_t LOGP = W == 1 ? 2 : (W == 2 ? 3 : 4);
@@ -164,13 +167,21 @@ __device__ inline void decide(const Params &p, const Group4<W> &G, uint32_t tm, 
         const uint32_t stm = (tm >> (8 * s)) & 0xFFu;
         const uint32_t stp = (tp >> (8 * s)) & 0xFFu;
         const float u = uniform01(base + s, p.pcb, j, h);
-        const float qm = ((hm >> s) & 1u) ? lookup_T(p, part, st, stm) : 0.0f;
-        const float qp = ((hp >> s) & 1u) ? lookup_T(p, part, st, stp) : 0.0f;
+        const float qm = ((hm >> s) & 1u) ? lookup_T_in(p, Tt, part, st, stm) : 0.0f;
+        const float qp = ((hp >> s) & 1u) ? lookup_T_in(p, Tt, part, st, stp) : 0.0f;
         if (u < qm) mi |= 1ull << b;
         else if (u >= 0.5f && u < (qp + 0.5f)) pl |= 1ull << b;
     }
     minus = mi;
     plus = pl;
+}
+
+template <int W>
+__device__ inline void decide(const Params &p, const Group4<W> &G, uint32_t tm, uint32_t tp, uint32_t hm,
+                              uint32_t hp, uint32_t base, unsigned long long h, unsigned long long pending,
+                              unsigned long long &minus, unsigned long long &plus)
+{
+    decide_t<W>(p, p.T, G, tm, tp, hm, hp, base, h, pending, minus, plus);
 }
 
 // New content of site i: own stayers, minus-source movers, plus-source movers.
@@ -790,27 +801,43 @@ __global__ void prob_table_kernel(Params p, float *prob1, float *prob0)
 // Halo rows and halo planes are computed redundantly and never recorded as
 // overflow, so every overflow entry (phase, site, emission) is recorded once,
 // by the owner, exactly as the separate kernels record it.
+//
+// Input planes are prefetched one plane step ahead, as the last memory
+// operation of a step: vector memory operations complete in issue order, so
+// any earlier wait (a table or spill load) would otherwise also wait for the
+// prefetch.  The transition and single-particle reaction tables are copied to
+// LDS (dynamic shared memory: T floats, then prob1 float2 pairs).
 // ---------------------------------------------------------------------------
 template <int W>
 constexpr int fused_threads() { return W == 4 ? 512 : 1024; }
 
 template <int W>
+using ymask_t = typename std::conditional<W == 4, unsigned long long, uint32_t>::type;
+
+template <int W>
+constexpr size_t fused_static_lds()
+{
+    return (size_t)fused_threads<W>() * (2 * W * sizeof(uint4) + 2 * sizeof(uint32_t) + 2 * sizeof(ymask_t<W>));
+}
+
+template <int W>
 __global__ __launch_bounds__(fused_threads<W>()) void fused_step_kernel(Params p, const uint32_t *__restrict__ src,
                                                                          uint32_t *__restrict__ dst, uint32_t gx,
                                                                          uint32_t rpw, uint32_t Ty, uint32_t nty,
-                                                                         uint32_t Zs, uint32_t nzs)
+                                                                         uint32_t Zs)
 {
     constexpr int NT = fused_threads<W>();
+    using mask_t = ymask_t<W>;
     __shared__ uint4 s_w[2][W][NT];
     __shared__ uint32_t s_t[2][NT];
-    __shared__ unsigned long long s_m[2][NT], s_p[2][NT];
-    __shared__ float2 s_tab[kTabLds];
+    __shared__ mask_t s_m[NT], s_p[NT];
+    extern __shared__ float s_dyn[];
 
-    const uint32_t ntab = p.nST * p.nS;
-    const bool tab_lds = ntab <= kTabLds;
-    if (tab_lds)
-        for (uint32_t k = threadIdx.x; k < ntab; k += NT) s_tab[k] = reinterpret_cast<const float2 *>(p.prob1)[k];
-    const float2 *tab = tab_lds ? s_tab : reinterpret_cast<const float2 *>(p.prob1);
+    const uint32_t nT = p.nST * p.nST * p.nS, ntab = p.nST * p.nS;
+    float *s_T = s_dyn;
+    float2 *s_tab = reinterpret_cast<float2 *>(s_dyn + ((nT + 1u) & ~1u));
+    for (uint32_t k = threadIdx.x; k < nT; k += NT) s_T[k] = p.T[k];
+    for (uint32_t k = threadIdx.x; k < ntab; k += NT) s_tab[k] = reinterpret_cast<const float2 *>(p.prob1)[k];
 
     const uint32_t X = p.X, Y = p.Y, Z = p.Z, XY = p.XY;
     const bool periodic = p.boundary == 0;
@@ -834,33 +861,33 @@ __global__ __launch_bounds__(fused_threads<W>()) void fused_step_kernel(Params p
     const int ll = lane_ok ? rb + (int)((g + gx - 1) % gx) : lane;
     const int lr = lane_ok ? rb + (int)((g + 1) % gx) : lane;
     const bool hmx = periodic || g > 0, hpx = periodic || g + 1 < gx;
+    // y neighbours of an owned row exist unless a reflecting edge is crossed
+    const bool hmy = periodic || yl > 0, hpy = periodic || yl + 1 < (int)Y;
     const uint32_t t = p.ctl->timestep;
     const unsigned long long h0 = timestep_hash(p.seed32, t, 0), h1 = timestep_hash(p.seed32, t, 1),
                              h2 = timestep_hash(p.seed32, t, 2), h3 = timestep_hash(p.seed32, t, 3);
 
     auto plane_exists = [&](int z) { return periodic || (z >= 0 && z < (int)Z); };
-    auto wrapz = [&](int z) { return (uint32_t)((z + (int)Z) % (int)Z); };
-    auto row_base = [&](int z) { return wrapz(z) * XY + y * X + 4u * g; };
+    auto row_base = [&](int z) { return (uint32_t)((z + (int)Z) % (int)Z) * XY + y * X + 4u * g; };
+    auto fetch = [&](int z, Group4<W> &G) {
+        zero_group<W>(G);
+        if (row_exists && plane_exists(z)) load_group<W>(p, src, row_base(z) >> 2, G);
+    };
 
-    Group4<W> Gn;  // prefetched input row of the next plane
-    zero_group<W>(Gn);
-    if (row_exists && plane_exists((int)z0 - 1)) load_group<W>(p, src, row_base((int)z0 - 1) >> 2, Gn);
+    Group4<W> G, Gn;  // input planes zi and zi+1
+    fetch((int)z0 - 1, G);
+    fetch((int)z0, Gn);
 
     // z window: Ya = YO(zi-2), Yb = YO(zi-1) (+ their z decisions)
     Group4<W> Ya, Yb;
     zero_group<W>(Ya);
     zero_group<W>(Yb);
-    unsigned long long occA = 0, pA = 0, occB = 0, mB = 0, pB = 0;
-    __syncthreads();  // s_tab ready
+    mask_t occA = 0, pA = 0, occB = 0, mB = 0, pB = 0;
+    __syncthreads();  // tables ready
 
     for (int zi = (int)z0 - 1; zi <= (int)z1; zi++) {
         const int par = (zi - (int)z0 + 1) & 1;
         const bool pz = plane_exists(zi);
-        Group4<W> G = Gn;
-        if (zi < (int)z1) {
-            zero_group<W>(Gn);
-            if (row_exists && plane_exists(zi + 1)) load_group<W>(p, src, row_base(zi + 1) >> 2, Gn);
-        }
         const bool own_plane = zi >= (int)z0 && zi < (int)z1;
         const uint32_t base = row_base(zi);
 
@@ -884,7 +911,7 @@ __global__ __launch_bounds__(fused_threads<W>()) void fused_step_kernel(Params p
         unsigned long long xm = 0, xp = 0;
         if (occ) {
             const uint32_t tm = (G.t << 8) | lt, tp = (G.t >> 8) | (rt << 24);
-            decide<W>(p, G, tm, tp, hmx ? 0xFu : 0xEu, hpx ? 0xFu : 0x7u, base, h0, occ, xm, xp);
+            decide_t<W>(p, s_T, G, tm, tp, hmx ? 0xFu : 0xEu, hpx ? 0xFu : 0x7u, base, h0, occ, xm, xp);
         }
         uint32_t lplus = __shfl(site_bits<W>(xp, 3), ll, 64);
         uint32_t rminus = __shfl(site_bits<W>(xm, 0), lr, 64);
@@ -939,25 +966,24 @@ __global__ __launch_bounds__(fused_threads<W>()) void fused_step_kernel(Params p
         if (yocc) {
             // Row k's y neighbours are work rows k-1 and k+1; the halo rows
             // only need the direction that enters the tile.
-            const bool hm = k > 0 && (periodic || yl > 0);
-            const bool hp = k <= nrows && (periodic || yl + 1 < (int)Y);
+            const bool hm = k > 0 && hmy;
+            const bool hp = k <= nrows && hpy;
             const uint32_t tmy = hm ? s_t[par][e - gx] : 0u;
             const uint32_t tpy = hp ? s_t[par][e + gx] : 0u;
-            decide<W>(p, XO, tmy, tpy, hm ? 0xFu : 0u, hp ? 0xFu : 0u, base, h1, yocc, ym, yp);
+            decide_t<W>(p, s_T, XO, tmy, tpy, hm ? 0xFu : 0u, hp ? 0xFu : 0u, base, h1, yocc, ym, yp);
         }
         if (work) {
-            s_m[par][e] = ym;
-            s_p[par][e] = yp;
+            s_m[e] = (mask_t)ym;
+            s_p[e] = (mask_t)yp;
         }
         __syncthreads();
 
-        if (!owned) continue;  // halo rows and idle lanes have no z state
+        if (owned) {  // halo rows and idle lanes have no z state
         Group4<W> Yc;
         Yc.t = XO.t;
         {
-            const bool hm = periodic || yl > 0, hp = periodic || yl + 1 < (int)Y;
-            const unsigned long long pa = hm ? s_p[par][e - gx] : 0ull;
-            const unsigned long long mc = hp ? s_m[par][e + gx] : 0ull;
+            const unsigned long long pa = hmy ? (unsigned long long)s_p[e - gx] : 0ull;
+            const unsigned long long mc = hpy ? (unsigned long long)s_m[e + gx] : 0ull;
             if ((yocc | pa | mc) == 0) {
 #pragma unroll
                 for (int q = 0; q < W; q++) Yc.w[q] = make_uint4(0u, 0u, 0u, 0u);
@@ -965,8 +991,8 @@ __global__ __launch_bounds__(fused_threads<W>()) void fused_step_kernel(Params p
                 Group4<W> A, C;
 #pragma unroll
                 for (int q = 0; q < W; q++) {
-                    A.w[q] = hm ? s_w[par][q][e - gx] : make_uint4(0u, 0u, 0u, 0u);
-                    C.w[q] = hp ? s_w[par][q][e + gx] : make_uint4(0u, 0u, 0u, 0u);
+                    A.w[q] = hmy ? s_w[par][q][e - gx] : make_uint4(0u, 0u, 0u, 0u);
+                    C.w[q] = hpy ? s_w[par][q][e + gx] : make_uint4(0u, 0u, 0u, 0u);
                 }
 #pragma unroll
                 for (int s = 0; s < 4; s++) {
@@ -985,13 +1011,17 @@ __global__ __launch_bounds__(fused_threads<W>()) void fused_step_kernel(Params p
         }
 
         // ---- 3. z decisions of YO(zi) -----------------------------------------
-        const unsigned long long occC = pz ? group_occ<W>(Yc) : 0ull;
-        unsigned long long mC = 0, pC = 0;
+        const mask_t occC = pz ? (mask_t)group_occ<W>(Yc) : (mask_t)0;
+        mask_t mC = 0, pC = 0;
         if (occC) {
             // YO(z0-1) only sends plus movers in, YO(z1) only minus movers.
             const bool hm = zi > (int)z0 - 1 && (periodic || zi > 0);
             const bool hp = zi < (int)z1 && (periodic || zi + 1 < (int)Z);
-            decide<W>(p, Yc, hm ? Yb.t : 0u, hp ? Gn.t : 0u, hm ? 0xFu : 0u, hp ? 0xFu : 0u, base, h2, occC, mC, pC);
+            unsigned long long m64, p64;
+            decide_t<W>(p, s_T, Yc, hm ? Yb.t : 0u, hp ? Gn.t : 0u, hm ? 0xFu : 0u, hp ? 0xFu : 0u, base, h2, occC,
+                        m64, p64);
+            mC = (mask_t)m64;
+            pC = (mask_t)p64;
         }
 
         // ---- 4. z assembly and reaction of plane zi-1 ------------------------
@@ -1032,7 +1062,7 @@ __global__ __launch_bounds__(fused_threads<W>()) void fused_step_kernel(Params p
                     uint32_t w[W];
 #pragma unroll
                     for (int q = 0; q < W; q++) w[q] = lane_dyn(res[q], s);
-                    if (react_site_table<W>(p, obase + s, (Yb.t >> (8 * s)) & 0xFFu, w, h3, tab)) {
+                    if (react_site_table<W>(p, obase + s, (Yb.t >> (8 * s)) & 0xFFu, w, h3, s_tab)) {
 #pragma unroll
                         for (int q = 0; q < W; q++) set_lane(res[q], (int)s, w[q]);
                     }
@@ -1048,6 +1078,10 @@ __global__ __launch_bounds__(fused_threads<W>()) void fused_step_kernel(Params p
         occB = occC;
         mB = mC;
         pB = pC;
+        }
+        // Rotate the input planes; Gn was loaded a whole plane step ago.
+        G = Gn;
+        if (zi + 2 <= (int)z1) fetch(zi + 2, Gn);
     }
 }
 
@@ -1056,8 +1090,26 @@ __global__ __launch_bounds__(fused_threads<W>()) void fused_step_kernel(Params p
 // ---------------------------------------------------------------------------
 static inline unsigned blocks_for(size_t n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }
 
-// The fused step needs whole rows in a wave: X % 4 == 0 and X <= 256.
-bool fused_ok(const Params &p) { return (p.X & 3u) == 0 && p.X >= 4 && p.X <= 256; }
+// Dynamic LDS of the fused step: T, then the single-particle table.
+static size_t fused_dyn_lds(const Params &p)
+{
+    const size_t nT = (size_t)p.nST * p.nST * p.nS;
+    return ((nT + 1) & ~(size_t)1) * sizeof(float) + (size_t)p.nST * p.nS * sizeof(float2);
+}
+
+static size_t fused_total_lds(const Params &p)
+{
+    const size_t st = p.W == 4 ? fused_static_lds<4>() : (p.W == 2 ? fused_static_lds<2>() : fused_static_lds<1>());
+    return st + fused_dyn_lds(p);
+}
+
+// The fused step needs whole rows in a wave (X % 4 == 0, X <= 256) and its
+// tables in LDS next to the row buffers, within gfx950's 160 KB per CU.
+constexpr size_t kFusedLdsMax = 160u * 1024u;
+bool fused_ok(const Params &p)
+{
+    return (p.X & 3u) == 0 && p.X >= 4 && p.X <= 256 && fused_total_lds(p) <= kFusedLdsMax;
+}
 
 // Owned rows per block: (waves x rows per wave) minus the two halo rows.
 uint32_t fused_tile_rows(uint32_t X, uint32_t W)
@@ -1076,10 +1128,19 @@ hipError_t launch_fused(const Params &p, const uint32_t *src, uint32_t *dst, uin
     if (zseg == 0) zseg = 1;
     const uint32_t nzs = (p.Z + zseg - 1) / zseg;
     const dim3 grid(nty * nzs);
+    const size_t dyn = fused_dyn_lds(p);
+    hipError_t e = hipSuccess;
     switch (p.W) {
-    case 1: hipLaunchKernelGGL(fused_step_kernel<1>, grid, dim3(fused_threads<1>()), 0, st, p, src, dst, gx, rpw, Ty, nty, zseg, nzs); break;
-    case 2: hipLaunchKernelGGL(fused_step_kernel<2>, grid, dim3(fused_threads<2>()), 0, st, p, src, dst, gx, rpw, Ty, nty, zseg, nzs); break;
-    case 4: hipLaunchKernelGGL(fused_step_kernel<4>, grid, dim3(fused_threads<4>()), 0, st, p, src, dst, gx, rpw, Ty, nty, zseg, nzs); break;
+    case 1: e = hipFuncSetAttribute(reinterpret_cast<const void *>(fused_step_kernel<1>), hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn); break;
+    case 2: e = hipFuncSetAttribute(reinterpret_cast<const void *>(fused_step_kernel<2>), hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn); break;
+    case 4: e = hipFuncSetAttribute(reinterpret_cast<const void *>(fused_step_kernel<4>), hipFuncAttributeMaxDynamicSharedMemorySize, (int)dyn); break;
+    default: return hipErrorInvalidValue;
+    }
+    if (e != hipSuccess) return e;
+    switch (p.W) {
+    case 1: hipLaunchKernelGGL(fused_step_kernel<1>, grid, dim3(fused_threads<1>()), dyn, st, p, src, dst, gx, rpw, Ty, nty, zseg); break;
+    case 2: hipLaunchKernelGGL(fused_step_kernel<2>, grid, dim3(fused_threads<2>()), dyn, st, p, src, dst, gx, rpw, Ty, nty, zseg); break;
+    case 4: hipLaunchKernelGGL(fused_step_kernel<4>, grid, dim3(fused_threads<4>()), dyn, st, p, src, dst, gx, rpw, Ty, nty, zseg); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
