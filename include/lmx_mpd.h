@@ -116,12 +116,19 @@ int lmx_set_path(lmx_solver *s, int path);
  *   LMX_OPT_MARCH_SEGMENT  rows per thread of the marching y/z sweep: 4, or 8
  *                          (default; any other value selects 8)
  *   LMX_OPT_FORCE_GENERIC  1 = one-site-per-thread kernels even when X%4==0
- *   LMX_OPT_YZ_KERNEL      y/z sweep kernel: 0 = LDS row tiles (default),
- *                          1 = register march
+ *   LMX_OPT_YZ_KERNEL      y/z sweep kernel: 0 = LDS row tiles, one group
+ *                          per thread (default), 1 = register march, 2 = row
+ *                          tiles with compacted work lists
  *   LMX_OPT_FUSED_ZSEG     z planes per block of the fused step; 0 (default)
- *                          picks one block per compute unit */
+ *                          picks one block per compute unit
+ *   LMX_OPT_FUSE_REACTION  1 = the tiled z sweep tests the reaction of its
+ *                          single-particle output sites and queues the
+ *                          rest, and a small kernel evaluates the queued
+ *                          sites (models without zero-order reactions);
+ *                          0 (default) = full reaction kernel */
 enum lmx_option {
-    LMX_OPT_MARCH_SEGMENT = 1, LMX_OPT_FORCE_GENERIC = 2, LMX_OPT_YZ_KERNEL = 3, LMX_OPT_FUSED_ZSEG = 4
+    LMX_OPT_MARCH_SEGMENT = 1, LMX_OPT_FORCE_GENERIC = 2, LMX_OPT_YZ_KERNEL = 3, LMX_OPT_FUSED_ZSEG = 4,
+    LMX_OPT_FUSE_REACTION = 5
 };
 int lmx_set_launch_option(lmx_solver *s, int option, int value);
 

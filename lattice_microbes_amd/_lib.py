@@ -30,6 +30,7 @@ OPT_MARCH_SEGMENT = 1
 OPT_FORCE_GENERIC = 2
 OPT_YZ_KERNEL = 3
 OPT_FUSED_ZSEG = 4
+OPT_FUSE_REACTION = 5
 
 PATH_SWEEPS = 0
 PATH_FUSED = 1

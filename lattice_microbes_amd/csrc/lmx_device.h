@@ -20,6 +20,8 @@ constexpr uint32_t kErrOverflowCap = 3, kErrUnplaceable = 4, kErrOutOfRange = 5,
 
 // Per-solver control block in device memory.  The step counter lives here so
 // that the step sequence does not depend on host-side arguments.
+constexpr int kDeferShards = 64;
+
 struct Control {
     uint32_t timestep;          // advanced by the resolver at the end of each step
     uint32_t ovf_count;         // entries appended this step (may exceed the cap)
@@ -28,6 +30,9 @@ struct Control {
     unsigned long long ovf_total;
     unsigned long long ovf_steps;
     unsigned long long ovf_keys[kMaxOverflows];
+    // Sites queued for the reaction phase by the fused z sweep, one counter
+    // per list shard (shards spread the atomic reservations).
+    uint32_t defer_count[kDeferShards];
 };
 
 // Overflow entry key: canonical order is phase, then site index, then

@@ -10,6 +10,7 @@
 
 #include <time.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstddef>
 #include <cstring>
@@ -105,8 +106,11 @@ struct lmx_solver {
     // launch options
     uint32_t march_seg = 8;
     bool force_generic = false;
-    int yz_kernel = 0;  // 0: LDS tiles, 1: register march
+    int yz_kernel = 0;  // 0: LDS tiles, 1: register march, 2: LDS tiles with compacted work lists
     uint32_t fused_zseg = 0;  // 0: automatic
+    bool fuse_reaction = false;  // reaction test inside the tiled z sweep
+    uint32_t *d_deferred = nullptr;  // sites queued for the reaction phase by that sweep
+    uint32_t defer_cap = 0;
     uint32_t num_cu = 256;
 };
 
@@ -336,18 +340,32 @@ int enqueue_step(lmx_solver *s, const lmx::Params &p, hipEvent_t *ev, int *kinds
         s->timestep++;
         return LMX_OK;
     }
+    // The tiled z sweeps can test the reaction of their output sites and
+    // queue the rest (not with zero-order reactions, which make every site a
+    // candidate).
+    const bool z_reacts = fast && s->yz_kernel != 1 && s->fuse_reaction && s->R > 0 && !s->any_qp0;
     for (int axis = 0; axis < 3; axis++) {
+        const bool react = axis == 2 && z_reacts;
         if (!fast)
             e = lmx::launch_sweep(p, s->d_lat[s->cur], s->d_lat[1 - s->cur], axis, s->stream);
         else if (axis > 0 && s->yz_kernel == 0)
-            e = lmx::launch_sweep_tile(p, s->d_lat[s->cur], s->d_lat[1 - s->cur], axis, s->stream);
+            e = lmx::launch_sweep_tile(p, s->d_lat[s->cur], s->d_lat[1 - s->cur], axis, react, s->d_deferred,
+                                       s->defer_cap, s->stream);
+        else if (axis > 0 && s->yz_kernel == 2)
+            e = lmx::launch_sweep_sparse(p, s->d_lat[s->cur], s->d_lat[1 - s->cur], axis, react, s->d_deferred,
+                                         s->defer_cap, s->stream);
         else
             e = lmx::launch_sweep_fast(p, s->d_lat[s->cur], s->d_lat[1 - s->cur], axis, s->march_seg, s->stream);
         if (e != hipSuccess) return hip_fail(e, "sweep launch");
         s->cur = 1 - s->cur;
         if (int rc = mark(LMX_K_SWEEP_X + axis)) return rc;
     }
-    if (s->R > 0) {
+    if (z_reacts) {
+        e = lmx::launch_reaction_deferred(p, s->d_lat[s->cur], s->d_deferred, s->defer_cap, (int)s->num_cu,
+                                          s->stream);
+        if (e != hipSuccess) return hip_fail(e, "reaction launch");
+        if (int rc = mark(LMX_K_REACTION)) return rc;
+    } else if (s->R > 0) {
         e = fast ? lmx::launch_reaction_fast(p, s->d_lat[s->cur], s->stream)
                  : lmx::launch_reaction(p, s->d_lat[s->cur], s->stream);
         if (e != hipSuccess) return hip_fail(e, "reaction launch");
@@ -366,6 +384,7 @@ void free_device(lmx_solver *s)
     dev_free(s->d_lat[0]);
     dev_free(s->d_lat[1]);
     dev_free(s->d_sites);
+    dev_free(s->d_deferred);
     dev_free(s->d_T);
     dev_free(s->d_qp0);
     dev_free(s->d_qp1);
@@ -579,6 +598,12 @@ int lmx_build_diffusion_model(lmx_solver *s, uint32_t nST, const double *DF, con
     hipError_t e = hipMalloc((void **)&s->d_lat[0], lat_bytes);
     if (e == hipSuccess) e = hipMalloc((void **)&s->d_lat[1], lat_bytes);
     if (e == hipSuccess) e = hipMalloc((void **)&s->d_sites, s->N);
+    // Deferred reaction sites, kDeferShards lists of defer_cap entries: any
+    // capacity is correct (an overflowing list falls back to a full reaction
+    // pass); N/16 in total covers typical densities.
+    s->defer_cap = (uint32_t)(std::max<size_t>(std::min<size_t>(s->N, 65536), s->N / 16) / lmx::kDeferShards + 1);
+    if (e == hipSuccess)
+        e = hipMalloc((void **)&s->d_deferred, (size_t)s->defer_cap * lmx::kDeferShards * sizeof(uint32_t));
     if (e != hipSuccess) {
         free_device(s);
         return fail(LMX_ERR_NOMEM, std::string("Failed to allocate the device lattice: ") + hipGetErrorString(e));
@@ -657,8 +682,12 @@ int lmx_set_launch_option(lmx_solver *s, int option, int value)
         s->force_generic = value != 0;
         return LMX_OK;
     case LMX_OPT_YZ_KERNEL:
-        if (value != 0 && value != 1) return fail(LMX_ERR_ARG, "y/z kernel must be 0 (tiles) or 1 (march)");
+        if (value < 0 || value > 2)
+            return fail(LMX_ERR_ARG, "y/z kernel must be 0 (LDS tiles), 1 (march) or 2 (compacted tiles)");
         s->yz_kernel = value;
+        return LMX_OK;
+    case LMX_OPT_FUSE_REACTION:
+        s->fuse_reaction = value != 0;
         return LMX_OK;
     case LMX_OPT_FUSED_ZSEG:
         if (value < 0 || value > 65536) return fail(LMX_ERR_ARG, "fused z segment must be in 0..65536");

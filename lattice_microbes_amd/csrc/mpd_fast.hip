@@ -528,14 +528,70 @@ __global__ __launch_bounds__(256) void sweep_march_kernel(Params p, const uint32
 // numbered so that the tiles a given XCD (blockIdx % 8) runs are adjacent
 // along the axis, keeping shared apron rows in that XCD's L2.
 // ---------------------------------------------------------------------------
+// This lane's first slot in a list (LDS or global), for per-lane counts
+// 0..7: a ballot prefix inside the wave and one atomic reservation per wave.
+// Every lane of the wave must call it.
+__device__ inline uint32_t wave_reserve(uint32_t *counter, uint32_t c)
+{
+    const unsigned long long b0 = __ballot(c & 1u), b1 = __ballot(c & 2u), b2 = __ballot(c & 4u);
+    const uint32_t lane = __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+    const unsigned long long lt = (1ull << lane) - 1ull;
+    const uint32_t pre = (uint32_t)(__popcll(b0 & lt) + 2 * __popcll(b1 & lt) + 4 * __popcll(b2 & lt));
+    const uint32_t tot = (uint32_t)(__popcll(b0) + 2 * __popcll(b1) + 4 * __popcll(b2));
+    uint32_t base = 0;
+    if (lane == 0 && tot) base = atomicAdd(counter, tot);
+    return __shfl(base, 0, 64) + pre;
+}
+
+// The cheap part of the reaction phase for a site with content w after the
+// z sweep: true if the site must be evaluated by reaction_deferred_kernel,
+// i.e. it holds more than one particle, or one particle whose reaction fires
+// (same table, key and comparison as react_site_table).  Without zero-order
+// reactions an empty site never reacts.
+template <int W>
+__device__ inline bool react_test(const Params &p, uint32_t i, uint32_t st, const uint32_t *w,
+                                  unsigned long long h3, const float2 *tab)
+{
+    uint32_t n = 0, any = 0;
+#pragma unroll
+    for (int k = 0; k < W; k++) {
+        n += (uint32_t)__builtin_popcount(nonzero_bytes(w[k]));
+        any |= w[k];
+    }
+    if (n != 1) return n > 1;
+    const uint32_t first = (any | (any >> 8) | (any >> 16) | (any >> 24)) & 0xFFu;
+    const float2 ap = tab[st * p.nS + (first - 1)];
+    return ap.x != 0.0f && ap.y > 0.0f && uniform01(i, 1, 0, h3) <= ap.y;
+}
+
+// Appends the sites i0 + s (bit s of mask) to this block's shard of the
+// deferred lists (shard b holds entries [b*shard_cap, (b+1)*shard_cap)) with
+// one reservation per wave.  Every lane of the wave must call it.
+__device__ inline void defer_sites(const Params &p, uint32_t *deferred, uint32_t shard_cap, uint32_t i0,
+                                   uint32_t mask)
+{
+    const uint32_t shard = blockIdx.x % (uint32_t)kDeferShards;
+    uint32_t slot = wave_reserve(&p.ctl->defer_count[shard], (uint32_t)__builtin_popcount(mask));
+    uint32_t *list = deferred + (size_t)shard * shard_cap;
+    while (mask) {
+        const uint32_t s = (uint32_t)__builtin_ctz(mask);
+        mask &= mask - 1;
+        if (slot < shard_cap) list[slot] = i0 + s;
+        slot++;
+    }
+}
+
 template <int W>
 constexpr int tile_threads() { return W == 4 ? 512 : 1024; }
 
-template <int W>
+// With REACT (z sweep) each output site also gets the cheap part of the
+// reaction phase (react_test); the sites it queues are evaluated by
+// reaction_deferred_kernel after the sweep.
+template <int W, bool REACT>
 __global__ __launch_bounds__(tile_threads<W>()) void sweep_tile_kernel(
     Params p, const uint32_t *__restrict__ src, uint32_t *__restrict__ dst, uint32_t stride, uint32_t len,
     uint32_t ostride, uint32_t nouter, uint32_t Gx, uint32_t Ty, uint32_t nxchunk, uint32_t ntiles,
-    uint32_t substep, uint32_t tile_first)
+    uint32_t substep, uint32_t tile_first, uint32_t *__restrict__ deferred, uint32_t defer_cap)
 {
     constexpr int kEntries = tile_threads<W>() + 128;
     __shared__ uint4 s_w[W][kEntries];
@@ -638,39 +694,350 @@ __global__ __launch_bounds__(tile_threads<W>()) void sweep_tile_kernel(
         s_m[(nrows + 1) * Gx + lx] = bm;
     }
     __syncthreads();
-    if (!valid) return;
 
-    const unsigned long long pa = s_p[e - Gx], mc = s_m[e + Gx];
-    uint4 *out4[W];
-#pragma unroll
-    for (int k = 0; k < W; k++) out4[k] = reinterpret_cast<uint4 *>(dst + (size_t)k * p.N + base + r * stride);
-    if ((occ | pa | mc) == 0) {
-#pragma unroll
-        for (int k = 0; k < W; k++) *out4[k] = make_uint4(0u, 0u, 0u, 0u);
-        return;
-    }
-    Group4<W> A, C;
-#pragma unroll
-    for (int k = 0; k < W; k++) {
-        A.w[k] = s_w[k][e - Gx];
-        C.w[k] = s_w[k][e + Gx];
-    }
+    const float2 *tab = reinterpret_cast<const float2 *>(p.prob1);
     uint4 res[W];
 #pragma unroll
-    for (int s = 0; s < 4; s++) {
-        uint32_t wo[W], wm[W], wp[W], ov[W];
-        site_words<W>(G, s, wo);
-        site_words<W>(A, s, wm);
-        site_words<W>(C, s, wp);
-        const uint32_t own_occ = site_bits<W>(occ, s);
-        const uint32_t stay = own_occ & ~(site_bits<W>(mi, s) | site_bits<W>(pl, s));
-        assemble<W>(p, base + r * stride + s, substep, wo, stay, own_occ, wm, site_bits<W>(pa, s), wp,
-                    site_bits<W>(mc, s), ov);
+    for (int k = 0; k < W; k++) res[k] = make_uint4(0u, 0u, 0u, 0u);
+    const uint32_t i0 = base + r * stride;
+    if (valid) {
+        const unsigned long long pa = s_p[e - Gx], mc = s_m[e + Gx];
+        if ((occ | pa | mc) != 0) {
+            Group4<W> A, C;
 #pragma unroll
-        for (int k = 0; k < W; k++) set_lane(res[k], s, ov[k]);
+            for (int k = 0; k < W; k++) {
+                A.w[k] = s_w[k][e - Gx];
+                C.w[k] = s_w[k][e + Gx];
+            }
+#pragma unroll
+            for (int s = 0; s < 4; s++) {
+                uint32_t wo[W], wm[W], wp[W], ov[W];
+                site_words<W>(G, s, wo);
+                site_words<W>(A, s, wm);
+                site_words<W>(C, s, wp);
+                const uint32_t own_occ = site_bits<W>(occ, s);
+                const uint32_t stay = own_occ & ~(site_bits<W>(mi, s) | site_bits<W>(pl, s));
+                assemble<W>(p, i0 + s, substep, wo, stay, own_occ, wm, site_bits<W>(pa, s), wp, site_bits<W>(mc, s),
+                            ov);
+#pragma unroll
+                for (int k = 0; k < W; k++) set_lane(res[k], s, ov[k]);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < W; k++) reinterpret_cast<uint4 *>(dst + (size_t)k * p.N + i0)[0] = res[k];
     }
+    if (REACT && p.R > 0) {
+        uint32_t dmask = 0;
+        if (valid) {
+            const unsigned long long h3 = timestep_hash(p.seed32, p.ctl->timestep, 3);
 #pragma unroll
-    for (int k = 0; k < W; k++) *out4[k] = res[k];
+            for (int s = 0; s < 4; s++) {
+                uint32_t w[W];
+#pragma unroll
+                for (int k = 0; k < W; k++) w[k] = lane_of(res[k], s);
+                dmask |= (react_test<W>(p, i0 + s, (G.t >> (8 * s)) & 0xFFu, w, h3, tab) ? 1u : 0u) << s;
+            }
+        }
+        defer_sites(p, deferred, defer_cap, i0, dmask);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// y / z sweep over compacted work lists.
+//
+// Particles occupy a few percent of the sites, so the per-particle work of a
+// sweep (move decisions and slot assembly) runs over block-wide lists rather
+// than over every lane: a lane takes one list entry, so the busy lanes are
+// packed into as few waves as the work needs.
+//   1. Every thread loads two groups of the tile (Gx groups x Ty rows, plus
+//      one group of the apron rows above and below), publishes their site
+//      types and occupancy in LDS and appends its occupied sites to the
+//      occupied list.
+//   2. Each occupied site decides its particles into a per-site decision
+//      word (apron rows only in the direction that enters the tile); owners
+//      list their work sites (occupied, or with an occupied neighbour along
+//      the axis) and store zeros for the others.
+//   3. Each work site assembles its new content and stores it.
+// The particle words themselves stay out of LDS: list entries re-read them
+// (cache hits), which keeps a block at ~32 KB of LDS so that four blocks,
+// each with two groups per thread in flight, share a compute unit.
+//
+// With REACT (z sweep) a work site also runs the cheap part of the reaction
+// phase on its new content: a site holding one particle looks up its
+// probability and tests it; a site whose reaction fires, or which holds more
+// than one particle, is appended to the deferred list, and
+// reaction_deferred_kernel evaluates those sites after the sweep.  The
+// reaction of a site reads only that site's post-z content and type, and a
+// site whose reaction does not fire is left unchanged by a second
+// evaluation, so the result equals a separate reaction pass.  Sites without
+// particles cannot react unless a zero-order reaction exists; the host does
+// not fuse then.
+// ---------------------------------------------------------------------------
+constexpr int kSparseThreads = 512;
+constexpr int kSparseGpt = 2;  // owned groups per thread
+template <int W>
+using dec_t = typename std::conditional<W == 4, uint32_t, uint16_t>::type;
+
+// Bit s set iff site s of the group holds a particle.
+template <int W>
+__device__ inline uint32_t group_nz(const Group4<W> &G)
+{
+    uint32_t m = 0;
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+        uint32_t a = 0;
+#pragma unroll
+        for (int k = 0; k < W; k++) a |= lane_of(G.w[k], s);
+        m |= (a != 0u ? 1u : 0u) << s;
+    }
+    return m;
+}
+
+template <int W>
+__device__ inline void global_site_words(const Params &p, const uint32_t *__restrict__ lat, uint32_t i, uint32_t *w)
+{
+#pragma unroll
+    for (int k = 0; k < W; k++) w[k] = lat[(size_t)k * p.N + i];
+}
+
+template <int W, bool REACT>
+__global__ __launch_bounds__(kSparseThreads) void sweep_sparse_kernel(
+    Params p, const uint32_t *__restrict__ src, uint32_t *__restrict__ dst, uint32_t stride, uint32_t len,
+    uint32_t ostride, uint32_t nouter, uint32_t Gx, uint32_t Ty, uint32_t nxchunk, uint32_t ntiles,
+    uint32_t substep, uint32_t tile_first, uint32_t *__restrict__ deferred, uint32_t defer_cap)
+{
+    constexpr int NT = kSparseThreads;
+    constexpr int GPT = kSparseGpt;
+    constexpr int kOwn = NT * GPT;
+    constexpr int kEnt = kOwn + 128;
+    constexpr uint32_t P = 4 * W;
+    constexpr uint32_t PM = (1u << P) - 1u;
+    using D = dec_t<W>;
+    __shared__ uint32_t s_t[kEnt];
+    __shared__ uint8_t s_nz[kEnt];
+    __shared__ D s_dec[4 * kEnt];
+    __shared__ uint16_t s_occ[4 * kEnt];
+    __shared__ uint16_t s_work[4 * kOwn];
+    __shared__ uint32_t s_cnt[2];
+
+    const uint32_t nblocks = nxchunk * nouter * ntiles;
+    const uint32_t per_xcd = (nblocks + 7) / 8;
+    const uint32_t logical = (blockIdx.x & 7u) * per_xcd + (blockIdx.x >> 3);
+    if (logical >= nblocks) return;  // whole block: no barrier is skipped by part of it
+    // Adjacent logical blocks take the work nearest in memory: the next row
+    // tile when tiles are closer than outer rows (y sweep), else the next
+    // outer row (z sweep).
+    uint32_t tile, xc, o;
+    if (tile_first) {
+        tile = logical % ntiles;
+        const uint32_t rest = logical / ntiles;
+        xc = rest % nxchunk;
+        o = rest / nxchunk;
+    } else {
+        o = logical % nouter;
+        const uint32_t rest = logical / nouter;
+        xc = rest % nxchunk;
+        tile = rest / nxchunk;
+    }
+    const uint32_t tid = threadIdx.x;
+    const uint32_t gx_lim = min(Gx, (p.X >> 2) - xc * Gx);  // groups of this x chunk
+    const uint32_t r0 = tile * Ty;
+    const uint32_t nrows = min(Ty, len - r0);
+    const uint32_t obase = o * ostride + xc * Gx * 4u;
+    const bool periodic = p.boundary == 0;
+    const uint32_t t = p.ctl->timestep;
+
+    // ---- 1. groups, types and occupancy; occupied list -------------------------------
+    Group4<W> G[GPT], A;
+#pragma unroll
+    for (int j = 0; j < GPT; j++) {
+        const uint32_t q = tid + (uint32_t)j * NT;
+        const uint32_t lx = q % Gx, ly = q / Gx;
+        zero_group<W>(G[j]);
+        if (lx < gx_lim && ly < nrows) load_group<W>(p, src, (obase + lx * 4u + (r0 + ly) * stride) >> 2, G[j]);
+    }
+    // Apron rows: threads [0, Gx) the row above the tile, [Gx, 2Gx) the row below.
+    const bool top = tid < Gx, bot = tid >= Gx && tid < 2 * Gx;
+    const uint32_t alx = top ? tid : tid - Gx;
+    const bool apron = (top || bot) && alx < gx_lim;
+    zero_group<W>(A);
+    if (apron) load_row<W>(p, src, obase + alx * 4u, stride, len, top ? (int)r0 - 1 : (int)(r0 + nrows), A);
+    if (tid < 2) s_cnt[tid] = 0u;
+    __syncthreads();  // counters cleared
+
+#pragma unroll
+    for (int j = 0; j <= GPT; j++) {
+        const bool is_apron = j == GPT;
+        const Group4<W> &g = is_apron ? A : G[j];
+        uint32_t e = 0;
+        bool have;
+        if (is_apron) {
+            have = apron;
+            e = top ? alx : (nrows + 1) * Gx + alx;
+        } else {
+            const uint32_t q = tid + (uint32_t)j * NT;
+            const uint32_t lx = q % Gx, ly = q / Gx;
+            have = lx < gx_lim && ly < nrows;
+            e = (ly + 1) * Gx + lx;
+        }
+        uint32_t nz = 0;
+        if (have) {
+            s_t[e] = g.t;
+            nz = group_nz<W>(g);
+            s_nz[e] = (uint8_t)nz;
+        }
+        uint32_t slot = wave_reserve(&s_cnt[0], (uint32_t)__builtin_popcount(nz));
+        while (nz) {
+            const uint32_t sb = (uint32_t)__builtin_ctz(nz);
+            nz &= nz - 1;
+            s_occ[slot++] = (uint16_t)(e * 4u + sb);
+        }
+    }
+    const float2 *tab = reinterpret_cast<const float2 *>(p.prob1);
+    __syncthreads();
+
+    // ---- 2a. decisions of the occupied sites ----------------------------------------
+    const unsigned long long h = timestep_hash(p.seed32, t, substep);
+    const uint32_t nocc = s_cnt[0];
+    for (uint32_t idx = tid; idx < nocc; idx += NT) {
+        const uint32_t c = s_occ[idx];
+        const uint32_t e = c >> 2, sb = c & 3u;
+        const uint32_t lx = e % Gx, k = e / Gx;
+        const int r = (int)(r0 + k) - 1;
+        const uint32_t rr = (uint32_t)((r + (int)len) % (int)len);
+        const uint32_t i = obase + lx * 4u + sb + rr * stride;
+        uint32_t w[W];
+        global_site_words<W>(p, src, i, w);
+        // The apron rows only need the direction that enters the tile.
+        const bool hm = k > 0 && (periodic || r > 0);
+        const bool hp = k <= nrows && (periodic || r + 1 < (int)len);
+        const uint32_t st = (s_t[e] >> (8 * sb)) & 0xFFu;
+        const uint32_t stm = hm ? (s_t[e - Gx] >> (8 * sb)) & 0xFFu : 0u;
+        const uint32_t stp = hp ? (s_t[e + Gx] >> (8 * sb)) & 0xFFu : 0u;
+        uint32_t m = site_occ<W>(w), mi = 0, pl = 0;
+        while (m) {
+            const uint32_t jb = (uint32_t)__builtin_ctz(m);
+            m &= m - 1;
+            const uint32_t part = slot_dyn<W>(w, jb);
+            const float u = uniform01(i, p.pcb, jb, h);
+            const float qm = hm ? lookup_T(p, part, st, stm) : 0.0f;
+            const float qp = hp ? lookup_T(p, part, st, stp) : 0.0f;
+            if (u < qm) mi |= 1u << jb;
+            else if (u >= 0.5f && u < (qp + 0.5f)) pl |= 1u << jb;
+        }
+        s_dec[c] = (D)(mi | (pl << P));
+    }
+
+    // ---- 2b. work list; zeros for the sites nothing can reach -----------------------
+#pragma unroll
+    for (int j = 0; j < GPT; j++) {
+        const uint32_t q = tid + (uint32_t)j * NT;
+        const uint32_t lx = q % Gx, ly = q / Gx;
+        const uint32_t e = (ly + 1) * Gx + lx;
+        uint32_t wk = 0;
+        if (lx < gx_lim && ly < nrows) {
+            wk = (uint32_t)(s_nz[e] | s_nz[e - Gx] | s_nz[e + Gx]);
+            const uint32_t i0 = obase + lx * 4u + (r0 + ly) * stride;
+            if (wk == 0) {
+#pragma unroll
+                for (int k = 0; k < W; k++)
+                    reinterpret_cast<uint4 *>(dst + (size_t)k * p.N)[i0 >> 2] = make_uint4(0u, 0u, 0u, 0u);
+            } else {
+#pragma unroll
+                for (int sb = 0; sb < 4; sb++)
+                    if (!((wk >> sb) & 1u)) {
+#pragma unroll
+                        for (int k = 0; k < W; k++) dst[(size_t)k * p.N + i0 + sb] = 0u;
+                    }
+            }
+        }
+        uint32_t slot = wave_reserve(&s_cnt[1], (uint32_t)__builtin_popcount(wk));
+        while (wk) {
+            const uint32_t sb = (uint32_t)__builtin_ctz(wk);
+            wk &= wk - 1;
+            s_work[slot++] = (uint16_t)(e * 4u + sb);
+        }
+    }
+    __syncthreads();
+
+    // ---- 3. assembly of the work sites (and the reaction test) ---------------------
+    const uint32_t nwork = s_cnt[1];
+    const unsigned long long h3 = REACT ? timestep_hash(p.seed32, t, 3) : 0ull;
+    // Uniform trip count, so that every lane reaches the wave-wide deferral.
+    for (uint32_t base0 = 0; base0 < nwork; base0 += NT) {
+        const uint32_t idx = base0 + tid;
+        if (idx >= nwork) {
+            if (REACT && p.R > 0) defer_sites(p, deferred, defer_cap, 0u, 0u);
+            continue;
+        }
+        const uint32_t c = s_work[idx];
+        const uint32_t e = c >> 2, sb = c & 3u;
+        const uint32_t lx = e % Gx, k = e / Gx;
+        const uint32_t r = r0 + k - 1;
+        const uint32_t i = obase + lx * 4u + sb + r * stride;
+        const uint32_t nzo = s_nz[e], nzm = s_nz[e - Gx], nzp = s_nz[e + Gx];
+        uint32_t wo[W], wm[W], wp[W], out[W];
+#pragma unroll
+        for (int q = 0; q < W; q++) wo[q] = wm[q] = wp[q] = 0u;
+        if ((nzo >> sb) & 1u) global_site_words<W>(p, src, i, wo);
+        if ((nzm >> sb) & 1u) {
+            const uint32_t rm = r > 0 ? r - 1 : len - 1;  // the apron row wraps; never read when absent
+            global_site_words<W>(p, src, obase + lx * 4u + sb + rm * stride, wm);
+        }
+        if ((nzp >> sb) & 1u) {
+            const uint32_t rp = r + 1 < len ? r + 1 : 0u;
+            global_site_words<W>(p, src, obase + lx * 4u + sb + rp * stride, wp);
+        }
+        const uint32_t own_occ = site_occ<W>(wo);
+        const uint32_t dec_o = own_occ ? (uint32_t)s_dec[c] : 0u;
+        const uint32_t dec_m = site_occ<W>(wm) ? (uint32_t)s_dec[c - 4u * Gx] : 0u;
+        const uint32_t dec_p = site_occ<W>(wp) ? (uint32_t)s_dec[c + 4u * Gx] : 0u;
+        const uint32_t stay = own_occ & ~((dec_o & PM) | (dec_o >> P));
+        assemble<W>(p, i, substep, wo, stay, own_occ, wm, dec_m >> P, wp, dec_p & PM, out);
+#pragma unroll
+        for (int q = 0; q < W; q++) dst[(size_t)q * p.N + i] = out[q];
+        if (REACT && p.R > 0)
+            defer_sites(p, deferred, defer_cap, i, react_test<W>(p, i, (s_t[e] >> (8 * sb)) & 0xFFu, out, h3, tab) ? 1u : 0u);
+    }
+}
+
+// Reaction phase for the sites the fused z sweep deferred: one evaluation
+// per listed site, or, when a list shard overflowed, a full pass over every
+// occupied site (sites whose reaction does not fire are left unchanged).
+// Block b serves shard b % kDeferShards; the resolver clears the counters.
+template <int W>
+__global__ __launch_bounds__(256) void reaction_deferred_kernel(Params p, uint32_t *__restrict__ lat,
+                                                                const uint32_t *__restrict__ deferred,
+                                                                uint32_t shard_cap)
+{
+    bool full = false;
+    for (int k = 0; k < kDeferShards; k++) full |= p.ctl->defer_count[k] > shard_cap;
+    const unsigned long long h3 = timestep_hash(p.seed32, p.ctl->timestep, 3);
+    const float2 *tab = reinterpret_cast<const float2 *>(p.prob1);
+    if (full) {
+        for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < p.N; i += gridDim.x * blockDim.x) {
+            uint32_t w[W];
+            global_site_words<W>(p, lat, i, w);
+            if (site_occ<W>(w) && react_site_table<W>(p, i, p.sites[i], w, h3, tab)) {
+#pragma unroll
+                for (int q = 0; q < W; q++) lat[(size_t)q * p.N + i] = w[q];
+            }
+        }
+        return;
+    }
+    const uint32_t shard = blockIdx.x % (uint32_t)kDeferShards, part = blockIdx.x / (uint32_t)kDeferShards;
+    const uint32_t nparts = gridDim.x / (uint32_t)kDeferShards;
+    const uint32_t n = p.ctl->defer_count[shard];
+    const uint32_t *list = deferred + (size_t)shard * shard_cap;
+    for (uint32_t k = part * blockDim.x + threadIdx.x; k < n; k += nparts * blockDim.x) {
+        const uint32_t i = list[k];
+        uint32_t w[W];
+        global_site_words<W>(p, lat, i, w);
+        if (react_site_table<W>(p, i, p.sites[i], w, h3, tab)) {
+#pragma unroll
+            for (int q = 0; q < W; q++) lat[(size_t)q * p.N + i] = w[q];
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1195,7 +1562,8 @@ hipError_t launch_sweep_fast(const Params &p, const uint32_t *src, uint32_t *dst
 }
 
 template <int W>
-static hipError_t launch_tile(const Params &p, const uint32_t *src, uint32_t *dst, int axis, hipStream_t st)
+static hipError_t launch_tile(const Params &p, const uint32_t *src, uint32_t *dst, int axis, bool react,
+                              uint32_t *deferred, uint32_t defer_cap, hipStream_t st)
 {
     const uint32_t stride = axis == 1 ? p.X : p.XY;
     const uint32_t len = axis == 1 ? p.Y : p.Z;
@@ -1208,20 +1576,82 @@ static hipError_t launch_tile(const Params &p, const uint32_t *src, uint32_t *ds
     const uint32_t ntiles = (len + Ty - 1) / Ty;
     const uint32_t nblocks = nxchunk * nouter * ntiles;
     const uint32_t per_xcd = (nblocks + 7) / 8;
-    hipLaunchKernelGGL(sweep_tile_kernel<W>, dim3(per_xcd * 8), dim3(Gx * Ty), 0, st, p, src, dst, stride, len,
-                       ostride, nouter, Gx, Ty, nxchunk, ntiles, (uint32_t)axis,
-                       (size_t)Ty * stride < (size_t)ostride ? 1u : 0u);
+    const uint32_t tile_first = (size_t)Ty * stride < (size_t)ostride ? 1u : 0u;
+    if (react)
+        hipLaunchKernelGGL((sweep_tile_kernel<W, true>), dim3(per_xcd * 8), dim3(Gx * Ty), 0, st, p, src, dst,
+                           stride, len, ostride, nouter, Gx, Ty, nxchunk, ntiles, (uint32_t)axis, tile_first, deferred,
+                           defer_cap);
+    else
+        hipLaunchKernelGGL((sweep_tile_kernel<W, false>), dim3(per_xcd * 8), dim3(Gx * Ty), 0, st, p, src, dst,
+                           stride, len, ostride, nouter, Gx, Ty, nxchunk, ntiles, (uint32_t)axis, tile_first, deferred,
+                           defer_cap);
     return hipGetLastError();
 }
 
-hipError_t launch_sweep_tile(const Params &p, const uint32_t *src, uint32_t *dst, int axis, hipStream_t st)
+hipError_t launch_sweep_tile(const Params &p, const uint32_t *src, uint32_t *dst, int axis, bool react,
+                             uint32_t *deferred, uint32_t defer_cap, hipStream_t st)
 {
+    if (axis < 1 || axis > 2 || (react && (axis != 2 || p.any_qp0 || !deferred))) return hipErrorInvalidValue;
     switch (p.W) {
-    case 1: return launch_tile<1>(p, src, dst, axis, st);
-    case 2: return launch_tile<2>(p, src, dst, axis, st);
-    case 4: return launch_tile<4>(p, src, dst, axis, st);
+    case 1: return launch_tile<1>(p, src, dst, axis, react, deferred, defer_cap, st);
+    case 2: return launch_tile<2>(p, src, dst, axis, react, deferred, defer_cap, st);
+    case 4: return launch_tile<4>(p, src, dst, axis, react, deferred, defer_cap, st);
     }
     return hipErrorInvalidValue;
+}
+
+template <int W>
+static hipError_t launch_sparse(const Params &p, const uint32_t *src, uint32_t *dst, int axis, bool react,
+                                uint32_t *deferred, uint32_t defer_cap, hipStream_t st)
+{
+    const uint32_t stride = axis == 1 ? p.X : p.XY;
+    const uint32_t len = axis == 1 ? p.Y : p.Z;
+    const uint32_t ostride = axis == 1 ? p.XY : p.X;
+    const uint32_t nouter = axis == 1 ? p.Z : p.Y;
+    const uint32_t gxn = p.X >> 2;
+    const uint32_t Gx = gxn < 64u ? gxn : 64u;
+    const uint32_t Ty = (uint32_t)(kSparseThreads * kSparseGpt) / Gx;
+    const uint32_t nxchunk = (gxn + Gx - 1) / Gx;
+    const uint32_t ntiles = (len + Ty - 1) / Ty;
+    const uint32_t nblocks = nxchunk * nouter * ntiles;
+    const uint32_t per_xcd = (nblocks + 7) / 8;
+    const uint32_t tile_first = (size_t)Ty * stride < (size_t)ostride ? 1u : 0u;
+    if (react)
+        hipLaunchKernelGGL((sweep_sparse_kernel<W, true>), dim3(per_xcd * 8), dim3(kSparseThreads), 0, st, p, src,
+                           dst, stride, len, ostride, nouter, Gx, Ty, nxchunk, ntiles, (uint32_t)axis, tile_first,
+                           deferred, defer_cap);
+    else
+        hipLaunchKernelGGL((sweep_sparse_kernel<W, false>), dim3(per_xcd * 8), dim3(kSparseThreads), 0, st, p, src,
+                           dst, stride, len, ostride, nouter, Gx, Ty, nxchunk, ntiles, (uint32_t)axis, tile_first,
+                           deferred, defer_cap);
+    return hipGetLastError();
+}
+
+hipError_t launch_sweep_sparse(const Params &p, const uint32_t *src, uint32_t *dst, int axis, bool react,
+                               uint32_t *deferred, uint32_t defer_cap, hipStream_t st)
+{
+    if (axis < 1 || axis > 2 || (react && (axis != 2 || p.any_qp0 || !deferred))) return hipErrorInvalidValue;
+    switch (p.W) {
+    case 1: return launch_sparse<1>(p, src, dst, axis, react, deferred, defer_cap, st);
+    case 2: return launch_sparse<2>(p, src, dst, axis, react, deferred, defer_cap, st);
+    case 4: return launch_sparse<4>(p, src, dst, axis, react, deferred, defer_cap, st);
+    }
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_reaction_deferred(const Params &p, uint32_t *lat, const uint32_t *deferred, uint32_t defer_cap,
+                                    int num_cu, hipStream_t st)
+{
+    // A multiple of the shard count: each shard is served by the same number of blocks.
+    const unsigned parts = (unsigned)((num_cu > 0 ? num_cu : 64) * 4 + kDeferShards - 1) / kDeferShards;
+    const dim3 grid(parts * kDeferShards), block(256);
+    switch (p.W) {
+    case 1: hipLaunchKernelGGL(reaction_deferred_kernel<1>, grid, block, 0, st, p, lat, deferred, defer_cap); break;
+    case 2: hipLaunchKernelGGL(reaction_deferred_kernel<2>, grid, block, 0, st, p, lat, deferred, defer_cap); break;
+    case 4: hipLaunchKernelGGL(reaction_deferred_kernel<4>, grid, block, 0, st, p, lat, deferred, defer_cap); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
 }
 
 hipError_t launch_reaction_fast(const Params &p, uint32_t *lat, hipStream_t st)

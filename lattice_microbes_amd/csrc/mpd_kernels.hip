@@ -122,6 +122,8 @@ __global__ __launch_bounds__(kResolveThreads) void resolve_kernel(Params p, uint
     if (tid == 0) s_t = ctl->timestep;
     const uint32_t n = ctl->ovf_count;
     __syncthreads();
+    // The step's reaction phase has consumed the deferred-site lists.
+    if (tid < kDeferShards) ctl->defer_count[tid] = 0u;
     const uint32_t t = s_t;
     if (n == 0) {
         if (tid == 0) ctl->timestep = t + 1;

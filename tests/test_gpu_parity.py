@@ -180,19 +180,25 @@ ALIGNED = [
 
 @pytest.mark.parametrize("case", ALIGNED, ids=[f"{c[0]}-P{c[1]}-{c[2]}" for c in ALIGNED])
 def test_vector_kernels_match_generic_and_oracle(gpu, case):
-    """X % 4 == 0 selects the 4-sites-per-thread kernels; they must agree with
-    the one-site-per-thread kernels and the oracle for every march segment."""
+    """X % 4 == 0 selects the 4-sites-per-thread kernels; every y/z kernel
+    (compacted tiles with and without the fused reaction, dense tiles, the
+    march for every segment) must agree with the one-site-per-thread kernels
+    and the oracle."""
     shape, P, boundary, density = case
     spec = models.random_model(shape=shape, P=P, nS=5, nST=3, R=7, density=density, seed=17, boundary=boundary)
     orc = oracle.Oracle(spec, seed_top=5, replicate=2)
     assert orc.run(6) == 0
     ref = orc.native()
-    for opt in ({"generic": 1}, {"tile": 1}, {"seg": 1}, {"seg": 3}, {"seg": 7}, {"seg": 32},
-                {"seg": 100000}):
+    for opt in ({"generic": 1}, {"sparse": 1}, {"sparse": 1, "fuse": 1}, {"fuse": 1}, {"seg": 1}, {"seg": 3},
+                {"seg": 7}, {"seg": 32}, {"seg": 100000}):
         s = lmx.MpdRdmeSolver(0)
         spec.load_into(s, replicate=2, seed=5)
         if "generic" in opt:
             s.setLaunchOption(lmx._lib.OPT_FORCE_GENERIC, 1)
+        if "sparse" in opt:
+            s.setLaunchOption(lmx._lib.OPT_YZ_KERNEL, 2)
+        if "fuse" in opt:
+            s.setLaunchOption(lmx._lib.OPT_FUSE_REACTION, 1)
         elif "seg" in opt:
             s.setLaunchOption(lmx._lib.OPT_YZ_KERNEL, 1)
             s.setLaunchOption(lmx._lib.OPT_MARCH_SEGMENT, min(opt["seg"], 65536))
