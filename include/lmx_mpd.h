@@ -181,6 +181,32 @@ int lmx_get_table_f32(lmx_solver *s, int which, float *out, size_t n, size_t *wr
 int lmx_device_exp(int device, const double *x, double *y, size_t n);
 int lmx_device_reaction_probability(int device, const float *a, float *prob, size_t n);
 
+/* ---- Ensembles -----------------------------------------------------------
+ * No reference counterpart: the reference runs every replicate in its own
+ * thread or process (src/core/ReplicateRunner.cpp, src/lm/main/Main.cpp).
+ * A batch steps many independently configured solvers on one device with one
+ * launch per kernel class per step (replayed from a captured HIP graph by
+ * default); members may differ in lattice size, model, seed and step count.
+ * Every member's lattice is bit-identical to running that solver alone.
+ * Members need X % 4 == 0 and the same particles per site.  A solver belongs
+ * to at most one batch and leaves it when destroyed; its own calls stay
+ * valid between batch runs (they are ordered after the batch's work). */
+typedef struct lmx_batch lmx_batch;
+int lmx_batch_create(int device, lmx_batch **out);
+int lmx_batch_destroy(lmx_batch *b);
+int lmx_batch_add(lmx_batch *b, lmx_solver *s);
+int lmx_batch_remove(lmx_batch *b, lmx_solver *s);
+int lmx_batch_size(const lmx_batch *b, uint32_t *n);
+/* LMX_BATCH_GRAPH: 1 (default) = replay a captured graph of one step. */
+enum lmx_batch_option { LMX_BATCH_GRAPH = 1 };
+int lmx_batch_set_option(lmx_batch *b, int option, int value);
+/* Advances every member by nsteps (asynchronous; errors are reported by
+ * lmx_batch_synchronize, or by a member's own calls). */
+int lmx_batch_run_steps(lmx_batch *b, uint32_t nsteps);
+int lmx_batch_synchronize(lmx_batch *b);
+/* lmx_batch_run_steps + synchronize, timed with HIP events (milliseconds). */
+int lmx_batch_time_steps(lmx_batch *b, uint32_t nsteps, double *total_ms);
+
 #ifdef __cplusplus
 }
 #endif

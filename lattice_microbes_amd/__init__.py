@@ -9,7 +9,8 @@ solver class, and generators for the benchmark models.
 from ._lib import (LmxError, InvalidArgException, OverflowException, device_count, library_path,
                    load as load_library)
 from .solver import MESolverFactory, MpdRdmeSolver, SOLVER_NAME, timestep_seed
+from .ensemble import Ensemble
 from . import models
 
 __all__ = ["LmxError", "InvalidArgException", "OverflowException", "device_count", "library_path",
-           "load_library", "MESolverFactory", "MpdRdmeSolver", "SOLVER_NAME", "timestep_seed", "models"]
+           "load_library", "MESolverFactory", "MpdRdmeSolver", "SOLVER_NAME", "timestep_seed", "Ensemble", "models"]

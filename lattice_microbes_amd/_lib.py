@@ -31,6 +31,7 @@ OPT_FORCE_GENERIC = 2
 OPT_YZ_KERNEL = 3
 OPT_FUSED_ZSEG = 4
 OPT_FUSE_REACTION = 5
+BATCH_GRAPH = 1
 
 PATH_SWEEPS = 0
 PATH_FUSED = 1
@@ -73,6 +74,15 @@ SIGNATURES = {
     "lmx_get_species_counts": (c_int, [c_void_p, _u32p, c_uint32]),
     "lmx_get_overflow_stats": (c_int, [c_void_p, _u64p, _u64p]),
     "lmx_time_steps": (c_int, [c_void_p, c_uint32, _dp, _dp, _u64p, c_uint32]),
+    "lmx_batch_create": (c_int, [c_int, POINTER(c_void_p)]),
+    "lmx_batch_destroy": (c_int, [c_void_p]),
+    "lmx_batch_add": (c_int, [c_void_p, c_void_p]),
+    "lmx_batch_remove": (c_int, [c_void_p, c_void_p]),
+    "lmx_batch_size": (c_int, [c_void_p, POINTER(c_uint32)]),
+    "lmx_batch_set_option": (c_int, [c_void_p, c_int, c_int]),
+    "lmx_batch_run_steps": (c_int, [c_void_p, c_uint32]),
+    "lmx_batch_synchronize": (c_int, [c_void_p]),
+    "lmx_batch_time_steps": (c_int, [c_void_p, c_uint32, _dp]),
     "lmx_get_table_f32": (c_int, [c_void_p, c_int, _fp, c_size_t, POINTER(c_size_t)]),
     "lmx_device_exp": (c_int, [c_int, _dp, _dp, c_size_t]),
     "lmx_device_reaction_probability": (c_int, [c_int, _fp, _fp, c_size_t]),

@@ -8,6 +8,7 @@
 namespace lmx {
 
 struct Params;
+struct Member;
 
 hipError_t launch_sweep(const Params &p, const uint32_t *src, uint32_t *dst, int axis, hipStream_t st);
 hipError_t launch_reaction(const Params &p, uint32_t *lat, hipStream_t st);
@@ -26,6 +27,15 @@ uint32_t fused_tile_rows(uint32_t X, uint32_t W);
 hipError_t launch_fused(const Params &p, const uint32_t *src, uint32_t *dst, uint32_t zseg, hipStream_t st);
 hipError_t launch_reaction_fast(const Params &p, uint32_t *lat, hipStream_t st);
 hipError_t launch_prob_tables(const Params &p, float *prob1, float *prob0, hipStream_t st);
+uint32_t x4_blocks(const Params &p);
+uint32_t reaction_blocks(const Params &p);
+// One batched step without the resolver: x, y, z sweeps and (if total[3] > 0)
+// the reaction phase; start_* are device prefix arrays of n+1 block offsets,
+// total[] the four grid sizes.
+hipError_t launch_batch_sweeps(const Member *m, const uint32_t *start_x, const uint32_t *start_y,
+                               const uint32_t *start_z, const uint32_t *start_r, uint32_t n, const uint32_t *total,
+                               uint32_t W, hipStream_t st);
+hipError_t launch_batch_resolve(Member *m, uint32_t n, hipStream_t st);
 hipError_t launch_counts(const uint32_t *lat, size_t nwords, uint32_t nS, uint32_t *counts, hipStream_t st);
 hipError_t launch_exp(const double *x, double *y, size_t n, hipStream_t st);
 hipError_t launch_prob(const float *a, float *y, size_t n, hipStream_t st);

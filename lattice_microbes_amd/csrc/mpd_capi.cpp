@@ -17,6 +17,7 @@
 #include <string>
 #include <vector>
 
+#include "lmx_batch.h"
 #include "lmx_device.h"
 #include "lmx_kernels.h"
 
@@ -107,6 +108,7 @@ struct lmx_solver {
     uint32_t march_seg = 8;
     bool force_generic = false;
     int yz_kernel = 0;  // 0: LDS tiles, 1: register march, 2: LDS tiles with compacted work lists
+    lmx_batch *batch = nullptr;  // ensemble this solver is a member of, if any
     uint32_t fused_zseg = 0;  // 0: automatic
     bool fuse_reaction = false;  // reaction test inside the tiled z sweep
     uint32_t *d_deferred = nullptr;  // sites queued for the reaction phase by that sweep
@@ -450,6 +452,7 @@ int lmx_create(int device, lmx_solver **out)
 int lmx_destroy(lmx_solver *s)
 {
     if (!s) return LMX_OK;
+    if (s->batch) lmx_batch_remove(s->batch, s);
     hipSetDevice(s->device);
     if (s->stream) hipStreamSynchronize(s->stream);
     free_device(s);
@@ -961,3 +964,273 @@ int lmx_device_reaction_probability(int device, const float *a, float *prob, siz
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Ensembles
+// ---------------------------------------------------------------------------
+struct lmx_batch {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t done = nullptr;
+    std::vector<lmx_solver *> members;
+    uint32_t W = 0;
+    bool use_graph = true;
+    // device descriptors and block prefixes (x, y, z, reaction), sized for cap members
+    uint32_t cap = 0;
+    lmx::Member *d_members = nullptr, *h_members = nullptr;
+    uint32_t *d_start = nullptr, *h_start = nullptr;
+    uint32_t total[4] = {0, 0, 0, 0};
+    // captured step, valid for the member list and totals it was made with
+    hipGraphExec_t exec = nullptr;
+    std::vector<lmx_solver *> graph_members;
+    uint32_t graph_total[4] = {0, 0, 0, 0};
+};
+
+namespace {
+
+void batch_drop_graph(lmx_batch *b)
+{
+    if (b->exec) hipGraphExecDestroy(b->exec);
+    b->exec = nullptr;
+    b->graph_members.clear();
+}
+
+// Descriptors of every member from its host-side state (buffer parity,
+// tables); the device copy is refreshed before each run.
+int batch_prepare(lmx_batch *b)
+{
+    const uint32_t n = (uint32_t)b->members.size();
+    if (n > b->cap) {
+        uint32_t cap = b->cap ? b->cap : 16;
+        while (cap < n) cap *= 2;
+        dev_free(b->d_members);
+        dev_free(b->d_start);
+        if (b->h_members) hipHostFree(b->h_members);
+        if (b->h_start) hipHostFree(b->h_start);
+        b->h_members = nullptr;
+        b->h_start = nullptr;
+        b->cap = 0;
+        LMX_HIP(hipMalloc((void **)&b->d_members, sizeof(lmx::Member) * cap));
+        LMX_HIP(hipMalloc((void **)&b->d_start, sizeof(uint32_t) * 4 * (cap + 1)));
+        LMX_HIP(hipHostMalloc((void **)&b->h_members, sizeof(lmx::Member) * cap, hipHostMallocDefault));
+        LMX_HIP(hipHostMalloc((void **)&b->h_start, sizeof(uint32_t) * 4 * (cap + 1), hipHostMallocDefault));
+        b->cap = cap;
+        batch_drop_graph(b);
+    }
+    uint32_t acc[4] = {0, 0, 0, 0};
+    for (uint32_t r = 0; r < n; r++) {
+        lmx_solver *s = b->members[r];
+        if (s->tables_dirty) {
+            int rc = upload_tables(s);
+            if (rc) return rc;
+        }
+        LMX_HIP(hipStreamSynchronize(s->stream));
+        lmx::Member &m = b->h_members[r];
+        m.p = make_params(s);
+        m.lat[0] = s->d_lat[0];
+        m.lat[1] = s->d_lat[1];
+        m.cur = (uint32_t)s->cur;
+        m.pad = 0;
+        const uint32_t threads = lmx::tile_threads_for(s->W);
+        m.geo[0] = lmx::tile_geom(m.p, 1, threads);
+        m.geo[1] = lmx::tile_geom(m.p, 2, threads);
+        const uint32_t blocks[4] = {lmx::x4_blocks(m.p), lmx::tile_blocks(m.geo[0]), lmx::tile_blocks(m.geo[1]),
+                                    lmx::reaction_blocks(m.p)};
+        for (int k = 0; k < 4; k++) {
+            b->h_start[k * (b->cap + 1) + r] = acc[k];
+            acc[k] += blocks[k];
+        }
+    }
+    for (int k = 0; k < 4; k++) {
+        b->h_start[k * (b->cap + 1) + n] = acc[k];
+        b->total[k] = acc[k];
+    }
+    LMX_HIP(hipMemcpyAsync(b->d_members, b->h_members, sizeof(lmx::Member) * n, hipMemcpyHostToDevice, b->stream));
+    LMX_HIP(hipMemcpyAsync(b->d_start, b->h_start, sizeof(uint32_t) * 4 * (b->cap + 1), hipMemcpyHostToDevice,
+                           b->stream));
+    return LMX_OK;
+}
+
+int batch_enqueue_step(lmx_batch *b)
+{
+    const uint32_t n = (uint32_t)b->members.size();
+    const uint32_t stride = b->cap + 1;
+    hipError_t e = lmx::launch_batch_sweeps(b->d_members, b->d_start, b->d_start + stride, b->d_start + 2 * stride,
+                                            b->d_start + 3 * stride, n, b->total, b->W, b->stream);
+    if (e != hipSuccess) return hip_fail(e, "batched sweep launch");
+    e = lmx::launch_batch_resolve(b->d_members, n, b->stream);
+    if (e != hipSuccess) return hip_fail(e, "batched resolve launch");
+    return LMX_OK;
+}
+
+int batch_capture(lmx_batch *b)
+{
+    if (b->exec && b->graph_members == b->members && std::memcmp(b->graph_total, b->total, sizeof(b->total)) == 0)
+        return LMX_OK;
+    batch_drop_graph(b);
+    hipGraph_t g = nullptr;
+    LMX_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    int rc = batch_enqueue_step(b);
+    hipError_t e = hipStreamEndCapture(b->stream, &g);
+    if (rc) {
+        if (g) hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+    e = hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0);
+    hipGraphDestroy(g);
+    if (e != hipSuccess) {
+        b->exec = nullptr;
+        return hip_fail(e, "hipGraphInstantiate");
+    }
+    b->graph_members = b->members;
+    std::memcpy(b->graph_total, b->total, sizeof(b->total));
+    return LMX_OK;
+}
+
+}  // namespace
+
+int lmx_batch_create(int device, lmx_batch **out)
+{
+    if (!out) return fail(LMX_ERR_ARG, "null output handle");
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return fail(LMX_ERR_HIP, "no HIP device is available");
+    if (device < 0 || device >= n) return fail(LMX_ERR_ARG, "device index out of range");
+    lmx_batch *b = new lmx_batch();
+    b->device = device;
+    hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->done, hipEventDisableTiming);
+    if (e != hipSuccess) {
+        int rc = hip_fail(e, "lmx_batch_create");
+        lmx_batch_destroy(b);
+        return rc;
+    }
+    *out = b;
+    return LMX_OK;
+}
+
+int lmx_batch_destroy(lmx_batch *b)
+{
+    if (!b) return LMX_OK;
+    hipSetDevice(b->device);
+    if (b->stream) hipStreamSynchronize(b->stream);
+    for (lmx_solver *s : b->members) s->batch = nullptr;
+    batch_drop_graph(b);
+    dev_free(b->d_members);
+    dev_free(b->d_start);
+    if (b->h_members) hipHostFree(b->h_members);
+    if (b->h_start) hipHostFree(b->h_start);
+    if (b->done) hipEventDestroy(b->done);
+    if (b->stream) hipStreamDestroy(b->stream);
+    delete b;
+    return LMX_OK;
+}
+
+int lmx_batch_add(lmx_batch *b, lmx_solver *s)
+{
+    if (!b || !s) return fail(LMX_ERR_ARG, "null batch or solver");
+    if (s->batch) return fail(LMX_ERR_ARG, "the solver is already a member of a batch");
+    if (s->device != b->device) return fail(LMX_ERR_ARG, "the solver lives on another device");
+    int rc = require_ready(s);
+    if (rc) return rc;
+    const lmx::Params p = make_params(s);
+    if (!lmx::fast_path_ok(p)) return fail(LMX_ERR_ARG, "batched solvers need X % 4 == 0");
+    if (!b->members.empty() && s->W != b->W) return fail(LMX_ERR_ARG, "batched solvers need the same particles per site");
+    b->W = s->W;
+    b->members.push_back(s);
+    s->batch = b;
+    return LMX_OK;
+}
+
+int lmx_batch_remove(lmx_batch *b, lmx_solver *s)
+{
+    if (!b || !s) return fail(LMX_ERR_ARG, "null batch or solver");
+    auto it = std::find(b->members.begin(), b->members.end(), s);
+    if (it == b->members.end()) return fail(LMX_ERR_ARG, "the solver is not a member of this batch");
+    hipSetDevice(b->device);
+    hipStreamSynchronize(b->stream);
+    b->members.erase(it);
+    s->batch = nullptr;
+    return LMX_OK;
+}
+
+int lmx_batch_size(const lmx_batch *b, uint32_t *n)
+{
+    if (!b || !n) return fail(LMX_ERR_ARG, "null argument");
+    *n = (uint32_t)b->members.size();
+    return LMX_OK;
+}
+
+int lmx_batch_set_option(lmx_batch *b, int option, int value)
+{
+    if (!b) return fail(LMX_ERR_ARG, "null batch");
+    if (option == LMX_BATCH_GRAPH) {
+        b->use_graph = value != 0;
+        if (!b->use_graph) batch_drop_graph(b);
+        return LMX_OK;
+    }
+    return fail(LMX_ERR_ARG, "unknown batch option");
+}
+
+int lmx_batch_run_steps(lmx_batch *b, uint32_t nsteps)
+{
+    if (!b) return fail(LMX_ERR_ARG, "null batch");
+    if (nsteps == 0 || b->members.empty()) return LMX_OK;
+    LMX_HIP(hipSetDevice(b->device));
+    int rc = batch_prepare(b);
+    if (rc) return rc;
+    if (b->use_graph) {
+        if ((rc = batch_capture(b))) return rc;
+        for (uint32_t i = 0; i < nsteps; i++) LMX_HIP(hipGraphLaunch(b->exec, b->stream));
+    } else {
+        for (uint32_t i = 0; i < nsteps; i++)
+            if ((rc = batch_enqueue_step(b))) return rc;
+    }
+    LMX_HIP(hipEventRecord(b->done, b->stream));
+    for (lmx_solver *s : b->members) {
+        if (nsteps & 1u) s->cur = 1 - s->cur;
+        s->timestep += nsteps;
+        // Later work on a member's own stream starts after the batch's.
+        LMX_HIP(hipStreamWaitEvent(s->stream, b->done, 0));
+    }
+    return LMX_OK;
+}
+
+int lmx_batch_synchronize(lmx_batch *b)
+{
+    if (!b) return fail(LMX_ERR_ARG, "null batch");
+    LMX_HIP(hipSetDevice(b->device));
+    LMX_HIP(hipStreamSynchronize(b->stream));
+    for (lmx_solver *s : b->members) {
+        int rc = check_device_error(s);
+        if (rc) return rc;
+    }
+    return LMX_OK;
+}
+
+int lmx_batch_time_steps(lmx_batch *b, uint32_t nsteps, double *total_ms)
+{
+    if (!b || !total_ms) return fail(LMX_ERR_ARG, "null argument");
+    LMX_HIP(hipSetDevice(b->device));
+    hipEvent_t t0, t1;
+    LMX_HIP(hipEventCreate(&t0));
+    LMX_HIP(hipEventCreate(&t1));
+    int rc = batch_prepare(b);
+    if (!rc && b->use_graph) rc = batch_capture(b);
+    if (!rc) {
+        LMX_HIP(hipStreamSynchronize(b->stream));
+        LMX_HIP(hipEventRecord(t0, b->stream));
+        rc = lmx_batch_run_steps(b, nsteps);
+        LMX_HIP(hipEventRecord(t1, b->stream));
+        LMX_HIP(hipEventSynchronize(t1));
+        float ms = 0.0f;
+        LMX_HIP(hipEventElapsedTime(&ms, t0, t1));
+        *total_ms = ms;
+    }
+    hipEventDestroy(t0);
+    hipEventDestroy(t1);
+    if (rc) return rc;
+    return lmx_batch_synchronize(b);
+}

@@ -26,6 +26,7 @@
 // each in slot order, first P kept, the rest recorded as overflow.
 #include <type_traits>
 
+#include "lmx_batch.h"
 #include "lmx_device.h"
 #include "lmx_kernels.h"
 #include "lmx_site.h"
@@ -240,13 +241,15 @@ __device__ inline uint32_t site_bits(unsigned long long m, int s)
 // ---------------------------------------------------------------------------
 // x sweep
 // ---------------------------------------------------------------------------
+// Body of the x sweep for 256-thread block number `block` (the kernel below,
+// or one replicate's share of a batched launch).
 template <int W>
-__global__ __launch_bounds__(256) void sweep_x4_kernel(Params p, const uint32_t *__restrict__ src,
-                                                       uint32_t *__restrict__ dst)
+__device__ inline void x4_body(const Params &p, const uint32_t *__restrict__ src, uint32_t *__restrict__ dst,
+                               uint32_t block)
 {
     constexpr uint32_t P = 4 * W;
     const uint32_t ngroups = p.N >> 2;
-    const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t g = block * 256u + threadIdx.x;
     const bool active = g < ngroups;
     const uint32_t gc = active ? g : ngroups - 1;
     const int lane = (int)(threadIdx.x & 63u);
@@ -375,6 +378,13 @@ __global__ __launch_bounds__(256) void sweep_x4_kernel(Params p, const uint32_t 
 #pragma unroll
     for (int k = 0; k < W; k++) *out4[k] = res[k];
     (void)P;
+}
+
+template <int W>
+__global__ __launch_bounds__(256) void sweep_x4_kernel(Params p, const uint32_t *__restrict__ src,
+                                                       uint32_t *__restrict__ dst)
+{
+    x4_body<W>(p, src, dst, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -583,16 +593,18 @@ __device__ inline void defer_sites(const Params &p, uint32_t *deferred, uint32_t
 
 template <int W>
 constexpr int tile_threads() { return W == 4 ? 512 : 1024; }
+static_assert(tile_threads<2>() == 1024 && tile_threads<4>() == 512, "tile_threads_for must agree");
 
 // With REACT (z sweep) each output site also gets the cheap part of the
 // reaction phase (react_test); the sites it queues are evaluated by
 // reaction_deferred_kernel after the sweep.
 template <int W, bool REACT>
-__global__ __launch_bounds__(tile_threads<W>()) void sweep_tile_kernel(
-    Params p, const uint32_t *__restrict__ src, uint32_t *__restrict__ dst, uint32_t stride, uint32_t len,
-    uint32_t ostride, uint32_t nouter, uint32_t Gx, uint32_t Ty, uint32_t nxchunk, uint32_t ntiles,
-    uint32_t substep, uint32_t tile_first, uint32_t *__restrict__ deferred, uint32_t defer_cap)
+__device__ inline void tile_body(const Params &p, const uint32_t *__restrict__ src, uint32_t *__restrict__ dst,
+                                 const TileGeom &geo, uint32_t substep, uint32_t block,
+                                 uint32_t *__restrict__ deferred, uint32_t defer_cap)
 {
+    const uint32_t stride = geo.stride, len = geo.len, ostride = geo.ostride, nouter = geo.nouter, Gx = geo.Gx,
+                   Ty = geo.Ty, nxchunk = geo.nxchunk, ntiles = geo.ntiles, tile_first = geo.tile_first;
     constexpr int kEntries = tile_threads<W>() + 128;
     __shared__ uint4 s_w[W][kEntries];
     __shared__ uint32_t s_t[kEntries];
@@ -600,7 +612,7 @@ __global__ __launch_bounds__(tile_threads<W>()) void sweep_tile_kernel(
 
     const uint32_t nblocks = nxchunk * nouter * ntiles;
     const uint32_t per_xcd = (nblocks + 7) / 8;
-    const uint32_t logical = (blockIdx.x & 7u) * per_xcd + (blockIdx.x >> 3);
+    const uint32_t logical = (block & 7u) * per_xcd + (block >> 3);
     if (logical >= nblocks) return;  // whole block: no barrier is skipped by part of it
     // Adjacent logical blocks take the work nearest in memory: the next row
     // tile when tiles are closer than outer rows (y sweep), else the next
@@ -740,6 +752,16 @@ __global__ __launch_bounds__(tile_threads<W>()) void sweep_tile_kernel(
         }
         defer_sites(p, deferred, defer_cap, i0, dmask);
     }
+}
+
+template <int W, bool REACT>
+__global__ __launch_bounds__(tile_threads<W>()) void sweep_tile_kernel(Params p, const uint32_t *__restrict__ src,
+                                                                       uint32_t *__restrict__ dst, TileGeom geo,
+                                                                       uint32_t substep,
+                                                                       uint32_t *__restrict__ deferred,
+                                                                       uint32_t defer_cap)
+{
+    tile_body<W, REACT>(p, src, dst, geo, substep, blockIdx.x, deferred, defer_cap);
 }
 
 // ---------------------------------------------------------------------------
@@ -1050,7 +1072,7 @@ constexpr int kRG = 4;
 constexpr uint32_t kTabLds = 2048;  // float2 entries (16 KB)
 
 template <int W>
-__global__ __launch_bounds__(256) void reaction4_kernel(Params p, uint32_t *__restrict__ lat)
+__device__ inline void reaction4_body(const Params &p, uint32_t *__restrict__ lat, uint32_t block)
 {
     __shared__ float2 s_tab[kTabLds];
     const uint32_t ntab = p.nST * p.nS;
@@ -1062,7 +1084,7 @@ __global__ __launch_bounds__(256) void reaction4_kernel(Params p, uint32_t *__re
     const float2 *tab = in_lds ? s_tab : reinterpret_cast<const float2 *>(p.prob1);
 
     const uint32_t ngroups = p.N >> 2;
-    const uint32_t g0 = blockIdx.x * (256u * kRG) + threadIdx.x;
+    const uint32_t g0 = block * (256u * kRG) + threadIdx.x;
     uint4 w4[kRG][W];
     uint32_t t4[kRG];
     uint32_t todo = 0;
@@ -1128,6 +1150,12 @@ __global__ __launch_bounds__(256) void reaction4_kernel(Params p, uint32_t *__re
             for (int k = 0; k < W; k++) reinterpret_cast<uint4 *>(lat + (size_t)k * p.N)[g] = w4[q][k];
         }
     }
+}
+
+template <int W>
+__global__ __launch_bounds__(256) void reaction4_kernel(Params p, uint32_t *__restrict__ lat)
+{
+    reaction4_body<W>(p, lat, blockIdx.x);
 }
 
 // Probability tables for sites with at most one particle, built with the
@@ -1565,26 +1593,14 @@ template <int W>
 static hipError_t launch_tile(const Params &p, const uint32_t *src, uint32_t *dst, int axis, bool react,
                               uint32_t *deferred, uint32_t defer_cap, hipStream_t st)
 {
-    const uint32_t stride = axis == 1 ? p.X : p.XY;
-    const uint32_t len = axis == 1 ? p.Y : p.Z;
-    const uint32_t ostride = axis == 1 ? p.XY : p.X;
-    const uint32_t nouter = axis == 1 ? p.Z : p.Y;
-    const uint32_t gxn = p.X >> 2;
-    const uint32_t Gx = gxn < 64u ? gxn : 64u;
-    const uint32_t Ty = (uint32_t)tile_threads<W>() / Gx;
-    const uint32_t nxchunk = (gxn + Gx - 1) / Gx;
-    const uint32_t ntiles = (len + Ty - 1) / Ty;
-    const uint32_t nblocks = nxchunk * nouter * ntiles;
-    const uint32_t per_xcd = (nblocks + 7) / 8;
-    const uint32_t tile_first = (size_t)Ty * stride < (size_t)ostride ? 1u : 0u;
+    const TileGeom geo = tile_geom(p, axis, (uint32_t)tile_threads<W>());
+    const dim3 grid(tile_blocks(geo)), block(geo.Gx * geo.Ty);
     if (react)
-        hipLaunchKernelGGL((sweep_tile_kernel<W, true>), dim3(per_xcd * 8), dim3(Gx * Ty), 0, st, p, src, dst,
-                           stride, len, ostride, nouter, Gx, Ty, nxchunk, ntiles, (uint32_t)axis, tile_first, deferred,
+        hipLaunchKernelGGL((sweep_tile_kernel<W, true>), grid, block, 0, st, p, src, dst, geo, (uint32_t)axis, deferred,
                            defer_cap);
     else
-        hipLaunchKernelGGL((sweep_tile_kernel<W, false>), dim3(per_xcd * 8), dim3(Gx * Ty), 0, st, p, src, dst,
-                           stride, len, ostride, nouter, Gx, Ty, nxchunk, ntiles, (uint32_t)axis, tile_first, deferred,
-                           defer_cap);
+        hipLaunchKernelGGL((sweep_tile_kernel<W, false>), grid, block, 0, st, p, src, dst, geo, (uint32_t)axis,
+                           deferred, defer_cap);
     return hipGetLastError();
 }
 
@@ -1671,6 +1687,70 @@ hipError_t launch_prob_tables(const Params &p, float *prob1, float *prob0, hipSt
     const size_t n = (size_t)p.nST * p.nS > p.nST ? (size_t)p.nST * p.nS : p.nST;
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(prob_table_kernel, dim3(blocks_for(n, 256)), dim3(256), 0, st, p, prob1, prob0);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Batched launches: block b of a launch belongs to member find_member(start, b)
+// and runs that member's share with the single-solver body.  Buffer roles
+// follow the member's parity: x reads lat[cur], y writes it back, z and the
+// reaction phase leave the step in lat[cur ^ 1].
+// ---------------------------------------------------------------------------
+template <int W>
+__global__ __launch_bounds__(256) void batch_x4_kernel(const Member *__restrict__ m,
+                                                       const uint32_t *__restrict__ start, uint32_t n)
+{
+    const uint32_t r = find_member(start, n, blockIdx.x);
+    const Member &M = m[r];
+    const uint32_t c = M.cur;
+    x4_body<W>(M.p, M.lat[c], M.lat[c ^ 1u], blockIdx.x - start[r]);
+}
+
+template <int W>
+__global__ __launch_bounds__(tile_threads<W>()) void batch_tile_kernel(const Member *__restrict__ m,
+                                                                       const uint32_t *__restrict__ start,
+                                                                       uint32_t n, uint32_t axis)
+{
+    const uint32_t r = find_member(start, n, blockIdx.x);
+    const Member &M = m[r];
+    const uint32_t c = M.cur;
+    const uint32_t *src = axis == 1 ? M.lat[c ^ 1u] : M.lat[c];
+    uint32_t *dst = axis == 1 ? M.lat[c] : M.lat[c ^ 1u];
+    tile_body<W, false>(M.p, src, dst, M.geo[axis - 1], axis, blockIdx.x - start[r], nullptr, 0u);
+}
+
+template <int W>
+__global__ __launch_bounds__(256) void batch_reaction_kernel(const Member *__restrict__ m,
+                                                             const uint32_t *__restrict__ start, uint32_t n)
+{
+    const uint32_t r = find_member(start, n, blockIdx.x);
+    const Member &M = m[r];
+    reaction4_body<W>(M.p, M.lat[M.cur ^ 1u], blockIdx.x - start[r]);
+}
+
+uint32_t x4_blocks(const Params &p) { return blocks_for(p.N >> 2, 256); }
+uint32_t reaction_blocks(const Params &p) { return p.R > 0 ? blocks_for(p.N >> 2, 256 * kRG) : 0u; }
+
+hipError_t launch_batch_sweeps(const Member *m, const uint32_t *start_x, const uint32_t *start_y,
+                               const uint32_t *start_z, const uint32_t *start_r, uint32_t n, const uint32_t *total,
+                               uint32_t W, hipStream_t st)
+{
+    switch (W) {
+#define LMX_BATCH_CASE(WW)                                                                                        \
+    case WW:                                                                                                      \
+        hipLaunchKernelGGL(batch_x4_kernel<WW>, dim3(total[0]), dim3(256), 0, st, m, start_x, n);                  \
+        hipLaunchKernelGGL(batch_tile_kernel<WW>, dim3(total[1]), dim3(tile_threads<WW>()), 0, st, m, start_y, n,  \
+                           1u);                                                                                   \
+        hipLaunchKernelGGL(batch_tile_kernel<WW>, dim3(total[2]), dim3(tile_threads<WW>()), 0, st, m, start_z, n,  \
+                           2u);                                                                                   \
+        if (total[3]) hipLaunchKernelGGL(batch_reaction_kernel<WW>, dim3(total[3]), dim3(256), 0, st, m, start_r, n); \
+        break;
+        LMX_BATCH_CASE(1)
+        LMX_BATCH_CASE(2)
+        LMX_BATCH_CASE(4)
+#undef LMX_BATCH_CASE
+    default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

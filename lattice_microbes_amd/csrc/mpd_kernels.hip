@@ -11,6 +11,7 @@
 // Every per-particle decision depends only on (site index, slot, timestep,
 // substep) through the counter-based hash, so the results do not depend on
 // how sites are distributed over threads.
+#include "lmx_batch.h"
 #include "lmx_device.h"
 #include "lmx_kernels.h"
 #include "lmx_site.h"
@@ -112,7 +113,7 @@ __device__ inline void set_error(Control *ctl, uint32_t code, uint32_t step)
 
 constexpr int kResolveThreads = 256;
 
-__global__ __launch_bounds__(kResolveThreads) void resolve_kernel(Params p, uint32_t *lat)
+__device__ void resolve_body(const Params &p, uint32_t *lat)
 {
     __shared__ unsigned long long keys[kMaxOverflows];
     __shared__ int best;
@@ -236,6 +237,21 @@ __global__ __launch_bounds__(kResolveThreads) void resolve_kernel(Params p, uint
     }
 }
 
+__global__ __launch_bounds__(kResolveThreads) void resolve_kernel(Params p, uint32_t *lat)
+{
+    resolve_body(p, lat);
+}
+
+// Batched resolver: block r resolves member r's step on its post-step buffer
+// and flips the member's buffer parity for the next step.
+__global__ __launch_bounds__(kResolveThreads) void batch_resolve_kernel(Member *__restrict__ m)
+{
+    Member &M = m[blockIdx.x];
+    const uint32_t c = M.cur;
+    resolve_body(M.p, M.lat[c ^ 1u]);
+    if (threadIdx.x == 0) M.cur = c ^ 1u;
+}
+
 // ---------------------------------------------------------------------------
 // Species counts (ByteLattice::getParticleCounts, ByteLattice.cpp:495-520):
 // each word counts up to its first empty byte.
@@ -307,6 +323,12 @@ hipError_t launch_reaction(const Params &p, uint32_t *lat, hipStream_t st)
 hipError_t launch_resolve(const Params &p, uint32_t *lat, hipStream_t st)
 {
     hipLaunchKernelGGL(resolve_kernel, dim3(1), dim3(kResolveThreads), 0, st, p, lat);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_resolve(Member *m, uint32_t n, hipStream_t st)
+{
+    hipLaunchKernelGGL(batch_resolve_kernel, dim3(n), dim3(kResolveThreads), 0, st, m);
     return hipGetLastError();
 }
 
