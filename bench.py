@@ -105,6 +105,97 @@ def cpu_baseline(spec, budget_s: float, seed: int):
     }
 
 
+def ensemble_bench(nrep: int, steps: int, seed: int):
+    """Batched ensemble: nrep replicates of the bimolecular model on uneven
+    lattices (32^3, 48^3, 64^3, 96^3, 128^3 drawn uniformly), each with its own
+    seed, stepped by one Ensemble against the same replicates run one at a
+    time; the first replicates are checked bit for bit between the two."""
+    import lattice_microbes_amd as lmx
+    from lattice_microbes_amd import models
+
+    sizes = [32, 48, 64, 96, 128]
+    rng = np.random.default_rng(seed)
+    picks = rng.integers(0, len(sizes), nrep)
+    templates = {L: models.bimolecular(shape=(L, L, L), seed=seed + L) for L in sizes}
+
+    def make(r):
+        L = sizes[picks[r]]
+        s = lmx.MpdRdmeSolver(0)
+        templates[L].load_into(s, replicate=r, seed=1000 + r)
+        return s
+
+    plain = [make(r) for r in range(nrep)]
+    sites = sum(int(np.prod(s.shape)) for s in plain)
+    for s in plain:
+        s.runTimesteps(1)
+    t0 = time.perf_counter()
+    for s in plain:
+        s.runTimesteps(steps, synchronize=False)
+    for s in plain:
+        s.synchronize()
+    t_plain = time.perf_counter() - t0
+    # the reference's cadence: the host waits for every step (MpdRdmeSolver.cu:1091)
+    sub = plain[nrep - max(1, nrep // 8):]
+    sub_sites = sum(int(np.prod(s.shape)) for s in sub)
+    t0 = time.perf_counter()
+    for s in sub:
+        for _ in range(steps):
+            s.runTimesteps(1)
+    t_sync = time.perf_counter() - t0
+    batch = [make(r) for r in range(nrep)]
+    ens = lmx.Ensemble(0)
+    for s in batch:
+        ens.add(s)
+    ens.run(1)
+    ms = ens.time_steps(steps)
+    # every plain replicate outside sub ran 1 + steps steps, like the batch
+    check = [r for r in range(min(nrep, 16)) if all(plain[r] is not q for q in sub)]
+    same = all(np.array_equal(plain[r].getLatticeNative(), batch[r].getLatticeNative()) for r in check)
+    ens.close()
+    for s in plain + batch:
+        s.close()
+    b_rate = sites * steps / (ms / 1e3)
+    p_rate = sites * steps / t_plain
+    sync_rate = sub_sites * steps / t_sync
+    sim_steps = 20000
+    return {
+        "workload": f"{nrep} bimolecular replicates, lattices drawn from {sizes} cubed, own seeds",
+        "replicates": nrep, "sites": sites, "steps_timed": steps,
+        "batched_site_steps_per_s": b_rate, "plain_site_steps_per_s": p_rate,
+        "plain_stepsync_site_steps_per_s": sync_rate,
+        "speedup_vs_plain": b_rate / p_rate, "speedup_vs_plain_stepsync": b_rate / sync_rate,
+        "sims_per_hr_batched": nrep / (ms / 1e3 / steps * sim_steps) * 3600.0,
+        "sims_per_hr_plain": nrep / (t_plain / steps * sim_steps) * 3600.0,
+        "sim_steps": sim_steps,
+        "bit_exact_vs_plain": {"checked": len(check), "identical": bool(same)},
+    }
+
+
+def serving_bench(njobs: int, seed: int):
+    """Continuous batching of a Poisson request stream (20 req/s) mixing the
+    bimolecular 64^3, capsule 128x128x256 and whole-cell 256x256x128 (30
+    species) models, 1e3-1e4 steps each, against the same requests served one
+    at a time; every job's final lattice is compared between the two."""
+    from lattice_microbes_amd import models, serving
+
+    templates = [models.bimolecular(shape=(64, 64, 64), seed=seed),
+                 models.capsule_minde(seed=seed + 1),
+                 models.whole_cell(shape=(256, 256, 128), nS=30, R=90, n_particles=250_000, seed=seed + 2)]
+    jobs = serving.poisson_jobs(templates, njobs, 20.0, (1000, 10000), seed=seed + 10)
+    served = serving.Server(0, hbm_budget=200 << 30, max_active=8).run(jobs)
+    digest = {j.id: j.digest for j in jobs}
+    jobs2 = serving.poisson_jobs(templates, njobs, 20.0, (1000, 10000), seed=seed + 10)
+    plain = serving.serve_plain(jobs2)
+    same = all(digest[j.id] == j.digest for j in jobs2)
+    return {
+        "workload": f"{njobs} requests, Poisson 20/s, bimolecular 64^3 / capsule 128x128x256 / "
+                    "whole-cell 256x256x128 (30 species), 1e3-1e4 steps",
+        "served": served, "plain": plain,
+        "throughput_vs_plain": served["site_steps_per_s"] / plain["site_steps_per_s"],
+        "bit_exact_vs_plain": {"jobs": njobs, "identical": bool(same)},
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -116,6 +207,11 @@ def main() -> int:
                     help="step implementation (results are bit-identical)")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--ensemble-replicates", type=int, default=256)
+    ap.add_argument("--ensemble-steps", type=int, default=20)
+    ap.add_argument("--serving-jobs", type=int, default=24)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain 1 GiB device copy (HBM counter calibration)")
     args = ap.parse_args()
@@ -213,6 +309,12 @@ def main() -> int:
         "final_particles": int(counts.sum()),
     }
     solver.close()
+    if not args.no_ensemble:
+        log(f"ensemble: {args.ensemble_replicates} replicates")
+        out["ensemble"] = ensemble_bench(args.ensemble_replicates, args.ensemble_steps, args.seed)
+    if not args.no_serving:
+        log(f"serving: {args.serving_jobs} requests")
+        out["serving"] = serving_bench(args.serving_jobs, args.seed)
     if not args.no_cpu:
         log(f"cpu baseline ({args.cpu_budget:.0f}s budget)")
         out["cpu_baseline"] = cpu_baseline(spec, args.cpu_budget, args.seed)
