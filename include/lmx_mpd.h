@@ -118,7 +118,9 @@ int lmx_set_path(lmx_solver *s, int path);
  *   LMX_OPT_FORCE_GENERIC  1 = one-site-per-thread kernels even when X%4==0
  *   LMX_OPT_YZ_KERNEL      y/z sweep kernel: 0 = LDS row tiles, one group
  *                          per thread (default), 1 = register march, 2 = row
- *                          tiles with compacted work lists
+ *                          tiles with compacted work lists, 3 = register
+ *                          columns of LMX_OPT_MARCH_SEGMENT rows (axes whose
+ *                          length the segment does not divide use 0)
  *   LMX_OPT_FUSED_ZSEG     z planes per block of the fused step; 0 (default)
  *                          picks one block per compute unit
  *   LMX_OPT_FUSE_REACTION  1 = the tiled z sweep tests the reaction of its

@@ -16,6 +16,9 @@ hipError_t launch_resolve(const Params &p, uint32_t *lat, hipStream_t st);
 bool fast_path_ok(const Params &p);
 hipError_t launch_sweep_fast(const Params &p, const uint32_t *src, uint32_t *dst, int axis, uint32_t seg,
                              hipStream_t st);
+bool col_ok(const Params &p, int axis, uint32_t seg);
+hipError_t launch_sweep_col(const Params &p, const uint32_t *src, uint32_t *dst, int axis, uint32_t seg,
+                            hipStream_t st);
 hipError_t launch_sweep_tile(const Params &p, const uint32_t *src, uint32_t *dst, int axis, bool react,
                              uint32_t *deferred, uint32_t defer_cap, hipStream_t st);
 hipError_t launch_sweep_sparse(const Params &p, const uint32_t *src, uint32_t *dst, int axis, bool react,

@@ -345,12 +345,15 @@ int enqueue_step(lmx_solver *s, const lmx::Params &p, hipEvent_t *ev, int *kinds
     // The tiled z sweeps can test the reaction of their output sites and
     // queue the rest (not with zero-order reactions, which make every site a
     // candidate).
-    const bool z_reacts = fast && s->yz_kernel != 1 && s->fuse_reaction && s->R > 0 && !s->any_qp0;
+    const bool z_reacts = fast && (s->yz_kernel == 0 || s->yz_kernel == 2) && s->fuse_reaction && s->R > 0 &&
+                          !s->any_qp0;
     for (int axis = 0; axis < 3; axis++) {
         const bool react = axis == 2 && z_reacts;
         if (!fast)
             e = lmx::launch_sweep(p, s->d_lat[s->cur], s->d_lat[1 - s->cur], axis, s->stream);
-        else if (axis > 0 && s->yz_kernel == 0)
+        else if (axis > 0 && s->yz_kernel == 3 && !react && lmx::col_ok(p, axis, s->march_seg))
+            e = lmx::launch_sweep_col(p, s->d_lat[s->cur], s->d_lat[1 - s->cur], axis, s->march_seg, s->stream);
+        else if (axis > 0 && (s->yz_kernel == 0 || s->yz_kernel == 3))
             e = lmx::launch_sweep_tile(p, s->d_lat[s->cur], s->d_lat[1 - s->cur], axis, react, s->d_deferred,
                                        s->defer_cap, s->stream);
         else if (axis > 0 && s->yz_kernel == 2)
@@ -685,8 +688,9 @@ int lmx_set_launch_option(lmx_solver *s, int option, int value)
         s->force_generic = value != 0;
         return LMX_OK;
     case LMX_OPT_YZ_KERNEL:
-        if (value < 0 || value > 2)
-            return fail(LMX_ERR_ARG, "y/z kernel must be 0 (LDS tiles), 1 (march) or 2 (compacted tiles)");
+        if (value < 0 || value > 3)
+            return fail(LMX_ERR_ARG,
+                        "y/z kernel must be 0 (LDS tiles), 1 (march), 2 (compacted tiles) or 3 (register columns)");
         s->yz_kernel = value;
         return LMX_OK;
     case LMX_OPT_FUSE_REACTION:

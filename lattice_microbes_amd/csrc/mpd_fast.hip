@@ -68,10 +68,17 @@ __device__ inline void zero_group(Group4<W> &G)
     G.t = 0u;
 }
 
-// Bit 7 of each byte set iff the byte is nonzero, folded to 4 bits.
+// 0x80 in each nonzero byte of v, 0 elsewhere.
+__device__ inline uint32_t nonzero_hi(uint32_t v)
+{
+    return (((v & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | v) & 0x80808080u;
+}
+
+// Bit b set iff byte b of v is nonzero (4 bits).  (A byte dot product can
+// gather the four flags in one instruction, but measured slower here.)
 __device__ inline uint32_t nonzero_bytes(uint32_t v)
 {
-    const uint32_t hi = (((v & 0x7F7F7F7Fu) + 0x7F7F7F7Fu) | v) & 0x80808080u;
+    const uint32_t hi = nonzero_hi(v);
     return ((hi >> 7) & 1u) | ((hi >> 14) & 2u) | ((hi >> 21) & 4u) | ((hi >> 28) & 8u);
 }
 
@@ -91,10 +98,12 @@ __device__ inline unsigned long long group_occ(const Group4<W> &G)
 {
     unsigned long long m = 0;
 #pragma unroll
-    for (int s = 0; s < 4; s++)
+    for (int s = 0; s < 4; s++) {
+        uint32_t w[W];
 #pragma unroll
-        for (int k = 0; k < W; k++)
-            m |= (unsigned long long)nonzero_bytes(lane_of(G.w[k], s)) << (s * 4 * W + 4 * k);
+        for (int k = 0; k < W; k++) w[k] = lane_of(G.w[k], s);
+        m |= (unsigned long long)site_occ<W>(w) << (s * 4 * W);
+    }
     return m;
 }
 
@@ -258,7 +267,7 @@ __device__ inline void x4_body(const Params &p, const uint32_t *__restrict__ src
     Group4<W> G;
     load_group<W>(p, src, gc, G);
     const uint32_t i0 = gc * 4u;
-    const uint32_t x0 = i0 % p.X;
+    const uint32_t x0 = (p.X & (p.X - 1u)) == 0u ? (i0 & (p.X - 1u)) : i0 % p.X;
     const bool has_m = (x0 > 0) || (p.boundary == 0);
     const bool has_p = (x0 + 4 < p.X) || (p.boundary == 0);
     const uint32_t il = (x0 > 0) ? i0 - 1 : i0 + p.X - 1;
@@ -530,6 +539,94 @@ __global__ __launch_bounds__(256) void sweep_march_kernel(Params p, const uint32
 }
 
 // ---------------------------------------------------------------------------
+// y / z sweep on register columns.  A thread owns one group (4 x-sites) and a
+// segment of SEG rows along the axis; it loads the SEG rows and the apron row
+// on either side at once, so every neighbour along the axis is in its own
+// registers: no LDS, no barriers, and each wave issues all of its loads
+// before any decision waits on them.  The apron rows are the neighbouring
+// segments' edge rows; segments are numbered so that the threads of a block
+// (and consecutive blocks of an XCD) take adjacent segments, and those rows
+// hit in cache.  Requires len % SEG == 0.
+// ---------------------------------------------------------------------------
+template <int W, int SEG>
+__global__ __launch_bounds__(256) void sweep_col_kernel(Params p, const uint32_t *__restrict__ src,
+                                                        uint32_t *__restrict__ dst, uint32_t stride, uint32_t len,
+                                                        uint32_t ostride, uint32_t nouter, uint32_t substep)
+{
+    using mask_t = typename std::conditional<W == 4, unsigned long long, uint32_t>::type;
+    const uint32_t gxn = p.X >> 2;
+    const uint32_t nseg = len / SEG;
+    const uint32_t nthreads = gxn * nseg * nouter;
+    const uint32_t nblocks = (nthreads + 255u) / 256u;
+    const uint32_t per_xcd = (nblocks + 7) / 8;
+    const uint32_t logical = (blockIdx.x & 7u) * per_xcd + (blockIdx.x >> 3);
+    const uint32_t t = logical * 256u + threadIdx.x;
+    if (logical >= nblocks || t >= nthreads) return;
+    const uint32_t g = t % gxn;
+    const uint32_t rest = t / gxn;
+    const uint32_t sgi = rest % nseg, o = rest / nseg;
+    const uint32_t r0 = sgi * SEG;
+    const uint32_t base = o * ostride + g * 4u;
+    const unsigned long long h = timestep_hash(p.seed32, p.ctl->timestep, substep);
+    const bool periodic = p.boundary == 0;
+    auto hm_of = [&](int r) -> uint32_t { return (r > 0 || periodic) ? 0xFu : 0u; };
+    auto hp_of = [&](int r) -> uint32_t { return (r + 1 < (int)len || periodic) ? 0xFu : 0u; };
+
+    // R[k] is row r0 - 1 + k: R[0] and R[SEG+1] are the apron rows.
+    Group4<W> R[SEG + 2];
+    load_row<W>(p, src, base, stride, len, (int)r0 - 1, R[0]);
+#pragma unroll
+    for (int k = 1; k <= SEG; k++) load_group<W>(p, src, (base + (r0 + k - 1) * stride) >> 2, R[k]);
+    load_row<W>(p, src, base, stride, len, (int)(r0 + SEG), R[SEG + 1]);
+
+    mask_t occ[SEG + 2], mi[SEG + 2], pl[SEG + 2];
+#pragma unroll
+    for (int k = 0; k < SEG + 2; k++) {
+        const int r = (int)r0 - 1 + k;
+        const unsigned long long oc = group_occ<W>(R[k]);
+        unsigned long long m = 0, q = 0;
+        if (oc) {
+            // The apron rows only need the direction that enters the segment.
+            const uint32_t hm = k == 0 ? 0u : hm_of(r);
+            const uint32_t hp = k == SEG + 1 ? 0u : hp_of(r);
+            const uint32_t tm = k == 0 ? 0u : R[k - 1].t;
+            const uint32_t tp = k == SEG + 1 ? 0u : R[k + 1].t;
+            const uint32_t ri = (k == 0 || k == SEG + 1) ? (uint32_t)row_index<W>(p, base, stride, len, r)
+                                                         : base + (uint32_t)r * stride;
+            decide<W>(p, R[k], tm, tp, hm, hp, ri, h, oc, m, q);
+        }
+        occ[k] = (mask_t)oc;
+        mi[k] = (mask_t)m;
+        pl[k] = (mask_t)q;
+    }
+#pragma unroll
+    for (int k = 1; k <= SEG; k++) {
+        const uint32_t rowi = base + (r0 + k - 1) * stride;
+        uint4 res[W];
+        if ((occ[k] | pl[k - 1] | mi[k + 1]) == 0) {
+#pragma unroll
+            for (int q = 0; q < W; q++) res[q] = make_uint4(0u, 0u, 0u, 0u);
+        } else {
+#pragma unroll
+            for (int s = 0; s < 4; s++) {
+                uint32_t wo[W], wm[W], wp[W], ov[W];
+                site_words<W>(R[k], s, wo);
+                site_words<W>(R[k - 1], s, wm);
+                site_words<W>(R[k + 1], s, wp);
+                const uint32_t own_occ = site_bits<W>(occ[k], s);
+                const uint32_t stay = own_occ & ~(site_bits<W>(mi[k], s) | site_bits<W>(pl[k], s));
+                assemble<W>(p, rowi + s, substep, wo, stay, own_occ, wm, site_bits<W>(pl[k - 1], s), wp,
+                            site_bits<W>(mi[k + 1], s), ov);
+#pragma unroll
+                for (int q = 0; q < W; q++) set_lane(res[q], s, ov[q]);
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < W; q++) reinterpret_cast<uint4 *>(dst + (size_t)q * p.N + rowi)[0] = res[q];
+    }
+}
+
+// ---------------------------------------------------------------------------
 // y / z sweep on LDS tiles.  A block covers Gx groups along x by Ty rows along
 // the axis (one group per thread), stages its rows plus one apron row on each
 // side in LDS, decides its own rows' particles (the apron rows only in the
@@ -607,7 +704,6 @@ __device__ inline void tile_body(const Params &p, const uint32_t *__restrict__ s
                    Ty = geo.Ty, nxchunk = geo.nxchunk, ntiles = geo.ntiles, tile_first = geo.tile_first;
     constexpr int kEntries = tile_threads<W>() + 128;
     __shared__ uint4 s_w[W][kEntries];
-    __shared__ uint32_t s_t[kEntries];
     __shared__ unsigned long long s_m[kEntries], s_p[kEntries];
 
     const uint32_t nblocks = nxchunk * nouter * ntiles;
@@ -645,43 +741,31 @@ __device__ inline void tile_body(const Params &p, const uint32_t *__restrict__ s
     Group4<W> G;
     zero_group<W>(G);
     if (valid) load_group<W>(p, src, (base + r * stride) >> 2, G);
-    Group4<W> Ap;  // apron row for this thread (top or bottom), if any
-    zero_group<W>(Ap);
     const bool top = valid && ly == 0;
     const bool bottom = valid && ly == nrows - 1;
-    if (top) load_row<W>(p, src, base, stride, len, (int)r0 - 1, Ap);
-    Group4<W> Bp;
+    Group4<W> Ap, Bp;  // apron rows above and below the tile
+    zero_group<W>(Ap);
     zero_group<W>(Bp);
+    if (top) load_row<W>(p, src, base, stride, len, (int)r0 - 1, Ap);
     if (bottom) load_row<W>(p, src, base, stride, len, (int)(r0 + nrows), Bp);
+    // Site types of the neighbouring rows along the axis, read directly (the
+    // block loads those rows itself, so these hit in cache): the decisions
+    // then need no barrier, and their table loads overlap other waves' work.
+    const bool hm_ok = valid && (r > 0 || periodic), hp_ok = valid && (r + 1 < len || periodic);
+    const uint32_t *sites32 = reinterpret_cast<const uint32_t *>(p.sites);
+    uint32_t tmy = 0, tpy = 0;
+    if (hm_ok) tmy = top ? Ap.t : sites32[(base + (r > 0 ? r - 1 : len - 1) * stride) >> 2];
+    if (hp_ok) tpy = bottom ? Bp.t : sites32[(base + (r + 1 < len ? r + 1 : 0u) * stride) >> 2];
+
+    const unsigned long long occ = valid ? group_occ<W>(G) : 0ull;
+    unsigned long long mi = 0, pl = 0;
+    if (occ) decide<W>(p, G, tmy, tpy, hm_ok ? 0xFu : 0u, hp_ok ? 0xFu : 0u, base + r * stride, h, occ, mi, pl);
     // Slot (row+1)*Gx + lx; rows at or beyond nrows write nothing, because
     // slot nrows+1 holds the bottom apron row.
     const uint32_t e = (ly + 1) * Gx + lx;
     if (ly < nrows) {
 #pragma unroll
         for (int k = 0; k < W; k++) s_w[k][e] = G.w[k];
-        s_t[e] = G.t;
-    }
-    if (top) {
-#pragma unroll
-        for (int k = 0; k < W; k++) s_w[k][lx] = Ap.w[k];
-        s_t[lx] = Ap.t;
-    }
-    if (bottom) {
-        const uint32_t eb = (nrows + 1) * Gx + lx;
-#pragma unroll
-        for (int k = 0; k < W; k++) s_w[k][eb] = Bp.w[k];
-        s_t[eb] = Bp.t;
-    }
-    __syncthreads();
-
-    unsigned long long occ = valid ? group_occ<W>(G) : 0ull;
-    unsigned long long mi = 0, pl = 0;
-    if (occ) {
-        const uint32_t hm = (r > 0 || periodic) ? 0xFu : 0u;
-        const uint32_t hp = (r + 1 < len || periodic) ? 0xFu : 0u;
-        decide<W>(p, G, s_t[e - Gx], s_t[e + Gx], hm, hp, base + r * stride, h, occ, mi, pl);
-    }
-    if (ly < nrows) {
         s_m[e] = mi;
         s_p[e] = pl;
     }
@@ -693,6 +777,8 @@ __device__ inline void tile_body(const Params &p, const uint32_t *__restrict__ s
             const int ra = (int)r0 - 1;
             decide<W>(p, Ap, 0u, G.t, 0u, 0xFu, (uint32_t)row_index<W>(p, base, stride, len, ra), h, ao, am, ap);
         }
+#pragma unroll
+        for (int k = 0; k < W; k++) s_w[k][lx] = Ap.w[k];
         s_p[lx] = ap;
     }
     if (bottom) {
@@ -703,7 +789,10 @@ __device__ inline void tile_body(const Params &p, const uint32_t *__restrict__ s
             const int rb = (int)(r0 + nrows);
             decide<W>(p, Bp, G.t, 0u, 0xFu, 0u, (uint32_t)row_index<W>(p, base, stride, len, rb), h, bo, bm, bpl);
         }
-        s_m[(nrows + 1) * Gx + lx] = bm;
+        const uint32_t eb = (nrows + 1) * Gx + lx;
+#pragma unroll
+        for (int k = 0; k < W; k++) s_w[k][eb] = Bp.w[k];
+        s_m[eb] = bm;
     }
     __syncthreads();
 
@@ -1601,6 +1690,43 @@ static hipError_t launch_tile(const Params &p, const uint32_t *src, uint32_t *ds
     else
         hipLaunchKernelGGL((sweep_tile_kernel<W, false>), grid, block, 0, st, p, src, dst, geo, (uint32_t)axis,
                            deferred, defer_cap);
+    return hipGetLastError();
+}
+
+bool col_ok(const Params &p, int axis, uint32_t seg)
+{
+    const uint32_t len = axis == 1 ? p.Y : p.Z;
+    return (seg == 4 || seg == 8) && len % seg == 0;
+}
+
+hipError_t launch_sweep_col(const Params &p, const uint32_t *src, uint32_t *dst, int axis, uint32_t seg,
+                            hipStream_t st)
+{
+    if (axis < 1 || axis > 2 || !col_ok(p, axis, seg)) return hipErrorInvalidValue;
+    const uint32_t stride = axis == 1 ? p.X : p.XY;
+    const uint32_t len = axis == 1 ? p.Y : p.Z;
+    const uint32_t ostride = axis == 1 ? p.XY : p.X;
+    const uint32_t nouter = axis == 1 ? p.Z : p.Y;
+    const size_t threads = (size_t)(p.X >> 2) * (len / seg) * nouter;
+    const dim3 grid((unsigned)((blocks_for(threads, 256) + 7) / 8 * 8)), block(256);
+    const uint32_t sub = (uint32_t)axis;
+#define LMX_COL(WW, SS) hipLaunchKernelGGL((sweep_col_kernel<WW, SS>), grid, block, 0, st, p, src, dst, stride, len, ostride, nouter, sub)
+    if (seg == 4) {
+        switch (p.W) {
+        case 1: LMX_COL(1, 4); break;
+        case 2: LMX_COL(2, 4); break;
+        case 4: LMX_COL(4, 4); break;
+        default: return hipErrorInvalidValue;
+        }
+    } else {
+        switch (p.W) {
+        case 1: LMX_COL(1, 8); break;
+        case 2: LMX_COL(2, 8); break;
+        case 4: LMX_COL(4, 8); break;
+        default: return hipErrorInvalidValue;
+        }
+    }
+#undef LMX_COL
     return hipGetLastError();
 }
 

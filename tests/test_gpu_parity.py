@@ -175,6 +175,8 @@ ALIGNED = [
     ((16, 1, 3), 16, "periodic", 0.25),
     ((32, 12, 20), 4, "reflecting", 0.3),
     ((64, 16, 16), 8, "periodic", 0.05),
+    ((8, 8, 16), 8, "reflecting", 0.3),
+    ((12, 24, 8), 16, "periodic", 0.2),
 ]
 
 
@@ -189,8 +191,8 @@ def test_vector_kernels_match_generic_and_oracle(gpu, case):
     orc = oracle.Oracle(spec, seed_top=5, replicate=2)
     assert orc.run(6) == 0
     ref = orc.native()
-    for opt in ({"generic": 1}, {"sparse": 1}, {"sparse": 1, "fuse": 1}, {"fuse": 1}, {"seg": 1}, {"seg": 3},
-                {"seg": 7}, {"seg": 32}, {"seg": 100000}):
+    for opt in ({"generic": 1}, {"sparse": 1}, {"sparse": 1, "fuse": 1}, {"fuse": 1}, {"col": 4}, {"col": 8},
+                {"seg": 1}, {"seg": 3}, {"seg": 7}, {"seg": 32}, {"seg": 100000}):
         s = lmx.MpdRdmeSolver(0)
         spec.load_into(s, replicate=2, seed=5)
         if "generic" in opt:
@@ -199,6 +201,9 @@ def test_vector_kernels_match_generic_and_oracle(gpu, case):
             s.setLaunchOption(lmx._lib.OPT_YZ_KERNEL, 2)
         if "fuse" in opt:
             s.setLaunchOption(lmx._lib.OPT_FUSE_REACTION, 1)
+        elif "col" in opt:
+            s.setLaunchOption(lmx._lib.OPT_YZ_KERNEL, 3)
+            s.setLaunchOption(lmx._lib.OPT_MARCH_SEGMENT, opt["col"])
         elif "seg" in opt:
             s.setLaunchOption(lmx._lib.OPT_YZ_KERNEL, 1)
             s.setLaunchOption(lmx._lib.OPT_MARCH_SEGMENT, min(opt["seg"], 65536))
