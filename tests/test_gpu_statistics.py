@@ -1,0 +1,133 @@
+"""Statistical agreement of the MPD-RDME ensemble with the CPU solvers.
+
+The integer state is checked bit for bit elsewhere; these tests check that
+the dynamics are the right ones: species-count means and variances of a GPU
+ensemble (one batched Ensemble of independent replicates) against
+
+  * the exact stationary distribution of the well-mixed chemical master
+    equation for A + B <-> C (detailed balance; the RDME's total-count
+    stationary distribution is the same), and
+  * an ensemble of the next-subvolume CPU counterpart (exact RDME SSA) at a
+    fixed time on the same lattice and model,
+
+each with a 99% confidence interval built from independent replicates (a
+replicate's time average is one sample, so autocorrelation within a replicate
+cannot narrow the interval).
+"""
+import ctypes
+import math
+
+import numpy as np
+import pytest
+
+import lattice_microbes_amd as lmx
+import oracle
+from helpers import q_to_D, spec, TAU
+
+pytestmark = pytest.mark.gpu
+
+Z99 = 2.5758
+A0 = 30
+L = 8
+
+
+def bimolecular_case(a_pair=0.03, kr_step=1e-3, q=0.2, seed=5):
+    """A + B <-> C on an 8^3 periodic lattice: a_pair is the per-step reaction
+    propensity of one co-located A-B pair (K * tau * N), kr_step the per-step
+    dissociation propensity of one C."""
+    N = L ** 3
+    K = a_pair / (TAU * N)
+    kr = kr_step / TAU
+    rx = [(2, K, {0: -1, 1: -1, 2: 1}, [0, 1]), (1, kr, {0: 1, 1: 1, 2: -1}, [2])]
+    rng = np.random.default_rng(seed)
+    pos = rng.integers(0, L, size=(2 * A0, 3))
+    parts = [(int(x), int(y), int(z), 1 + (k >= A0)) for k, (x, y, z) in enumerate(pos)]
+    s = spec((L, L, L), P=8, nS=3, DF=q_to_D([q, q, q]).reshape(1, 1, 3), reactions=rx, particles=parts)
+    return s, K, kr
+
+
+def exact_stationary(K, kr):
+    w = np.array([(K / kr) ** c / (math.factorial(A0 - c) ** 2 * math.factorial(c)) for c in range(A0 + 1)])
+    pi = w / w.sum()
+    c = np.arange(A0 + 1)
+    mu = float((pi * c).sum())
+    return mu, float((pi * c * c).sum() - mu * mu)
+
+
+def ensemble(s, replicates, seed0):
+    ens = lmx.Ensemble(0)
+    solvers = []
+    for r in range(replicates):
+        sv = lmx.MpdRdmeSolver(0)
+        s.load_into(sv, replicate=r, seed=seed0 + r)
+        ens.add(sv)
+        solvers.append(sv)
+    return ens, solvers
+
+
+def test_stationary_distribution_matches_cme():
+    s, K, kr = bimolecular_case()
+    mu, var = exact_stationary(K, kr)
+    R, burn, every, nsamp = 256, 4000, 100, 32
+    ens, solvers = ensemble(s, R, 100)
+    ens.run(burn)  # ~12 relaxation times from C = 0
+    samp = np.zeros((R, nsamp))
+    for k in range(nsamp):
+        ens.run(every)
+        samp[:, k] = [int(sv.getSpeciesCounts()[2]) for sv in solvers]
+    # conservation: A + C and B + C stay A0 in every replicate
+    for sv in solvers[:8]:
+        a, b, c = (int(v) for v in sv.getSpeciesCounts())
+        assert a + c == A0 and b + c == A0
+    ens.close()
+    rmean, rsq = samp.mean(axis=1), (samp ** 2).mean(axis=1)
+    gmean = rmean.mean()
+    se = rmean.std(ddof=1) / math.sqrt(R)
+    assert abs(gmean - mu) < Z99 * se, f"mean {gmean:.4f} vs exact {mu:.4f} (99% half-width {Z99 * se:.4f})"
+    # variance: 99% bootstrap interval over replicates
+    rng = np.random.default_rng(0)
+    boot = []
+    for _ in range(2000):
+        idx = rng.integers(0, R, R)
+        m = rmean[idx].mean()
+        boot.append(rsq[idx].mean() - m * m)
+    lo, hi = np.percentile(boot, [0.5, 99.5])
+    assert lo <= var <= hi, f"variance interval [{lo:.4f}, {hi:.4f}] misses exact {var:.4f}"
+
+
+def test_transient_mean_matches_next_subvolume_ensemble():
+    """Mean and variance of C after 200 steps from the same initial lattice:
+    MPD on the GPU against the exact RDME (next-subvolume method) on the CPU."""
+    s, K, kr = bimolecular_case()
+    steps, R_mpd, R_nsm = 200, 512, 256
+    ens, solvers = ensemble(s, R_mpd, 300)
+    ens.run(steps)
+    c_mpd = np.array([int(sv.getSpeciesCounts()[2]) for sv in solvers], dtype=np.float64)
+    ens.close()
+
+    lib = oracle.load()
+    S = np.ascontiguousarray(s.S, dtype=np.int32).reshape(-1)
+    DF = np.ascontiguousarray(s.DF, dtype=np.float64).reshape(-1)
+    c_nsm = np.zeros(R_nsm)
+    counts = np.zeros(3, np.uint32)
+    for r in range(R_nsm):
+        o = oracle.Oracle(s, seed_top=7, replicate=r)
+        h = lib.orc_nsm_create(L, L, L, s.P, s.nS, s.nST, s.R, oracle._p(o.types, ctypes.c_uint32),
+                               oracle._p(o.k, ctypes.c_double), oracle._p(o.si, ctypes.c_uint32),
+                               oracle._p(o.s2i, ctypes.c_uint32), oracle._p(S, ctypes.c_int32),
+                               oracle._p(DF, ctypes.c_double), oracle._p(o.RL32, ctypes.c_uint32), s.spacing,
+                               oracle._p(o.lattice, ctypes.c_uint32), oracle._p(o.sites, ctypes.c_uint8), 11, r + 1)
+        assert h
+        assert lib.orc_nsm_run(h, steps * s.tau, 1 << 62, 0.0, 0, None, None) == 0
+        lib.orc_nsm_counts(h, oracle._p(counts, ctypes.c_uint32))
+        lib.orc_nsm_destroy(h)
+        c_nsm[r] = counts[2]
+    diff = c_mpd.mean() - c_nsm.mean()
+    se = math.sqrt(c_mpd.var(ddof=1) / R_mpd + c_nsm.var(ddof=1) / R_nsm)
+    assert abs(diff) < Z99 * se, f"MPD {c_mpd.mean():.3f} vs NSM {c_nsm.mean():.3f} (99% half-width {Z99 * se:.3f})"
+    assert c_mpd.mean() > 3.0  # the reaction has progressed: the comparison is not between two zeros
+    # variance ratio within the 99% F interval
+    from scipy import stats
+    f = c_mpd.var(ddof=1) / c_nsm.var(ddof=1)
+    lo, hi = stats.f.ppf([0.005, 0.995], R_mpd - 1, R_nsm - 1)
+    assert lo <= f <= hi, f"variance ratio {f:.3f} outside [{lo:.3f}, {hi:.3f}]"
