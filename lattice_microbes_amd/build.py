@@ -18,11 +18,15 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_build")
 LIB = os.path.join(OUT_DIR, "liblmx_mpd.so")
+IO_LIB = os.path.join(OUT_DIR, "liblmx_io.so")
 
 SOURCES = ["mpd_kernels.hip", "mpd_fast.hip", "mpd_capi.cpp"]
 HEADERS = ["lmx_device.h", "lmx_kernels.h", "lmx_site.h"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+CXX = os.environ.get("CXX", "g++")
+# The HDF5 C library (1.10, with the high-level API) of this image.
+HDF5_PREFIX = os.environ.get("HDF5_PREFIX", "/opt/conda")
 ARCH = "gfx950"
 
 COMMON_FLAGS = [
@@ -47,5 +51,26 @@ def build_library(verbose: bool = False) -> str:
     return LIB
 
 
+def build_io_library(verbose: bool = False) -> str:
+    """The SimulationFile (HDF5) library: host C++ only, built with g++."""
+    inc, libdir = os.path.join(HDF5_PREFIX, "include"), os.path.join(HDF5_PREFIX, "lib")
+    if not os.path.exists(os.path.join(inc, "hdf5_hl.h")):
+        raise RuntimeError(f"HDF5 headers not found under {inc}; set HDF5_PREFIX")
+    os.makedirs(OUT_DIR, exist_ok=True)
+    tmp = IO_LIB + ".tmp"
+    cmd = [CXX, "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-Wextra", "-I" + os.path.join(ROOT, "include"),
+           "-I" + inc, os.path.join(CSRC, "lmx_io.cpp"), "-o", tmp, "-L" + libdir, "-lhdf5_hl", "-lhdf5",
+           "-Wl,-rpath," + libdir, "-Wl,-z,defs"]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError("g++ failed building liblmx_io.so")
+    os.replace(tmp, IO_LIB)
+    return IO_LIB
+
+
 if __name__ == "__main__":
     print(build_library(verbose=True))
+    print(build_io_library(verbose=True))

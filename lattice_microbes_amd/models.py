@@ -77,6 +77,66 @@ class ModelSpec:
                                    self.lattice, self.sites, True)
         return solver
 
+    # ---------------------------------------------------------------- files
+    def reaction_model(self):
+        """The ReactionModel message of this model (one rate constant each)."""
+        from .simfile import ReactionModel
+
+        return ReactionModel(number_species=self.nS, number_reactions=self.R,
+                             initial_species_count=self.initial_counts(), types=self.types.astype(np.uint32),
+                             rate_constants=[[float(k)] for k in self.K], S=self.S.astype(np.int32),
+                             D=self.D.astype(np.uint32))
+
+    def diffusion_model(self):
+        """The DiffusionModel message of this model."""
+        from .simfile import DiffusionModel
+
+        X, Y, Z = self.shape
+        return DiffusionModel(number_species=self.nS, number_reactions=self.R, number_site_types=self.nST,
+                              lattice_spacing=self.spacing, lattice_x_size=X, lattice_y_size=Y, lattice_z_size=Z,
+                              particles_per_site=self.P, bytes_per_particle=1,
+                              diffusion_matrix=self.DF.astype(np.float64),
+                              reaction_location_matrix=self.RL.astype(np.uint32))
+
+    def to_file(self, path: str, parameters: Optional[Dict[str, str]] = None) -> None:
+        """A new Lattice Microbes HDF5 file holding this model, its initial
+        lattice and the given /Parameters (``timestep`` is always written):
+        what pyLM/jLM's ``save`` produces for the solver to run."""
+        from .simfile import SimulationFile
+
+        SimulationFile.create(path)
+        prm = {"timestep": repr(self.tau)}
+        prm.update({k: str(v) for k, v in (parameters or {}).items()})
+        with SimulationFile(path) as f:
+            for k, v in prm.items():
+                f.setParameter(k, v)
+            f.setReactionModel(self.reaction_model())
+            f.setDiffusionModel(self.diffusion_model(), self.lattice, self.sites)
+
+    @classmethod
+    def from_file(cls, path: str, name: Optional[str] = None, boundary: str = "periodic") -> "ModelSpec":
+        """The model, initial lattice and ``timestep`` of a Lattice Microbes
+        HDF5 file.  Reactions keep their first rate constant, the only one the
+        MPD-RDME solver reads."""
+        from .simfile import SimulationFile
+
+        with SimulationFile(path, writable=False) as f:
+            tau = float(f.getParameter("timestep", "0") or 0)
+            rm = f.getReactionModel()
+            dm = f.getDiffusionModel()
+            lattice, sites = f.getDiffusionModelLattice(dm)
+        if lattice.dtype != np.uint8:
+            raise ValueError("only byte lattices (bytes_per_particle 1) are supported by the MPD-RDME solver")
+        R = rm.number_reactions
+        K = np.array([ks[0] if ks else 0.0 for ks in rm.rate_constants], dtype=np.float64)
+        return cls(name=name or path, shape=dm.shape, spacing=dm.lattice_spacing, tau=tau,
+                   P=dm.particles_per_site, nS=rm.number_species, nST=dm.number_site_types,
+                   types=rm.types.astype(np.uint32).reshape(R), K=K.reshape(R),
+                   S=rm.S.astype(np.int32).reshape(rm.number_species, R),
+                   D=rm.D.astype(np.uint32).reshape(rm.number_species, R),
+                   DF=dm.diffusion_matrix, RL=dm.reaction_location_matrix.reshape(R, dm.number_site_types),
+                   lattice=lattice, sites=sites, boundary=boundary)
+
     def native_lattice(self) -> np.ndarray:
         """(P/4, Z, Y, X) uint32 planes, slots compacted in row-major order."""
         X, Y, Z = self.shape

@@ -249,70 +249,313 @@ class MpdRdmeSolver:
         _lib.check(self._lib.lmx_get_table_f32(self._h, idx, _ptr(out, ctypes.c_float), out.size, byref(n)))
         return out[: n.value]
 
+    # ------------------------------------------------------------------ models from a file
+    def setReactionModel(self, rm) -> None:
+        """CMESolver::setReactionModel (CMESolver.cpp:278-305): buildModel from a
+        ReactionModel (``simfile.ReactionModel``), K with as many columns as the
+        reaction with the most rate constants."""
+        R = int(rm.number_reactions)
+        if R != len(rm.types) or R != len(rm.rate_constants):
+            raise _lib.InvalidArgException(_lib.ERR_ARG, "rm: number of reactions does not agree with reaction list size")
+        kCols = max([len(k) for k in rm.rate_constants] + [0])
+        K = np.zeros((R, max(kCols, 1)))
+        for r, ks in enumerate(rm.rate_constants):
+            K[r, :len(ks)] = ks
+        self.buildModel(rm.number_species, R, rm.initial_species_count, rm.types, K, rm.S, rm.D, max(kCols, 1))
+
+    def setDiffusionModel(self, dm, lattice, latticeSites) -> None:
+        """RDMESolver::setDiffusionModel (RDMESolver.cpp:97-108): a
+        DiffusionModel (``simfile.DiffusionModel``) with its row-major lattice
+        and site arrays, checked against the reaction model's sizes."""
+        if dm.number_species != self.numberSpecies:
+            raise _lib.InvalidArgException(_lib.ERR_ARG, "dm.number_species: number of species in the diffusion model does not agree with the number in the reaction model")
+        if dm.number_reactions != self.numberReactions:
+            raise _lib.InvalidArgException(_lib.ERR_ARG, "dm.number_reactions: number of reactions in the diffusion model does not agree with the number in the reaction model")
+        RL = dm.reaction_location_matrix if dm.number_reactions else np.zeros(0, np.uint32)
+        self.buildDiffusionModel(dm.number_site_types, dm.diffusion_matrix, RL, dm.lattice_x_size,
+                                 dm.lattice_y_size, dm.lattice_z_size, dm.particles_per_site,
+                                 dm.bytes_per_particle, dm.lattice_spacing, lattice, latticeSites, True)
+
     # ------------------------------------------------------------------ trajectory
-    def hookSimulation(self, time: float, solver: "MpdRdmeSolver") -> int:
-        """Overridable; return codes as MpdRdmeSolver::hookCheckSimulation
-        (MpdRdmeSolver.cu:958-985): 0 unchanged, 1 lattice changed (already
-        written back with setLattice), 2 also sites changed, 3 "stop" (which,
-        as in the reference, does not stop)."""
+    def hookSimulation(self, time: float, lattice: "HookLattice") -> int:
+        """Overridable (MpdRdmeSolver.cu:789-799).  ``lattice`` is a host copy
+        of the solver's lattice; return 0 if unchanged, 1 if the particles or
+        sites were changed (they are copied back to the GPU), 2 if so and the
+        sites are also to be recorded, 3 to record the sites without copying
+        back (hookCheckSimulation, MpdRdmeSolver.cu:958-985; as there, 3 does
+        not end the trajectory).  setReactionRate may be called from here."""
         return 0
 
-    def onBeginTrajectory(self, solver: "MpdRdmeSolver") -> int:
+    def onBeginTrajectory(self, lattice: "HookLattice") -> int:
+        """Overridable (MpdRdmeSolver.cu:813-816); changes to ``lattice`` are
+        copied to the GPU before the first step."""
         return 0
 
-    def onEndTrajectory(self, solver: "MpdRdmeSolver") -> int:
+    def onEndTrajectory(self, lattice: "HookLattice") -> int:
+        """Overridable (MpdRdmeSolver.cu:818-821); sees the final lattice."""
         return 0
 
-    def generateTrajectory(self) -> Dict[str, object]:
-        """MpdRdmeSolver::generateTrajectory (MpdRdmeSolver.cu:485-668) without
-        file output: species counts are recorded every ``writeInterval`` steps,
-        lattices every ``latticeWriteInterval`` steps and the hook runs every
-        ``hookInterval`` steps.  Returns the recorded data."""
+    def generateTrajectory(self, output=None) -> Dict[str, object]:
+        """MpdRdmeSolver::generateTrajectory (MpdRdmeSolver.cu:485-668).
+
+        Species counts are recorded at time 0 and every ``writeInterval``
+        steps, lattices at time 0 and every ``latticeWriteInterval`` steps, the
+        hook runs at time 0 and every ``hookInterval`` steps (all intervals in
+        steps, as the reference compares them with the step counter), until
+        ``maxTime``.  Steps between events run back to back on the GPU; the
+        lattice comes to the host only at an event.  With ``output`` (an open
+        ``simfile.SimulationFile``) the records go to replicate
+        ``self.replicate`` of that file the way the reference's output queue
+        writes them: SpeciesCounts/SpeciesCountTimes, Lattice/LatticeTimes,
+        Sites/SiteTimes (hook codes 2 and 3) and, at the end,
+        MaxParticleCounts and MaxSiteCounts.  Returns the same records."""
         prm = self.parameters
         maxTime = float(prm.get("maxTime", "0") or 0)
         wi = int(float(prm.get("writeInterval", "0") or 0))
         lwi = int(float(prm.get("latticeWriteInterval", "0") or 0))
         hook_iv = prm.get("hookInterval", "")
-        hi = int(float(hook_iv)) if hook_iv not in ("", None) else 0
         hook_enabled = hook_iv not in ("", None)
-        wi, lwi, hi = max(wi, 1), max(lwi, 1), max(hi, 1)  # an interval of 0 steps fires every step
-        tau = self.tau
-        counts_t: List[float] = [0.0]
-        counts: List[np.ndarray] = []
-        lattice_t: List[float] = [0.0]
-        lattices: List[np.ndarray] = []
-        self.onBeginTrajectory(self)
-        counts.append(self.getSpeciesCounts())
-        lattices.append(self.getLattice())
-        t = 0
-        if hook_enabled:
-            self.hookSimulation(0.0, self)
+        hi = int(float(hook_iv)) if hook_enabled else 0
+        rec = _Recorder(self, output)
+        lattice = HookLattice(self)
+
+        lattice.copyFromGPU()
+        self.onBeginTrajectory(lattice)
+        lattice.copyToGPU()
+        time, t = 0.0, 0
+        rec.species_counts(time, lattice)
+        rec.lattice(time, lattice)
+        rec.init_max_counts(lattice)
+        self._hook_check(time, lattice, rec)
+
         next_c, next_l, next_h = wi, lwi, hi
-        time = 0.0
+        end = _last_step(self.tau, maxTime)
         while time < maxTime:
-            # Run until the next event step in one batch (no per-step host sync).
-            targets = [x for x in (next_c, next_l, next_h if hook_enabled else None) if x is not None and x > t]
-            stop = int(np.ceil(maxTime / tau)) if tau > 0 else t + 1
-            nxt = min(targets + [stop]) if targets else stop
-            n = max(nxt - t, 1)
-            self.runTimesteps(n, synchronize=True)
-            t += n
-            time = t * tau
-            if t >= next_c or t >= next_l or (hook_enabled and t >= next_h):
-                if hook_enabled and t >= next_h:
-                    self.hookSimulation(time, self)
-                    next_h += hi
-                if t >= next_l:
-                    lattice_t.append(time)
-                    lattices.append(self.getLattice())
-                    next_l += lwi
-                if t >= next_c:
-                    counts_t.append(time)
-                    counts.append(self.getSpeciesCounts())
-                    next_c += wi
-        self.onEndTrajectory(self)
-        return {"species_count_times": np.array(counts_t), "species_counts": np.array(counts),
-                "lattice_times": np.array(lattice_t), "lattices": lattices}
+            # Run to the next step at which anything happens (an interval of 0
+            # fires after every step, as in the reference) or the last step.
+            due = min([next_c, next_l] + ([next_h] if hook_enabled else []))
+            stop = min(max(due, t + 1), max(end, t + 1))
+            self.runTimesteps(stop - t, synchronize=True)
+            t = stop
+            time = t * self.tau
+            # The lattice comes to the host for a hook or a lattice frame; counts
+            # alone are taken on the GPU (the same numbers as counting the
+            # host copy, which then equals the device lattice).
+            host = t >= next_l or (hook_enabled and t >= next_h)
+            if host:
+                lattice.copyFromGPU()
+            if hook_enabled and t >= next_h:
+                self._hook_check(time, lattice, rec)
+                next_h += hi
+            if t >= next_l:
+                rec.lattice(time, lattice)
+                next_l += lwi
+            if t >= next_c:
+                rec.species_counts(time, lattice if host else None)
+                next_c += wi
+        lattice.copyFromGPU()
+        self.onEndTrajectory(lattice)
+        return rec.finish()
+
+    def _hook_check(self, time: float, lattice: "HookLattice", rec: "_Recorder") -> None:
+        code = self.hookSimulation(time, lattice)
+        if code == 0:
+            return
+        if code == 1:
+            lattice.copyToGPU()
+        elif code == 2:
+            lattice.copyToGPU()
+            rec.sites(time, lattice)
+        elif code == 3:
+            rec.sites(time, lattice)
+        else:
+            raise _lib.LmxError(_lib.ERR_ARG, f"Unknown hook return value {code}")
+
+
+def _last_step(tau: float, maxTime: float) -> int:
+    """The step after which ``step * tau < maxTime`` first fails: the
+    reference's loop (``while (time < maxTime)`` with ``time =
+    current_timestep * tau``) runs exactly this many steps."""
+    if not maxTime > 0.0:
+        return 0
+    k = max(int(np.ceil(maxTime / tau)), 1)
+    while k > 1 and (k - 1) * tau >= maxTime:
+        k -= 1
+    while k * tau < maxTime:
+        k += 1
+    return k
+
+
+class HookLattice:
+    """The lattice a hook sees, in place of ``lm::rdme::CudaByteLattice``: a
+    host copy of the solver's particles and site types, with the accessors the
+    reference's Python module gives it (src/swig/lm.i:875-896).
+
+    ``getParticleLatticeView()`` is the (W, Z, Y, X, 4) uint8 array of the
+    device layout (particle slot 4w+b of site (x, y, z) at [w, z, y, x, b]) and
+    ``getSiteLatticeView()`` the (Z, Y, X) site types, both writable; changes
+    reach the GPU when the hook returns 1 or 2."""
+
+    def __init__(self, solver: MpdRdmeSolver):
+        self._solver = solver
+        self._words: Optional[np.ndarray] = None
+        self._sites: Optional[np.ndarray] = None
+
+    def copyFromGPU(self) -> None:
+        self._words = self._solver.getLatticeNative()
+        self._sites = np.ascontiguousarray(np.transpose(self._solver.getSites(), (2, 1, 0)))
+
+    def copyToGPU(self) -> None:
+        self._solver.setLatticeNative(self._words)
+        self._solver.setSites(np.transpose(self._sites, (2, 1, 0)))
+
+    # views
+    def getParticleLatticeView(self) -> np.ndarray:
+        W, Z, Y, X = self._words.shape
+        return self._words.view(np.uint8).reshape(W, Z, Y, X, 4)
+
+    def getSiteLatticeView(self) -> np.ndarray:
+        return self._sites
+
+    # geometry
+    def getXSize(self) -> int:
+        return self._solver.shape[0]
+
+    def getYSize(self) -> int:
+        return self._solver.shape[1]
+
+    def getZSize(self) -> int:
+        return self._solver.shape[2]
+
+    def getSize(self):
+        return tuple(self._solver.shape)
+
+    def getNumberSites(self) -> int:
+        X, Y, Z = self._solver.shape
+        return X * Y * Z
+
+    def getMaxOccupancy(self) -> int:
+        return self._solver.particlesPerSite
+
+    def getSpacing(self) -> float:
+        return self._solver.latticeSpacing
+
+    # site access (ByteLattice, src/rdme/ByteLattice.cpp)
+    def _slots(self, x: int, y: int, z: int) -> np.ndarray:
+        return self.getParticleLatticeView()[:, z, y, x, :].reshape(-1)
+
+    def getSiteType(self, x: int, y: int, z: int) -> int:
+        return int(self._sites[z, y, x])
+
+    def setSiteType(self, x: int, y: int, z: int, siteType: int) -> None:
+        self._sites[z, y, x] = siteType
+
+    def getOccupancy(self, x: int, y: int, z: int) -> int:
+        return int(np.count_nonzero(self._slots(x, y, z)))
+
+    def getParticle(self, x: int, y: int, z: int, particleIndex: int) -> int:
+        s = self._slots(x, y, z)
+        if particleIndex >= s.size or s[particleIndex] == 0:
+            raise _lib.InvalidArgException(_lib.ERR_ARG, "InvalidParticleException: no particle at that index")
+        return int(s[particleIndex])
+
+    def addParticle(self, x: int, y: int, z: int, particle: int) -> None:
+        view = self.getParticleLatticeView()
+        s = self._slots(x, y, z)
+        k = int(np.count_nonzero(s))
+        if k >= s.size:
+            raise _lib.InvalidArgException(_lib.ERR_ARG, "InvalidParticleException: the site is full")
+        view[k // 4, z, y, x, k % 4] = particle
+
+    def removeParticles(self, x: int, y: int, z: int) -> None:
+        self.getParticleLatticeView()[:, z, y, x, :] = 0
+
+    def getParticleCounts(self) -> Dict[int, int]:
+        c = np.bincount(self.getParticleLatticeView().reshape(-1), minlength=256)
+        return {int(p): int(c[p]) for p in np.flatnonzero(c) if p != 0}
+
+    def row_major(self) -> np.ndarray:
+        """Particles as row-major (X, Y, Z, P) bytes, the file layout."""
+        v = self.getParticleLatticeView()
+        W, Z, Y, X, _ = v.shape
+        return np.ascontiguousarray(np.transpose(v, (3, 2, 1, 0, 4)).reshape(X, Y, Z, 4 * W))
+
+    def sites_row_major(self) -> np.ndarray:
+        return np.ascontiguousarray(np.transpose(self._sites, (2, 1, 0)))
+
+
+class _Recorder:
+    """What the reference's solver pushes to its output queue during a
+    trajectory (MpdRdmeSolver.cu:824-956), kept in memory and, with a file,
+    written to it."""
+
+    def __init__(self, solver: MpdRdmeSolver, output):
+        self.solver = solver
+        self.out = output
+        self.counts_t: List[float] = []
+        self.counts: List[np.ndarray] = []
+        self.lattice_t: List[float] = []
+        self.lattices: List[np.ndarray] = []
+        self.site_t: List[float] = []
+        self.site_frames: List[np.ndarray] = []
+        self.max_particles = np.zeros(solver.numberSpecies + 1, np.uint32)
+        self.max_sites = np.zeros(solver.numberSiteTypes, np.uint32)
+
+    def _particle_counts(self, lattice: HookLattice) -> np.ndarray:
+        c = np.bincount(lattice.getParticleLatticeView().reshape(-1), minlength=256)[: self.max_particles.size]
+        c[0] = 0
+        return c.astype(np.uint32)
+
+    def _site_counts(self, lattice: HookLattice) -> np.ndarray:
+        c = np.bincount(lattice.getSiteLatticeView().reshape(-1), minlength=256)[: self.max_sites.size]
+        if c.size:
+            c[0] = 0
+        return c.astype(np.uint32)
+
+    def init_max_counts(self, lattice: HookLattice) -> None:
+        np.maximum(self.max_sites, self._site_counts(lattice), out=self.max_sites)
+        np.maximum(self.max_particles, self._particle_counts(lattice), out=self.max_particles)
+
+    def species_counts(self, time: float, lattice: Optional[HookLattice] = None) -> None:
+        """recordSpeciesCounts (MpdRdmeSolver.cu:933-942): particles of types
+        1..numberSpecies, from the host copy when there is one."""
+        self.counts_t.append(time)
+        if lattice is None:
+            self.counts.append(self.solver.getSpeciesCounts())
+        else:
+            n = self.solver.numberSpecies
+            c = np.bincount(lattice.getParticleLatticeView().reshape(-1), minlength=256)
+            self.counts.append(c[1:n + 1].astype(np.uint32))
+
+    def lattice(self, time: float, lattice: HookLattice) -> None:
+        np.maximum(self.max_particles, self._particle_counts(lattice), out=self.max_particles)
+        frame = lattice.row_major()
+        self.lattice_t.append(time)
+        self.lattices.append(frame)
+        if self.out is not None:
+            self.out.appendLattice(self.solver.replicate, time, frame)
+
+    def sites(self, time: float, lattice: HookLattice) -> None:
+        np.maximum(self.max_sites, self._site_counts(lattice), out=self.max_sites)
+        frame = lattice.sites_row_major()
+        self.site_t.append(time)
+        self.site_frames.append(frame)
+        if self.out is not None:
+            self.out.appendSites(self.solver.replicate, time, frame)
+
+    def finish(self) -> Dict[str, object]:
+        counts = np.array(self.counts, dtype=np.uint32).reshape(len(self.counts), self.solver.numberSpecies)
+        if self.out is not None:
+            r = self.solver.replicate
+            if len(self.counts):
+                self.out.appendSpeciesCounts(r, np.array(self.counts_t), counts)
+            self.out.writeReplicateDataset(r, "MaxParticleCounts", self.max_particles)
+            if self.max_sites.size:
+                self.out.writeReplicateDataset(r, "MaxSiteCounts", self.max_sites)
+        return {"species_count_times": np.array(self.counts_t), "species_counts": counts,
+                "lattice_times": np.array(self.lattice_t), "lattices": self.lattices,
+                "site_times": np.array(self.site_t), "sites": self.site_frames,
+                "max_particle_counts": self.max_particles.copy(), "max_site_counts": self.max_sites.copy()}
 
 
 class MESolverFactory:

@@ -1,5 +1,5 @@
-"""The C-ABI library loads and exports every entry point include/lmx_mpd.h
-declares (no compute calls: this runs without a GPU)."""
+"""The C-ABI libraries load and export every entry point include/lmx_mpd.h and
+include/lmx_io.h declare (no compute calls: this runs without a GPU)."""
 import os
 import re
 
@@ -45,3 +45,20 @@ def test_create_without_device_fails_loudly():
     with pytest.raises(lmx.LmxError) as ei:
         lmx.MpdRdmeSolver(0)
     assert "device" in str(ei.value)
+
+
+def declared_io_functions():
+    text = open(os.path.join(ROOT, "include", "lmx_io.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(lmxio_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_io_library_exports_every_declared_symbol():
+    from lattice_microbes_amd import simfile
+
+    names = declared_io_functions()
+    assert "lmxio_open" in names and "lmxio_append_lattice" in names
+    lib = simfile.load()
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    assert set(names) <= set(simfile.SIGNATURES)

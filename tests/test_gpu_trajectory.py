@@ -1,0 +1,85 @@
+"""Trajectories into Lattice Microbes HDF5 files, and the hook protocol, on
+the GPU, checked bit for bit against the oracle stepped to the same steps.
+
+runSimulation reads the model, parameters and initial lattice from the file
+and writes species counts, lattice frames, site frames and the max-count
+datasets back into it (MpdRdmeSolver.cu:485-668, 824-985); a solver subclass
+overriding hookSimulation / onBeginTrajectory changes the lattice, the site
+types and a rate constant mid-run, and the oracle applies the same changes at
+the same steps.
+"""
+import numpy as np
+import pytest
+
+import lattice_microbes_amd as lmx
+from lattice_microbes_amd.runner import runSimulation, runSolver
+from lattice_microbes_amd.simfile import SimulationFile
+import oracle
+from trajectory_cases import (HOOK_HI, HOOK_LWI, HOOK_STEPS, HOOK_WI, SEED, EditingHooks, expected_hooked,
+                              expected_plain, small_spec, type_counts)
+
+pytestmark = pytest.mark.gpu
+
+
+def test_run_simulation_writes_trajectory(tmp_path):
+    spec = small_spec()
+    path = str(tmp_path / "traj.lm")
+    steps, wi, lwi = 23, 5, 7
+    spec.to_file(path, {"seed": SEED, "maxTime": repr(steps * spec.tau), "writeInterval": wi,
+                        "latticeWriteInterval": lwi})
+    assert runSimulation(path, 3)
+    counts, frames, maxp = expected_plain(spec, 3, steps, wi, lwi)
+    with SimulationFile(path, writable=False) as f:
+        assert f.replicateExists(3) and not f.replicateExists(0)
+        np.testing.assert_array_equal(f.getSpeciesCountTimes(3), [t * spec.tau for t in sorted(counts)])
+        np.testing.assert_array_equal(f.getSpeciesCounts(3), np.array([counts[t] for t in sorted(counts)]))
+        np.testing.assert_array_equal(f.getLatticeTimes(3), [t * spec.tau for t in sorted(frames)])
+        for k, t in enumerate(sorted(frames)):
+            np.testing.assert_array_equal(f.getLattice(3, k), frames[t], err_msg=f"frame at step {t}")
+        np.testing.assert_array_equal(f.getReplicateDataset(3, "MaxParticleCounts"), maxp)
+        np.testing.assert_array_equal(f.getReplicateDataset(3, "MaxSiteCounts"), type_counts(spec.sites, spec.nST))
+
+
+def test_second_replicate_and_bad_file(tmp_path, capsys):
+    spec = small_spec()
+    path = str(tmp_path / "two.lm")
+    spec.to_file(path, {"seed": SEED, "maxTime": repr(6 * spec.tau), "writeInterval": 3,
+                        "latticeWriteInterval": 6})
+    assert runSimulation(path, 0) and runSimulation(path, 1)
+    with SimulationFile(path, writable=False) as f:
+        for r in (0, 1):
+            orc = oracle.Oracle(spec, seed_top=SEED, replicate=r)
+            assert orc.run(6) == 0
+            np.testing.assert_array_equal(f.getLattice(r, 1), orc.row_major())
+            np.testing.assert_array_equal(f.getSpeciesCounts(r)[-1], orc.species_counts())
+        assert not np.array_equal(f.getLattice(0, 1), f.getLattice(1, 1))
+    # an unsupported parameter is reported the way the reference does, not raised
+    bad = str(tmp_path / "bad.lm")
+    spec.to_file(bad, {"seed": SEED, "maxTime": repr(2 * spec.tau), "rdme.mpd.overflowhandler": "other"})
+    assert not runSimulation(bad, 0)
+    assert "Exception during execution of replicate" in capsys.readouterr().err
+
+
+class EditingSolver(EditingHooks, lmx.MpdRdmeSolver):
+    pass
+
+
+def test_hook_protocol_matches_oracle(tmp_path):
+    spec = small_spec()
+    path = str(tmp_path / "hook.lm")
+    spec.to_file(path, {"seed": SEED, "maxTime": repr(HOOK_STEPS * spec.tau), "writeInterval": HOOK_WI,
+                        "latticeWriteInterval": HOOK_LWI, "hookInterval": HOOK_HI})
+    solver = EditingSolver(0)
+    try:
+        assert runSolver(path, 5, solver)
+        assert solver.calls == [0, 4, 8, 12, 16]
+        frames, counts, site_steps, final = expected_hooked(spec, 5)
+        with SimulationFile(path, writable=False) as f:
+            assert len(f.getLatticeTimes(5)) == len(frames)
+            for k, fr in enumerate(frames):
+                np.testing.assert_array_equal(f.getLattice(5, k), fr, err_msg=f"frame {k}")
+            np.testing.assert_array_equal(f.getSpeciesCounts(5), counts)
+        np.testing.assert_array_equal(solver.ended.reshape(-1), final)
+        assert site_steps == [8, 16]
+    finally:
+        solver.close()
