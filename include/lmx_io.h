@@ -93,11 +93,19 @@ int lmxio_get_diffusion_info(lmxio_file *f, lmxio_diffusion_info *info);
  * sites row-major (x,y,z) bytes.  Any output may be NULL. */
 int lmxio_get_diffusion_model(lmxio_file *f, double *DF, uint32_t *RL, void *lattice, size_t latticeBytes,
                               uint8_t *sites, size_t sitesBytes);
-/* setDiffusionModel + setDiffusionModelLattice (SimulationFile.cpp:696-761,
- * :763-870); lattice or sites may be NULL (not written). */
+/* setDiffusionModel (SimulationFile.cpp:696-761), and with lattice and sites
+ * (either may be NULL: not written) setDiffusionModelLattice. */
 int lmxio_set_diffusion_model(lmxio_file *f, uint32_t nS, uint32_t R, uint32_t nST, const double *DF,
                               const uint32_t *RL, double latticeSpacing, uint32_t X, uint32_t Y, uint32_t Z,
                               uint32_t P, uint32_t bytesPerParticle, const void *lattice, const uint8_t *sites);
+/* setDiffusionModelLattice (SimulationFile.cpp:763-870) into an existing
+ * /Model/Diffusion, replacing Lattice and LatticeSites. */
+int lmxio_set_diffusion_model_lattice(lmxio_file *f, uint32_t X, uint32_t Y, uint32_t Z, uint32_t P,
+                                      uint32_t bytesPerParticle, const void *lattice, const uint8_t *sites);
+/* setSpatialModel (SimulationFile.cpp:970-1060) for a model without regions
+ * or obstacles: /Model/Spatial/Regions (numberRegions 0) and
+ * /Model/Spatial/Obstacles (numberObstacles 0). */
+int lmxio_set_empty_spatial_model(lmxio_file *f);
 
 /* Trajectories in /Simulations/%07d: appendSpeciesCounts
  * (SimulationFile.cpp:1090-1164, SpeciesCounts I32 [n][nS] and

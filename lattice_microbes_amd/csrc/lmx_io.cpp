@@ -658,11 +658,44 @@ int lmxio_set_diffusion_model(lmxio_file *f, uint32_t nS, uint32_t R, uint32_t n
         ok = H5LTmake_dataset(h, "/Model/Diffusion/ReactionLocationMatrix", 2, dims, H5T_STD_U32LE, RL) >= 0;
     }
     if (!ok) return fail(LMXIO_ERR_IO, "cannot write the diffusion model");
-    const hid_t dg = H5Gopen2(h, g, H5P_DEFAULT);
+    return lmxio_set_diffusion_model_lattice(f, X, Y, Z, P, bytesPerParticle, lattice, sites);
+}
+
+int lmxio_set_diffusion_model_lattice(lmxio_file *f, uint32_t X, uint32_t Y, uint32_t Z, uint32_t P,
+                                      uint32_t bytesPerParticle, const void *lattice, const uint8_t *sites)
+{
+    if (!f || !X || !Y || !Z || !P) return fail(LMXIO_ERR_ARG, "invalid lattice arguments");
+    hid_t ft, mt;
+    int rc = lattice_types(bytesPerParticle, &ft, &mt);
+    if (rc) return rc;
+    if (!f->writable) return fail(LMXIO_ERR_ARG, "file opened read-only");
+    const char *g = "/Model/Diffusion";
+    if (!link_exists(f->file, g)) return fail(LMXIO_ERR_FORMAT, "the file has no diffusion model");
+    const hid_t dg = H5Gopen2(f->file, g, H5P_DEFAULT);
+    if (dg < 0) return fail(LMXIO_ERR_IO, "cannot open /Model/Diffusion");
     const hsize_t ldims[4] = {X, Y, Z, P};
+    if (lattice && H5Lexists(dg, "Lattice", H5P_DEFAULT) > 0) H5Ldelete(dg, "Lattice", H5P_DEFAULT);
+    if (sites && H5Lexists(dg, "LatticeSites", H5P_DEFAULT) > 0) H5Ldelete(dg, "LatticeSites", H5P_DEFAULT);
     if (lattice) rc = write_frame(dg, "Lattice", 4, ldims, ft, mt, lattice);
     if (!rc && sites) rc = write_frame(dg, "LatticeSites", 3, ldims, H5T_STD_U8LE, H5T_NATIVE_UINT8, sites);
     H5Gclose(dg);
+    return rc;
+}
+
+int lmxio_set_empty_spatial_model(lmxio_file *f)
+{
+    if (!f) return fail(LMXIO_ERR_ARG, "null argument");
+    if (!f->writable) return fail(LMXIO_ERR_ARG, "file opened read-only");
+    const hid_t h = f->file;
+    if (H5Lexists(h, "/Model/Spatial", H5P_DEFAULT) > 0 && H5Ldelete(h, "/Model/Spatial", H5P_DEFAULT) < 0)
+        return fail(LMXIO_ERR_IO, "cannot replace /Model/Spatial");
+    int rc = make_group(h, "/Model/Spatial");
+    if (!rc) rc = make_group(h, "/Model/Spatial/Regions");
+    if (!rc) rc = make_group(h, "/Model/Spatial/Obstacles");
+    const unsigned int zero = 0;
+    if (!rc && (H5LTset_attribute_uint(h, "/Model/Spatial/Regions", "numberRegions", &zero, 1) < 0 ||
+                H5LTset_attribute_uint(h, "/Model/Spatial/Obstacles", "numberObstacles", &zero, 1) < 0))
+        rc = fail(LMXIO_ERR_IO, "cannot write the spatial model");
     return rc;
 }
 

@@ -59,6 +59,9 @@ SIGNATURES = {
     "lmxio_get_diffusion_model": (c_int, [c_void_p, _f64p, _u32p, c_void_p, c_size_t, _u8p, c_size_t]),
     "lmxio_set_diffusion_model": (c_int, [c_void_p, c_uint32, c_uint32, c_uint32, _f64p, _u32p, c_double, c_uint32,
                                           c_uint32, c_uint32, c_uint32, c_uint32, c_void_p, _u8p]),
+    "lmxio_set_diffusion_model_lattice": (c_int, [c_void_p, c_uint32, c_uint32, c_uint32, c_uint32, c_uint32,
+                                                  c_void_p, _u8p]),
+    "lmxio_set_empty_spatial_model": (c_int, [c_void_p]),
     "lmxio_append_species_counts": (c_int, [c_void_p, c_uint32, c_uint32, c_uint32, _f64p, _u32p]),
     "lmxio_append_lattice": (c_int, [c_void_p, c_uint32, c_double, c_uint32, c_uint32, c_uint32, c_uint32, c_uint32,
                                      c_void_p]),
@@ -336,9 +339,8 @@ class SimulationFile:
 
     def setDiffusionModel(self, m: DiffusionModel, lattice: Optional[np.ndarray] = None,
                           sites: Optional[np.ndarray] = None) -> None:
-        """setDiffusionModel, and setDiffusionModelLattice when lattice and
-        sites are given (the reference's two calls; both replace
-        /Model/Diffusion, so the lattice must come with the model here)."""
+        """setDiffusionModel (replaces /Model/Diffusion), and with lattice
+        and sites also setDiffusionModelLattice."""
         nST, nS, R = int(m.number_site_types), int(m.number_species), int(m.number_reactions)
         DF = np.ascontiguousarray(m.diffusion_matrix, dtype=np.float64).reshape(-1)
         if DF.size != nST * nST * nS:
@@ -361,6 +363,22 @@ class SimulationFile:
             site_p = _p(sites, ctypes.c_uint8)
         _check(self._lib.lmxio_set_diffusion_model(self._h, nS, R, nST, _p(DF, c_double), _p(RL, c_uint32),
                                                    float(m.lattice_spacing), X, Y, Z, P, bpp, lat_p, site_p))
+
+    def setDiffusionModelLattice(self, m: DiffusionModel, lattice: np.ndarray, sites: np.ndarray) -> None:
+        """The initial lattice [X, Y, Z, P] and sites [X, Y, Z] (row-major) of
+        the diffusion model already in the file (SimulationFile.cpp:833-870)."""
+        X, Y, Z, P = m.lattice_x_size, m.lattice_y_size, m.lattice_z_size, m.particles_per_site
+        bpp = int(m.bytes_per_particle)
+        lattice = np.ascontiguousarray(lattice, dtype=np.uint8 if bpp == 1 else np.uint32)
+        sites = np.ascontiguousarray(sites, dtype=np.uint8)
+        if lattice.shape != (X, Y, Z, P) or sites.shape != (X, Y, Z):
+            raise InvalidArgException(ERR_ARG, "lattice or sites shape does not agree with the diffusion model")
+        _check(self._lib.lmxio_set_diffusion_model_lattice(self._h, X, Y, Z, P, bpp, lattice.ctypes.data_as(c_void_p),
+                                                           _p(sites, ctypes.c_uint8)))
+
+    def setEmptySpatialModel(self) -> None:
+        """setSpatialModel for a model with no regions or obstacles."""
+        _check(self._lib.lmxio_set_empty_spatial_model(self._h))
 
     # ---------------------------------------------------------------- trajectories
     def replicateExists(self, replicate: int) -> bool:

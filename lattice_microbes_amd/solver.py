@@ -16,6 +16,7 @@ from typing import Callable, Dict, List, Optional
 import numpy as np
 
 from . import _lib
+from .lattice import ByteLattice
 
 SOLVER_NAME = "lm::rdme::MpdRdmeSolver"
 
@@ -386,102 +387,25 @@ def _last_step(tau: float, maxTime: float) -> int:
     return k
 
 
-class HookLattice:
+class HookLattice(ByteLattice):
     """The lattice a hook sees, in place of ``lm::rdme::CudaByteLattice``: a
-    host copy of the solver's particles and site types, with the accessors the
-    reference's Python module gives it (src/swig/lm.i:875-896).
-
-    ``getParticleLatticeView()`` is the (W, Z, Y, X, 4) uint8 array of the
-    device layout (particle slot 4w+b of site (x, y, z) at [w, z, y, x, b]) and
-    ``getSiteLatticeView()`` the (Z, Y, X) site types, both writable; changes
-    reach the GPU when the hook returns 1 or 2."""
+    host copy of the solver's particles and site types (``ByteLattice``
+    accessors and views over it).  ``copyFromGPU``/``copyToGPU`` move it
+    between the host and the solver; the trajectory loop copies it back when a
+    hook returns 1 or 2."""
 
     def __init__(self, solver: MpdRdmeSolver):
+        X, Y, Z = solver.shape
+        super().__init__(X, Y, Z, solver.latticeSpacing, solver.particlesPerSite)
         self._solver = solver
-        self._words: Optional[np.ndarray] = None
-        self._sites: Optional[np.ndarray] = None
 
     def copyFromGPU(self) -> None:
-        self._words = self._solver.getLatticeNative()
-        self._sites = np.ascontiguousarray(np.transpose(self._solver.getSites(), (2, 1, 0)))
+        self._words[:] = self._solver.getLatticeNative()
+        self._sites[:] = np.transpose(self._solver.getSites(), (2, 1, 0))
 
     def copyToGPU(self) -> None:
         self._solver.setLatticeNative(self._words)
-        self._solver.setSites(np.transpose(self._sites, (2, 1, 0)))
-
-    # views
-    def getParticleLatticeView(self) -> np.ndarray:
-        W, Z, Y, X = self._words.shape
-        return self._words.view(np.uint8).reshape(W, Z, Y, X, 4)
-
-    def getSiteLatticeView(self) -> np.ndarray:
-        return self._sites
-
-    # geometry
-    def getXSize(self) -> int:
-        return self._solver.shape[0]
-
-    def getYSize(self) -> int:
-        return self._solver.shape[1]
-
-    def getZSize(self) -> int:
-        return self._solver.shape[2]
-
-    def getSize(self):
-        return tuple(self._solver.shape)
-
-    def getNumberSites(self) -> int:
-        X, Y, Z = self._solver.shape
-        return X * Y * Z
-
-    def getMaxOccupancy(self) -> int:
-        return self._solver.particlesPerSite
-
-    def getSpacing(self) -> float:
-        return self._solver.latticeSpacing
-
-    # site access (ByteLattice, src/rdme/ByteLattice.cpp)
-    def _slots(self, x: int, y: int, z: int) -> np.ndarray:
-        return self.getParticleLatticeView()[:, z, y, x, :].reshape(-1)
-
-    def getSiteType(self, x: int, y: int, z: int) -> int:
-        return int(self._sites[z, y, x])
-
-    def setSiteType(self, x: int, y: int, z: int, siteType: int) -> None:
-        self._sites[z, y, x] = siteType
-
-    def getOccupancy(self, x: int, y: int, z: int) -> int:
-        return int(np.count_nonzero(self._slots(x, y, z)))
-
-    def getParticle(self, x: int, y: int, z: int, particleIndex: int) -> int:
-        s = self._slots(x, y, z)
-        if particleIndex >= s.size or s[particleIndex] == 0:
-            raise _lib.InvalidArgException(_lib.ERR_ARG, "InvalidParticleException: no particle at that index")
-        return int(s[particleIndex])
-
-    def addParticle(self, x: int, y: int, z: int, particle: int) -> None:
-        view = self.getParticleLatticeView()
-        s = self._slots(x, y, z)
-        k = int(np.count_nonzero(s))
-        if k >= s.size:
-            raise _lib.InvalidArgException(_lib.ERR_ARG, "InvalidParticleException: the site is full")
-        view[k // 4, z, y, x, k % 4] = particle
-
-    def removeParticles(self, x: int, y: int, z: int) -> None:
-        self.getParticleLatticeView()[:, z, y, x, :] = 0
-
-    def getParticleCounts(self) -> Dict[int, int]:
-        c = np.bincount(self.getParticleLatticeView().reshape(-1), minlength=256)
-        return {int(p): int(c[p]) for p in np.flatnonzero(c) if p != 0}
-
-    def row_major(self) -> np.ndarray:
-        """Particles as row-major (X, Y, Z, P) bytes, the file layout."""
-        v = self.getParticleLatticeView()
-        W, Z, Y, X, _ = v.shape
-        return np.ascontiguousarray(np.transpose(v, (3, 2, 1, 0, 4)).reshape(X, Y, Z, 4 * W))
-
-    def sites_row_major(self) -> np.ndarray:
-        return np.ascontiguousarray(np.transpose(self._sites, (2, 1, 0)))
+        self._solver.setSites(self.sites_row_major())
 
 
 class _Recorder:

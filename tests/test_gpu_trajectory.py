@@ -83,3 +83,40 @@ def test_hook_protocol_matches_oracle(tmp_path):
         assert site_steps == [8, 16]
     finally:
         solver.close()
+
+
+def test_lm_module_runs_a_jlm_style_file(tmp_path):
+    """A file built through the lm compatibility module, run with lm.runSolver
+    and a solver composed the way jLM's makeSolver does it."""
+    from lattice_microbes_amd import compat
+    from test_lm_compat import build_with_lm
+
+    compat.install()
+    import lm
+
+    spec = small_spec()
+    path = str(tmp_path / "jlm.h5")
+    build_with_lm(spec, path)
+    with lm.SimulationFile(path) as f:
+        f.setParameter("seed", str(SEED))
+
+    class Watch:
+        def __init__(self):
+            super().__init__()
+            self.seen = []
+
+        def hookSimulation(self, t, lattice):
+            self.seen.append(round(t / self.tau))
+            return 0
+
+    solver = type("Watch_MpdRdmeSolver", (Watch, lm.MpdRdmeSolver), {})()
+    try:
+        lm.runSolver(path, 2, solver, [0], 0)
+        assert solver.seen == [0]            # hookInterval "" disables all but the initial check
+    finally:
+        solver.close()
+    counts, frames, _ = expected_plain(spec, 2, 20, 5, 10)
+    with SimulationFile(path, writable=False) as f:
+        for k, t in enumerate(sorted(frames)):
+            np.testing.assert_array_equal(f.getLattice(2, k), frames[t])
+        np.testing.assert_array_equal(f.getSpeciesCounts(2), np.array([counts[t] for t in sorted(counts)]))
