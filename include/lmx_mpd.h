@@ -123,11 +123,12 @@ int lmx_set_path(lmx_solver *s, int path);
  *                          length the segment does not divide use 0)
  *   LMX_OPT_FUSED_ZSEG     z planes per block of the fused step; 0 (default)
  *                          picks one block per compute unit
- *   LMX_OPT_FUSE_REACTION  1 = the tiled z sweep tests the reaction of its
- *                          single-particle output sites and queues the
- *                          rest, and a small kernel evaluates the queued
- *                          sites (models without zero-order reactions);
- *                          0 (default) = full reaction kernel */
+ *   LMX_OPT_FUSE_REACTION  1 (default) = the tiled z sweep lists its
+ *                          output sites that hold particles, with their
+ *                          words, and the reaction phase evaluates only the
+ *                          listed sites (models without zero-order
+ *                          reactions; a full list falls back to the full
+ *                          pass); 0 = full reaction kernel */
 enum lmx_option {
     LMX_OPT_MARCH_SEGMENT = 1, LMX_OPT_FORCE_GENERIC = 2, LMX_OPT_YZ_KERNEL = 3, LMX_OPT_FUSED_ZSEG = 4,
     LMX_OPT_FUSE_REACTION = 5

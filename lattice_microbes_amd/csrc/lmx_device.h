@@ -20,7 +20,7 @@ constexpr uint32_t kErrOverflowCap = 3, kErrUnplaceable = 4, kErrOutOfRange = 5,
 
 // Per-solver control block in device memory.  The step counter lives here so
 // that the step sequence does not depend on host-side arguments.
-constexpr int kDeferShards = 64;
+constexpr int kListShards = 1024;
 
 struct Control {
     uint32_t timestep;          // advanced by the resolver at the end of each step
@@ -30,9 +30,23 @@ struct Control {
     unsigned long long ovf_total;
     unsigned long long ovf_steps;
     unsigned long long ovf_keys[kMaxOverflows];
-    // Sites queued for the reaction phase by the fused z sweep, one counter
-    // per list shard (shards spread the atomic reservations).
-    uint32_t defer_count[kDeferShards];
+    // The reaction list a fused z sweep fills: entries per shard (may exceed
+    // the capacity) and whether any shard overflowed.  The resolver clears
+    // both at the end of each step.
+    uint32_t list_overflow;
+    uint32_t list_count[kListShards];
+};
+
+// Sites queued for the reaction phase by a fused z sweep: the site index, its
+// W words after the sweep and its type.  Shard s holds entries
+// [s*cap, (s+1)*cap) of idx and type and of each word plane
+// words[k*kListShards*cap ...]; a wave appends to shard (global wave index) %
+// kListShards.
+struct ReactionList {
+    uint32_t *idx;
+    uint32_t *words;
+    uint8_t *type;
+    uint32_t cap;
 };
 
 // Overflow entry key: canonical order is phase, then site index, then

@@ -20,11 +20,10 @@ bool col_ok(const Params &p, int axis, uint32_t seg);
 hipError_t launch_sweep_col(const Params &p, const uint32_t *src, uint32_t *dst, int axis, uint32_t seg,
                             hipStream_t st);
 hipError_t launch_sweep_tile(const Params &p, const uint32_t *src, uint32_t *dst, int axis, bool react,
-                             uint32_t *deferred, uint32_t defer_cap, hipStream_t st);
+                             const ReactionList &rl, hipStream_t st);
 hipError_t launch_sweep_sparse(const Params &p, const uint32_t *src, uint32_t *dst, int axis, bool react,
-                               uint32_t *deferred, uint32_t defer_cap, hipStream_t st);
-hipError_t launch_reaction_deferred(const Params &p, uint32_t *lat, const uint32_t *deferred, uint32_t defer_cap,
-                                    int num_cu, hipStream_t st);
+                               const ReactionList &rl, hipStream_t st);
+hipError_t launch_reaction_list(const Params &p, uint32_t *lat, const ReactionList &rl, int num_cu, hipStream_t st);
 bool fused_ok(const Params &p);
 uint32_t fused_tile_rows(uint32_t X, uint32_t W);
 hipError_t launch_fused(const Params &p, const uint32_t *src, uint32_t *dst, uint32_t zseg, hipStream_t st);

@@ -110,9 +110,9 @@ struct lmx_solver {
     int yz_kernel = 0;  // 0: LDS tiles, 1: register march, 2: LDS tiles with compacted work lists
     lmx_batch *batch = nullptr;  // ensemble this solver is a member of, if any
     uint32_t fused_zseg = 0;  // 0: automatic
-    bool fuse_reaction = false;  // reaction test inside the tiled z sweep
-    uint32_t *d_deferred = nullptr;  // sites queued for the reaction phase by that sweep
-    uint32_t defer_cap = 0;
+    bool fuse_reaction = true;  // the tiled z sweep lists its occupied output sites for the reaction phase
+    uint32_t *d_rl = nullptr;   // the reaction list: index and word planes, then the type bytes
+    lmx::ReactionList rlist{};
     uint32_t num_cu = 256;
 };
 
@@ -342,11 +342,11 @@ int enqueue_step(lmx_solver *s, const lmx::Params &p, hipEvent_t *ev, int *kinds
         s->timestep++;
         return LMX_OK;
     }
-    // The tiled z sweeps can test the reaction of their output sites and
-    // queue the rest (not with zero-order reactions, which make every site a
-    // candidate).
+    // The tiled z sweeps list their occupied output sites, and the reaction
+    // phase evaluates only those (not with zero-order reactions, which make
+    // every site a candidate).
     const bool z_reacts = fast && (s->yz_kernel == 0 || s->yz_kernel == 2) && s->fuse_reaction && s->R > 0 &&
-                          !s->any_qp0;
+                          !s->any_qp0 && s->rlist.idx;
     for (int axis = 0; axis < 3; axis++) {
         const bool react = axis == 2 && z_reacts;
         if (!fast)
@@ -354,11 +354,10 @@ int enqueue_step(lmx_solver *s, const lmx::Params &p, hipEvent_t *ev, int *kinds
         else if (axis > 0 && s->yz_kernel == 3 && !react && lmx::col_ok(p, axis, s->march_seg))
             e = lmx::launch_sweep_col(p, s->d_lat[s->cur], s->d_lat[1 - s->cur], axis, s->march_seg, s->stream);
         else if (axis > 0 && (s->yz_kernel == 0 || s->yz_kernel == 3))
-            e = lmx::launch_sweep_tile(p, s->d_lat[s->cur], s->d_lat[1 - s->cur], axis, react, s->d_deferred,
-                                       s->defer_cap, s->stream);
+            e = lmx::launch_sweep_tile(p, s->d_lat[s->cur], s->d_lat[1 - s->cur], axis, react, s->rlist, s->stream);
         else if (axis > 0 && s->yz_kernel == 2)
-            e = lmx::launch_sweep_sparse(p, s->d_lat[s->cur], s->d_lat[1 - s->cur], axis, react, s->d_deferred,
-                                         s->defer_cap, s->stream);
+            e = lmx::launch_sweep_sparse(p, s->d_lat[s->cur], s->d_lat[1 - s->cur], axis, react, s->rlist,
+                                         s->stream);
         else
             e = lmx::launch_sweep_fast(p, s->d_lat[s->cur], s->d_lat[1 - s->cur], axis, s->march_seg, s->stream);
         if (e != hipSuccess) return hip_fail(e, "sweep launch");
@@ -366,8 +365,7 @@ int enqueue_step(lmx_solver *s, const lmx::Params &p, hipEvent_t *ev, int *kinds
         if (int rc = mark(LMX_K_SWEEP_X + axis)) return rc;
     }
     if (z_reacts) {
-        e = lmx::launch_reaction_deferred(p, s->d_lat[s->cur], s->d_deferred, s->defer_cap, (int)s->num_cu,
-                                          s->stream);
+        e = lmx::launch_reaction_list(p, s->d_lat[s->cur], s->rlist, (int)s->num_cu, s->stream);
         if (e != hipSuccess) return hip_fail(e, "reaction launch");
         if (int rc = mark(LMX_K_REACTION)) return rc;
     } else if (s->R > 0) {
@@ -389,7 +387,8 @@ void free_device(lmx_solver *s)
     dev_free(s->d_lat[0]);
     dev_free(s->d_lat[1]);
     dev_free(s->d_sites);
-    dev_free(s->d_deferred);
+    dev_free(s->d_rl);
+    s->rlist = lmx::ReactionList{};
     dev_free(s->d_T);
     dev_free(s->d_qp0);
     dev_free(s->d_qp1);
@@ -604,12 +603,19 @@ int lmx_build_diffusion_model(lmx_solver *s, uint32_t nST, const double *DF, con
     hipError_t e = hipMalloc((void **)&s->d_lat[0], lat_bytes);
     if (e == hipSuccess) e = hipMalloc((void **)&s->d_lat[1], lat_bytes);
     if (e == hipSuccess) e = hipMalloc((void **)&s->d_sites, s->N);
-    // Deferred reaction sites, kDeferShards lists of defer_cap entries: any
-    // capacity is correct (an overflowing list falls back to a full reaction
-    // pass); N/16 in total covers typical densities.
-    s->defer_cap = (uint32_t)(std::max<size_t>(std::min<size_t>(s->N, 65536), s->N / 16) / lmx::kDeferShards + 1);
-    if (e == hipSuccess)
-        e = hipMalloc((void **)&s->d_deferred, (size_t)s->defer_cap * lmx::kDeferShards * sizeof(uint32_t));
+    // The reaction list: kListShards shards of cap entries (index, W words,
+    // type byte).  Any capacity is correct (an overflowing shard makes the
+    // reaction phase a full pass); N/8 entries in total hold every occupied
+    // site up to 12.5% occupancy.
+    const size_t rcap = std::max<size_t>(64, (s->N / 8 + lmx::kListShards - 1) / lmx::kListShards);
+    const size_t rtotal = rcap * lmx::kListShards;
+    if (e == hipSuccess) e = hipMalloc((void **)&s->d_rl, rtotal * (1 + s->W) * sizeof(uint32_t) + rtotal);
+    if (e == hipSuccess) {
+        s->rlist.idx = s->d_rl;
+        s->rlist.words = s->d_rl + rtotal;
+        s->rlist.type = reinterpret_cast<uint8_t *>(s->d_rl + rtotal * (1 + s->W));
+        s->rlist.cap = (uint32_t)rcap;
+    }
     if (e != hipSuccess) {
         free_device(s);
         return fail(LMX_ERR_NOMEM, std::string("Failed to allocate the device lattice: ") + hipGetErrorString(e));

@@ -15,6 +15,11 @@
 //  sweep_march_kernel  y or z sweep; each thread walks a segment of rows along
 //                      the axis with rows r-1..r+2 in registers, decides row
 //                      r+2 while writing row r, and carries the decisions
+//  sweep_tile_kernel   y or z sweep over LDS row tiles (the default); the z
+//                      sweep also lists its output sites that hold particles
+//                      (reaction list)
+//  reaction_list_kernel  reaction phase of the listed sites (default without
+//                      zero-order reactions)
 //  reaction4_kernel    reaction phase in place, looping over occupied sites;
 //                      single-particle sites take their probability from the
 //                      device-built table (react_site_table)
@@ -24,6 +29,7 @@
 // 0.5 <= u < qp + 0.5; assembly (performPropagation, :199-270): stayers, then
 // the minus neighbour's plus movers, then the plus neighbour's minus movers,
 // each in slot order, first P kept, the rest recorded as overflow.
+#include <algorithm>
 #include <type_traits>
 
 #include "lmx_batch.h"
@@ -650,55 +656,39 @@ __device__ inline uint32_t wave_reserve(uint32_t *counter, uint32_t c)
     return __shfl(base, 0, 64) + pre;
 }
 
-// The cheap part of the reaction phase for a site with content w after the
-// z sweep: true if the site must be evaluated by reaction_deferred_kernel,
-// i.e. it holds more than one particle, or one particle whose reaction fires
-// (same table, key and comparison as react_site_table).  Without zero-order
-// reactions an empty site never reacts.
-template <int W>
-__device__ inline bool react_test(const Params &p, uint32_t i, uint32_t st, const uint32_t *w,
-                                  unsigned long long h3, const float2 *tab)
+// Reserves c < 8 entries of the reaction list for the calling lane, in the
+// shard of the calling wave; *shard receives the shard.  Every lane of the
+// wave must call it.
+__device__ inline uint32_t list_reserve(const Params &p, uint32_t c, uint32_t *shard)
 {
-    uint32_t n = 0, any = 0;
-#pragma unroll
-    for (int k = 0; k < W; k++) {
-        n += (uint32_t)__builtin_popcount(nonzero_bytes(w[k]));
-        any |= w[k];
-    }
-    if (n != 1) return n > 1;
-    const uint32_t first = (any | (any >> 8) | (any >> 16) | (any >> 24)) & 0xFFu;
-    const float2 ap = tab[st * p.nS + (first - 1)];
-    return ap.x != 0.0f && ap.y > 0.0f && uniform01(i, 1, 0, h3) <= ap.y;
+    *shard = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) % (uint32_t)kListShards;
+    return wave_reserve(&p.ctl->list_count[*shard], c);
 }
 
-// Appends the sites i0 + s (bit s of mask) to this block's shard of the
-// deferred lists (shard b holds entries [b*shard_cap, (b+1)*shard_cap)) with
-// one reservation per wave.  Every lane of the wave must call it.
-__device__ inline void defer_sites(const Params &p, uint32_t *deferred, uint32_t shard_cap, uint32_t i0,
-                                   uint32_t mask)
+// Writes one reaction-list entry; false (nothing written) when the shard is full.
+template <int W>
+__device__ inline bool list_put(const ReactionList &rl, uint32_t shard, uint32_t slot, uint32_t i,
+                                const uint32_t *w, uint32_t st)
 {
-    const uint32_t shard = blockIdx.x % (uint32_t)kDeferShards;
-    uint32_t slot = wave_reserve(&p.ctl->defer_count[shard], (uint32_t)__builtin_popcount(mask));
-    uint32_t *list = deferred + (size_t)shard * shard_cap;
-    while (mask) {
-        const uint32_t s = (uint32_t)__builtin_ctz(mask);
-        mask &= mask - 1;
-        if (slot < shard_cap) list[slot] = i0 + s;
-        slot++;
-    }
+    if (slot >= rl.cap) return false;
+    const size_t e = (size_t)shard * rl.cap + slot, total = (size_t)kListShards * rl.cap;
+    rl.idx[e] = i;
+#pragma unroll
+    for (int k = 0; k < W; k++) rl.words[(size_t)k * total + e] = w[k];
+    rl.type[e] = (uint8_t)st;
+    return true;
 }
 
 template <int W>
 constexpr int tile_threads() { return W == 4 ? 512 : 1024; }
 static_assert(tile_threads<2>() == 1024 && tile_threads<4>() == 512, "tile_threads_for must agree");
 
-// With REACT (z sweep) each output site also gets the cheap part of the
-// reaction phase (react_test); the sites it queues are evaluated by
-// reaction_deferred_kernel after the sweep.
+// With REACT (z sweep) the output sites that hold particles are appended to
+// the reaction list, and reaction_list_kernel evaluates their reactions after
+// the sweep instead of a full reaction pass.
 template <int W, bool REACT>
 __device__ inline void tile_body(const Params &p, const uint32_t *__restrict__ src, uint32_t *__restrict__ dst,
-                                 const TileGeom &geo, uint32_t substep, uint32_t block,
-                                 uint32_t *__restrict__ deferred, uint32_t defer_cap)
+                                 const TileGeom &geo, uint32_t substep, uint32_t block, const ReactionList &rl)
 {
     const uint32_t stride = geo.stride, len = geo.len, ostride = geo.ostride, nouter = geo.nouter, Gx = geo.Gx,
                    Ty = geo.Ty, nxchunk = geo.nxchunk, ntiles = geo.ntiles, tile_first = geo.tile_first;
@@ -796,7 +786,6 @@ __device__ inline void tile_body(const Params &p, const uint32_t *__restrict__ s
     }
     __syncthreads();
 
-    const float2 *tab = reinterpret_cast<const float2 *>(p.prob1);
     uint4 res[W];
 #pragma unroll
     for (int k = 0; k < W; k++) res[k] = make_uint4(0u, 0u, 0u, 0u);
@@ -827,30 +816,41 @@ __device__ inline void tile_body(const Params &p, const uint32_t *__restrict__ s
 #pragma unroll
         for (int k = 0; k < W; k++) reinterpret_cast<uint4 *>(dst + (size_t)k * p.N + i0)[0] = res[k];
     }
-    if (REACT && p.R > 0) {
-        uint32_t dmask = 0;
+    if (REACT) {
+        // The output sites that hold particles go to the reaction list with
+        // their words and types (without zero-order reactions an empty site
+        // cannot react).
+        uint32_t occ_out = 0;
         if (valid) {
-            const unsigned long long h3 = timestep_hash(p.seed32, p.ctl->timestep, 3);
 #pragma unroll
             for (int s = 0; s < 4; s++) {
-                uint32_t w[W];
+                uint32_t a = 0;
 #pragma unroll
-                for (int k = 0; k < W; k++) w[k] = lane_of(res[k], s);
-                dmask |= (react_test<W>(p, i0 + s, (G.t >> (8 * s)) & 0xFFu, w, h3, tab) ? 1u : 0u) << s;
+                for (int k = 0; k < W; k++) a |= lane_of(res[k], s);
+                occ_out |= (a != 0u ? 1u : 0u) << s;
             }
         }
-        defer_sites(p, deferred, defer_cap, i0, dmask);
+        uint32_t shard;
+        uint32_t slot = list_reserve(p, (uint32_t)__builtin_popcount(occ_out), &shard);
+        bool full = false;
+        while (occ_out) {
+            const uint32_t s = (uint32_t)__builtin_ctz(occ_out);
+            occ_out &= occ_out - 1;
+            uint32_t w[W];
+#pragma unroll
+            for (int k = 0; k < W; k++) w[k] = lane_dyn(res[k], s);
+            full |= !list_put<W>(rl, shard, slot++, i0 + s, w, (G.t >> (8 * s)) & 0xFFu);
+        }
+        if (full) p.ctl->list_overflow = 1u;
     }
 }
 
 template <int W, bool REACT>
 __global__ __launch_bounds__(tile_threads<W>()) void sweep_tile_kernel(Params p, const uint32_t *__restrict__ src,
                                                                        uint32_t *__restrict__ dst, TileGeom geo,
-                                                                       uint32_t substep,
-                                                                       uint32_t *__restrict__ deferred,
-                                                                       uint32_t defer_cap)
+                                                                       uint32_t substep, ReactionList rl)
 {
-    tile_body<W, REACT>(p, src, dst, geo, substep, blockIdx.x, deferred, defer_cap);
+    tile_body<W, REACT>(p, src, dst, geo, substep, blockIdx.x, rl);
 }
 
 // ---------------------------------------------------------------------------
@@ -873,16 +873,8 @@ __global__ __launch_bounds__(tile_threads<W>()) void sweep_tile_kernel(Params p,
 // (cache hits), which keeps a block at ~32 KB of LDS so that four blocks,
 // each with two groups per thread in flight, share a compute unit.
 //
-// With REACT (z sweep) a work site also runs the cheap part of the reaction
-// phase on its new content: a site holding one particle looks up its
-// probability and tests it; a site whose reaction fires, or which holds more
-// than one particle, is appended to the deferred list, and
-// reaction_deferred_kernel evaluates those sites after the sweep.  The
-// reaction of a site reads only that site's post-z content and type, and a
-// site whose reaction does not fire is left unchanged by a second
-// evaluation, so the result equals a separate reaction pass.  Sites without
-// particles cannot react unless a zero-order reaction exists; the host does
-// not fuse then.
+// With REACT (z sweep) a work site that holds particles after the sweep is
+// appended to the reaction list (see tile_body).
 // ---------------------------------------------------------------------------
 constexpr int kSparseThreads = 512;
 constexpr int kSparseGpt = 2;  // owned groups per thread
@@ -915,7 +907,7 @@ template <int W, bool REACT>
 __global__ __launch_bounds__(kSparseThreads) void sweep_sparse_kernel(
     Params p, const uint32_t *__restrict__ src, uint32_t *__restrict__ dst, uint32_t stride, uint32_t len,
     uint32_t ostride, uint32_t nouter, uint32_t Gx, uint32_t Ty, uint32_t nxchunk, uint32_t ntiles,
-    uint32_t substep, uint32_t tile_first, uint32_t *__restrict__ deferred, uint32_t defer_cap)
+    uint32_t substep, uint32_t tile_first, ReactionList rl)
 {
     constexpr int NT = kSparseThreads;
     constexpr int GPT = kSparseGpt;
@@ -1004,7 +996,6 @@ __global__ __launch_bounds__(kSparseThreads) void sweep_sparse_kernel(
             s_occ[slot++] = (uint16_t)(e * 4u + sb);
         }
     }
-    const float2 *tab = reinterpret_cast<const float2 *>(p.prob1);
     __syncthreads();
 
     // ---- 2a. decisions of the occupied sites ----------------------------------------
@@ -1073,12 +1064,14 @@ __global__ __launch_bounds__(kSparseThreads) void sweep_sparse_kernel(
 
     // ---- 3. assembly of the work sites (and the reaction test) ---------------------
     const uint32_t nwork = s_cnt[1];
-    const unsigned long long h3 = REACT ? timestep_hash(p.seed32, t, 3) : 0ull;
-    // Uniform trip count, so that every lane reaches the wave-wide deferral.
+    // Uniform trip count, so that every lane reaches the wave-wide list reservation.
     for (uint32_t base0 = 0; base0 < nwork; base0 += NT) {
         const uint32_t idx = base0 + tid;
         if (idx >= nwork) {
-            if (REACT && p.R > 0) defer_sites(p, deferred, defer_cap, 0u, 0u);
+            if (REACT) {
+                uint32_t shard;
+                list_reserve(p, 0u, &shard);
+            }
             continue;
         }
         const uint32_t c = s_work[idx];
@@ -1107,46 +1100,13 @@ __global__ __launch_bounds__(kSparseThreads) void sweep_sparse_kernel(
         assemble<W>(p, i, substep, wo, stay, own_occ, wm, dec_m >> P, wp, dec_p & PM, out);
 #pragma unroll
         for (int q = 0; q < W; q++) dst[(size_t)q * p.N + i] = out[q];
-        if (REACT && p.R > 0)
-            defer_sites(p, deferred, defer_cap, i, react_test<W>(p, i, (s_t[e] >> (8 * sb)) & 0xFFu, out, h3, tab) ? 1u : 0u);
-    }
-}
-
-// Reaction phase for the sites the fused z sweep deferred: one evaluation
-// per listed site, or, when a list shard overflowed, a full pass over every
-// occupied site (sites whose reaction does not fire are left unchanged).
-// Block b serves shard b % kDeferShards; the resolver clears the counters.
-template <int W>
-__global__ __launch_bounds__(256) void reaction_deferred_kernel(Params p, uint32_t *__restrict__ lat,
-                                                                const uint32_t *__restrict__ deferred,
-                                                                uint32_t shard_cap)
-{
-    bool full = false;
-    for (int k = 0; k < kDeferShards; k++) full |= p.ctl->defer_count[k] > shard_cap;
-    const unsigned long long h3 = timestep_hash(p.seed32, p.ctl->timestep, 3);
-    const float2 *tab = reinterpret_cast<const float2 *>(p.prob1);
-    if (full) {
-        for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < p.N; i += gridDim.x * blockDim.x) {
-            uint32_t w[W];
-            global_site_words<W>(p, lat, i, w);
-            if (site_occ<W>(w) && react_site_table<W>(p, i, p.sites[i], w, h3, tab)) {
+        if (REACT) {
+            uint32_t any = 0, shard;
 #pragma unroll
-                for (int q = 0; q < W; q++) lat[(size_t)q * p.N + i] = w[q];
-            }
-        }
-        return;
-    }
-    const uint32_t shard = blockIdx.x % (uint32_t)kDeferShards, part = blockIdx.x / (uint32_t)kDeferShards;
-    const uint32_t nparts = gridDim.x / (uint32_t)kDeferShards;
-    const uint32_t n = p.ctl->defer_count[shard];
-    const uint32_t *list = deferred + (size_t)shard * shard_cap;
-    for (uint32_t k = part * blockDim.x + threadIdx.x; k < n; k += nparts * blockDim.x) {
-        const uint32_t i = list[k];
-        uint32_t w[W];
-        global_site_words<W>(p, lat, i, w);
-        if (react_site_table<W>(p, i, p.sites[i], w, h3, tab)) {
-#pragma unroll
-            for (int q = 0; q < W; q++) lat[(size_t)q * p.N + i] = w[q];
+            for (int q = 0; q < W; q++) any |= out[q];
+            const uint32_t slot = list_reserve(p, any != 0u ? 1u : 0u, &shard);
+            if (any != 0u && !list_put<W>(rl, shard, slot, i, out, (s_t[e] >> (8 * sb)) & 0xFFu))
+                p.ctl->list_overflow = 1u;
         }
     }
 }
@@ -1245,6 +1205,46 @@ template <int W>
 __global__ __launch_bounds__(256) void reaction4_kernel(Params p, uint32_t *__restrict__ lat)
 {
     reaction4_body<W>(p, lat, blockIdx.x);
+}
+
+// Reaction phase of the sites a fused z sweep listed: one evaluation per
+// entry, from the words and type in the list (the same per-site evaluation as
+// reaction4_kernel; a site's reaction reads only that site, and an empty site
+// cannot react without zero-order reactions, so the result equals the full
+// pass).  Block b serves shards b, b + gridDim.x, ...  If any shard
+// overflowed, the kernel runs the full in-place pass instead.
+template <int W>
+__global__ __launch_bounds__(256) void reaction_list_kernel(Params p, uint32_t *__restrict__ lat, ReactionList rl)
+{
+    if (p.ctl->list_overflow) {
+        const uint32_t nblk = ((p.N >> 2) + 256u * kRG - 1u) / (256u * kRG);
+        for (uint32_t blk = blockIdx.x; blk < nblk; blk += gridDim.x) reaction4_body<W>(p, lat, blk);
+        return;
+    }
+    __shared__ float2 s_tab[kTabLds];
+    const uint32_t ntab = p.nST * p.nS;
+    const bool in_lds = ntab <= kTabLds;
+    if (in_lds) {
+        for (uint32_t k = threadIdx.x; k < ntab; k += blockDim.x) s_tab[k] = reinterpret_cast<const float2 *>(p.prob1)[k];
+    }
+    __syncthreads();
+    const float2 *tab = in_lds ? s_tab : reinterpret_cast<const float2 *>(p.prob1);
+    const unsigned long long h3 = timestep_hash(p.seed32, p.ctl->timestep, 3);
+    const size_t total = (size_t)kListShards * rl.cap;
+    for (uint32_t shard = blockIdx.x; shard < (uint32_t)kListShards; shard += gridDim.x) {
+        const uint32_t n = min(p.ctl->list_count[shard], rl.cap);
+        for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) {
+            const size_t e = (size_t)shard * rl.cap + k;
+            const uint32_t i = rl.idx[e];
+            uint32_t w[W];
+#pragma unroll
+            for (int q = 0; q < W; q++) w[q] = rl.words[(size_t)q * total + e];
+            if (react_site_table<W>(p, i, rl.type[e], w, h3, tab)) {
+#pragma unroll
+                for (int q = 0; q < W; q++) lat[(size_t)q * p.N + i] = w[q];
+            }
+        }
+    }
 }
 
 // Probability tables for sites with at most one particle, built with the
@@ -1680,16 +1680,14 @@ hipError_t launch_sweep_fast(const Params &p, const uint32_t *src, uint32_t *dst
 
 template <int W>
 static hipError_t launch_tile(const Params &p, const uint32_t *src, uint32_t *dst, int axis, bool react,
-                              uint32_t *deferred, uint32_t defer_cap, hipStream_t st)
+                              const ReactionList &rl, hipStream_t st)
 {
     const TileGeom geo = tile_geom(p, axis, (uint32_t)tile_threads<W>());
     const dim3 grid(tile_blocks(geo)), block(geo.Gx * geo.Ty);
     if (react)
-        hipLaunchKernelGGL((sweep_tile_kernel<W, true>), grid, block, 0, st, p, src, dst, geo, (uint32_t)axis, deferred,
-                           defer_cap);
+        hipLaunchKernelGGL((sweep_tile_kernel<W, true>), grid, block, 0, st, p, src, dst, geo, (uint32_t)axis, rl);
     else
-        hipLaunchKernelGGL((sweep_tile_kernel<W, false>), grid, block, 0, st, p, src, dst, geo, (uint32_t)axis,
-                           deferred, defer_cap);
+        hipLaunchKernelGGL((sweep_tile_kernel<W, false>), grid, block, 0, st, p, src, dst, geo, (uint32_t)axis, rl);
     return hipGetLastError();
 }
 
@@ -1731,20 +1729,20 @@ hipError_t launch_sweep_col(const Params &p, const uint32_t *src, uint32_t *dst,
 }
 
 hipError_t launch_sweep_tile(const Params &p, const uint32_t *src, uint32_t *dst, int axis, bool react,
-                             uint32_t *deferred, uint32_t defer_cap, hipStream_t st)
+                             const ReactionList &rl, hipStream_t st)
 {
-    if (axis < 1 || axis > 2 || (react && (axis != 2 || p.any_qp0 || !deferred))) return hipErrorInvalidValue;
+    if (axis < 1 || axis > 2 || (react && (axis != 2 || p.any_qp0 || !rl.idx || !rl.cap))) return hipErrorInvalidValue;
     switch (p.W) {
-    case 1: return launch_tile<1>(p, src, dst, axis, react, deferred, defer_cap, st);
-    case 2: return launch_tile<2>(p, src, dst, axis, react, deferred, defer_cap, st);
-    case 4: return launch_tile<4>(p, src, dst, axis, react, deferred, defer_cap, st);
+    case 1: return launch_tile<1>(p, src, dst, axis, react, rl, st);
+    case 2: return launch_tile<2>(p, src, dst, axis, react, rl, st);
+    case 4: return launch_tile<4>(p, src, dst, axis, react, rl, st);
     }
     return hipErrorInvalidValue;
 }
 
 template <int W>
 static hipError_t launch_sparse(const Params &p, const uint32_t *src, uint32_t *dst, int axis, bool react,
-                                uint32_t *deferred, uint32_t defer_cap, hipStream_t st)
+                                const ReactionList &rl, hipStream_t st)
 {
     const uint32_t stride = axis == 1 ? p.X : p.XY;
     const uint32_t len = axis == 1 ? p.Y : p.Z;
@@ -1761,36 +1759,34 @@ static hipError_t launch_sparse(const Params &p, const uint32_t *src, uint32_t *
     if (react)
         hipLaunchKernelGGL((sweep_sparse_kernel<W, true>), dim3(per_xcd * 8), dim3(kSparseThreads), 0, st, p, src,
                            dst, stride, len, ostride, nouter, Gx, Ty, nxchunk, ntiles, (uint32_t)axis, tile_first,
-                           deferred, defer_cap);
+                           rl);
     else
         hipLaunchKernelGGL((sweep_sparse_kernel<W, false>), dim3(per_xcd * 8), dim3(kSparseThreads), 0, st, p, src,
                            dst, stride, len, ostride, nouter, Gx, Ty, nxchunk, ntiles, (uint32_t)axis, tile_first,
-                           deferred, defer_cap);
+                           rl);
     return hipGetLastError();
 }
 
 hipError_t launch_sweep_sparse(const Params &p, const uint32_t *src, uint32_t *dst, int axis, bool react,
-                               uint32_t *deferred, uint32_t defer_cap, hipStream_t st)
+                               const ReactionList &rl, hipStream_t st)
 {
-    if (axis < 1 || axis > 2 || (react && (axis != 2 || p.any_qp0 || !deferred))) return hipErrorInvalidValue;
+    if (axis < 1 || axis > 2 || (react && (axis != 2 || p.any_qp0 || !rl.idx || !rl.cap))) return hipErrorInvalidValue;
     switch (p.W) {
-    case 1: return launch_sparse<1>(p, src, dst, axis, react, deferred, defer_cap, st);
-    case 2: return launch_sparse<2>(p, src, dst, axis, react, deferred, defer_cap, st);
-    case 4: return launch_sparse<4>(p, src, dst, axis, react, deferred, defer_cap, st);
+    case 1: return launch_sparse<1>(p, src, dst, axis, react, rl, st);
+    case 2: return launch_sparse<2>(p, src, dst, axis, react, rl, st);
+    case 4: return launch_sparse<4>(p, src, dst, axis, react, rl, st);
     }
     return hipErrorInvalidValue;
 }
 
-hipError_t launch_reaction_deferred(const Params &p, uint32_t *lat, const uint32_t *deferred, uint32_t defer_cap,
-                                    int num_cu, hipStream_t st)
+hipError_t launch_reaction_list(const Params &p, uint32_t *lat, const ReactionList &rl, int num_cu, hipStream_t st)
 {
-    // A multiple of the shard count: each shard is served by the same number of blocks.
-    const unsigned parts = (unsigned)((num_cu > 0 ? num_cu : 64) * 4 + kDeferShards - 1) / kDeferShards;
-    const dim3 grid(parts * kDeferShards), block(256);
+    const unsigned nb = (unsigned)std::min(kListShards, (num_cu > 0 ? num_cu : 64) * 4);
+    const dim3 grid(nb), block(256);
     switch (p.W) {
-    case 1: hipLaunchKernelGGL(reaction_deferred_kernel<1>, grid, block, 0, st, p, lat, deferred, defer_cap); break;
-    case 2: hipLaunchKernelGGL(reaction_deferred_kernel<2>, grid, block, 0, st, p, lat, deferred, defer_cap); break;
-    case 4: hipLaunchKernelGGL(reaction_deferred_kernel<4>, grid, block, 0, st, p, lat, deferred, defer_cap); break;
+    case 1: hipLaunchKernelGGL(reaction_list_kernel<1>, grid, block, 0, st, p, lat, rl); break;
+    case 2: hipLaunchKernelGGL(reaction_list_kernel<2>, grid, block, 0, st, p, lat, rl); break;
+    case 4: hipLaunchKernelGGL(reaction_list_kernel<4>, grid, block, 0, st, p, lat, rl); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -1842,7 +1838,8 @@ __global__ __launch_bounds__(tile_threads<W>()) void batch_tile_kernel(const Mem
     const uint32_t c = M.cur;
     const uint32_t *src = axis == 1 ? M.lat[c ^ 1u] : M.lat[c];
     uint32_t *dst = axis == 1 ? M.lat[c] : M.lat[c ^ 1u];
-    tile_body<W, false>(M.p, src, dst, M.geo[axis - 1], axis, blockIdx.x - start[r], nullptr, 0u);
+    const ReactionList none{};
+    tile_body<W, false>(M.p, src, dst, M.geo[axis - 1], axis, blockIdx.x - start[r], none);
 }
 
 template <int W>

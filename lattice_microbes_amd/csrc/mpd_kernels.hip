@@ -123,8 +123,9 @@ __device__ void resolve_body(const Params &p, uint32_t *lat)
     if (tid == 0) s_t = ctl->timestep;
     const uint32_t n = ctl->ovf_count;
     __syncthreads();
-    // The step's reaction phase has consumed the deferred-site lists.
-    if (tid < kDeferShards) ctl->defer_count[tid] = 0u;
+    // The step's reaction phase has consumed the reaction list.
+    for (int k = tid; k < kListShards; k += (int)blockDim.x) ctl->list_count[k] = 0u;
+    if (tid == 0) ctl->list_overflow = 0u;
     const uint32_t t = s_t;
     if (n == 0) {
         if (tid == 0) ctl->timestep = t + 1;

@@ -177,21 +177,28 @@ ALIGNED = [
     ((64, 16, 16), 8, "periodic", 0.05),
     ((8, 8, 16), 8, "reflecting", 0.3),
     ((12, 24, 8), 16, "periodic", 0.2),
+    # sparse enough that no wave lists more than a shard holds: the reaction
+    # phase evaluates the listed sites only
+    ((64, 16, 16), 8, "periodic", 0.01),
+    ((32, 24, 20), 16, "reflecting", 0.004),
+    ((256, 20, 6), 8, "periodic", 0.01),
 ]
 
 
 @pytest.mark.parametrize("case", ALIGNED, ids=[f"{c[0]}-P{c[1]}-{c[2]}" for c in ALIGNED])
 def test_vector_kernels_match_generic_and_oracle(gpu, case):
     """X % 4 == 0 selects the 4-sites-per-thread kernels; every y/z kernel
-    (compacted tiles with and without the fused reaction, dense tiles, the
-    march for every segment) must agree with the one-site-per-thread kernels
-    and the oracle."""
+    (dense and compacted tiles, each with the reaction list filled by the z
+    sweep (the default) and with the full reaction pass, register columns,
+    the march for every segment) must agree with the one-site-per-thread
+    kernels and the oracle.  The dense cases overflow the reaction list's
+    shards, so its full-pass fallback runs there."""
     shape, P, boundary, density = case
     spec = models.random_model(shape=shape, P=P, nS=5, nST=3, R=7, density=density, seed=17, boundary=boundary)
     orc = oracle.Oracle(spec, seed_top=5, replicate=2)
     assert orc.run(6) == 0
     ref = orc.native()
-    for opt in ({"generic": 1}, {"sparse": 1}, {"sparse": 1, "fuse": 1}, {"fuse": 1}, {"col": 4}, {"col": 8},
+    for opt in ({}, {"generic": 1}, {"sparse": 1}, {"sparse": 1, "nofuse": 1}, {"nofuse": 1}, {"col": 4}, {"col": 8},
                 {"seg": 1}, {"seg": 3}, {"seg": 7}, {"seg": 32}, {"seg": 100000}):
         s = lmx.MpdRdmeSolver(0)
         spec.load_into(s, replicate=2, seed=5)
@@ -199,8 +206,8 @@ def test_vector_kernels_match_generic_and_oracle(gpu, case):
             s.setLaunchOption(lmx._lib.OPT_FORCE_GENERIC, 1)
         if "sparse" in opt:
             s.setLaunchOption(lmx._lib.OPT_YZ_KERNEL, 2)
-        if "fuse" in opt:
-            s.setLaunchOption(lmx._lib.OPT_FUSE_REACTION, 1)
+        if "nofuse" in opt:
+            s.setLaunchOption(lmx._lib.OPT_FUSE_REACTION, 0)
         elif "col" in opt:
             s.setLaunchOption(lmx._lib.OPT_YZ_KERNEL, 3)
             s.setLaunchOption(lmx._lib.OPT_MARCH_SEGMENT, opt["col"])
