@@ -57,6 +57,20 @@ def reaction_bytes_per_site(P: int) -> int:
     return P + 1
 
 
+def list_entry_bytes(P: int) -> int:
+    # one reaction-list entry: site index, P particle bytes, site-type byte
+    return 4 + P + 1
+
+
+def uses_reaction_list(spec) -> bool:
+    """The default step lists the z sweep's occupied output sites and runs the
+    reaction phase on them only: the 4-sites-per-thread kernels (X % 4 == 0),
+    reactions present, none of them zero-order."""
+    from lattice_microbes_amd.models import ZEROTH
+
+    return spec.shape[0] % 4 == 0 and spec.R > 0 and not bool((spec.types == ZEROTH).any())
+
+
 def cpu_baseline(spec, budget_s: float, seed: int):
     """NSM port on the same model, single-threaded, bounded by wall time."""
     import ctypes
@@ -254,16 +268,22 @@ def main() -> int:
     log(f"timed {args.steps} steps: {total_ms:.2f} ms")
     counts = solver.getSpeciesCounts()
     ovf_entries, ovf_steps = solver.getOverflowStats()
+    native = solver.getLatticeNative()
+    occupied = int(np.count_nonzero(np.bitwise_or.reduce(native, axis=0)))  # sites holding particles
 
     N, P, K = spec.N, spec.P, args.steps
     value = N * K / (total_ms / 1e3)
-    # per-kernel averages and algorithmic bytes per launch
+    listed = args.path == "sweeps" and uses_reaction_list(spec)
+    list_bytes = occupied * list_entry_bytes(P) if listed else 0
+    # per-kernel averages and algorithmic bytes per launch (with the reaction
+    # list, the z sweep also writes one entry per occupied site and the
+    # reaction kernel reads them)
     kernels = {}
     for name, (ms, launches) in per.items():
         if name.startswith("sweep") or name == "fused":
-            b = sweep_bytes_per_site(P) * N
+            b = sweep_bytes_per_site(P) * N + (list_bytes if name == "sweep_z" else 0)
         elif name == "reaction":
-            b = reaction_bytes_per_site(P) * N
+            b = list_bytes if listed else reaction_bytes_per_site(P) * N
         else:
             b = 0
         avg_ms = ms / max(launches, 1)
@@ -281,8 +301,12 @@ def main() -> int:
     roofline = {"bound": "hbm", "kernel": dom, "achieved": dk["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": dk["GBps"] / HBM_PEAK_GBS, "traffic": traffic,
                 "alg_bytes_per_launch": dk["alg_bytes"], "avg_launch_us": dk["avg_us"]}
-    # Unfused-equivalent bytes: three sweeps plus the reaction read.
-    step_alg = (3 * sweep_bytes_per_site(P) + (reaction_bytes_per_site(P) if spec.R else 0)) * N
+    # Bytes of the step as run: three sweeps plus the reaction list (written
+    # and read once), or plus the full reaction read.
+    if listed:
+        step_alg = 3 * sweep_bytes_per_site(P) * N + 2 * list_bytes
+    else:
+        step_alg = (3 * sweep_bytes_per_site(P) + (reaction_bytes_per_site(P) if spec.R else 0)) * N
     step_ms = total_ms / K
     out = {
         "metric": "site-steps/s & % HBM roofline; ensemble sims/hr, p50/p99 latency vs plain",
@@ -300,7 +324,8 @@ def main() -> int:
         "config": {"workload": f"{spec.name} {spec.shape[0]}x{spec.shape[1]}x{spec.shape[2]}",
                    "sites": N, "slots_per_site": P, "species": spec.nS, "reactions": spec.R,
                    "site_types": spec.nST, "particles": int(spec.initial_counts().sum()),
-                   "lambda_m": spec.spacing, "tau_s": spec.tau, "path": args.path},
+                   "lambda_m": spec.spacing, "tau_s": spec.tau, "path": args.path,
+                   "reaction_phase": "list" if listed else "full pass", "occupied_sites_end": occupied},
         "roofline": roofline,
         "step_roofline": {"alg_bytes_per_step": step_alg, "achieved_GBps": step_alg / (step_ms / 1e3) / 1e9,
                           "frac": step_alg / (step_ms / 1e3) / 1e9 / HBM_PEAK_GBS},
