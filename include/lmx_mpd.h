@@ -42,7 +42,8 @@ enum lmx_status {
     LMX_ERR_OVERFLOW_OUT_OF_RANGE = 5,  /* getNearbySites reached outside the lattice (Lattice.cpp:106-128) */
     LMX_ERR_REACTION_UNDERFLOW = 6,     /* reaction applied without its reactants (byte_reaction_dev.cu:226) */
     LMX_ERR_STATE = 7,                  /* call out of protocol order */
-    LMX_ERR_NOMEM = 8                   /* device or host allocation failed */
+    LMX_ERR_NOMEM = 8,                  /* device or host allocation failed */
+    LMX_ERR_INVALID_LATTICE = 9         /* a lattice changed on the device holds an invalid species or site type */
 };
 
 /* Boundary conditions.  Every reference build is periodic
@@ -154,6 +155,16 @@ int lmx_get_sites(lmx_solver *s, uint8_t *rowMajor, size_t size);
 int lmx_set_lattice(lmx_solver *s, const uint8_t *rowMajor, size_t size);
 int lmx_set_lattice_native(lmx_solver *s, const uint32_t *native, size_t words);
 int lmx_set_sites(lmx_solver *s, const uint8_t *rowMajor, size_t size);
+/* Device-side hooks (the host copies of copyFromGPU/copyToGPU around
+ * hookSimulation, MpdRdmeSolver.cu:958-985, without the copies): the current
+ * device particle planes (W planes of N uint32, the native layout) and site
+ * types (N bytes, index x + X*y + X*Y*z), after the solver's stream has
+ * drained.  They stay valid until the next lmx_run_steps or lmx_set_*; the
+ * caller finishes its own device work on them before the next step.
+ * lmx_validate_lattice then checks them on the device (particle types at most
+ * numberSpecies, site types below numberSiteTypes; LMX_ERR_INVALID_LATTICE). */
+int lmx_device_lattice(lmx_solver *s, void **particles, void **sites);
+int lmx_validate_lattice(lmx_solver *s);
 
 /* ByteLattice::getParticleCounts (ByteLattice.cpp:495-520), computed on the
  * device; counts[nS] for species 1..nS. */

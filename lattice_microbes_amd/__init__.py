@@ -7,10 +7,10 @@ the Python side of that boundary: a ctypes binding, a mirror of the reference
 solver class, and generators for the benchmark models.
 """
 from ._lib import (LmxError, InvalidArgException, OverflowException, device_count, library_path,
-                   load as load_library)
+                   load as load_library, torch_interop)
 from .solver import MESolverFactory, MpdRdmeSolver, SOLVER_NAME, timestep_seed
 from .ensemble import Ensemble
 from . import models
 
 __all__ = ["LmxError", "InvalidArgException", "OverflowException", "device_count", "library_path",
-           "load_library", "MESolverFactory", "MpdRdmeSolver", "SOLVER_NAME", "timestep_seed", "Ensemble", "models"]
+           "load_library", "torch_interop", "MESolverFactory", "MpdRdmeSolver", "SOLVER_NAME", "timestep_seed", "Ensemble", "models"]

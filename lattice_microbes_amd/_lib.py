@@ -22,6 +22,7 @@ ERR_OVERFLOW_OUT_OF_RANGE = 5
 ERR_REACTION_UNDERFLOW = 6
 ERR_STATE = 7
 ERR_NOMEM = 8
+ERR_INVALID_LATTICE = 9
 
 BOUNDARY_PERIODIC = 0
 BOUNDARY_REFLECTING = 1
@@ -71,6 +72,8 @@ SIGNATURES = {
     "lmx_set_lattice": (c_int, [c_void_p, _u8p, c_size_t]),
     "lmx_set_lattice_native": (c_int, [c_void_p, _u32p, c_size_t]),
     "lmx_set_sites": (c_int, [c_void_p, _u8p, c_size_t]),
+    "lmx_device_lattice": (c_int, [c_void_p, ctypes.POINTER(c_void_p), ctypes.POINTER(c_void_p)]),
+    "lmx_validate_lattice": (c_int, [c_void_p]),
     "lmx_get_species_counts": (c_int, [c_void_p, _u32p, c_uint32]),
     "lmx_get_overflow_stats": (c_int, [c_void_p, _u64p, _u64p]),
     "lmx_time_steps": (c_int, [c_void_p, c_uint32, _dp, _dp, _u64p, c_uint32]),
@@ -133,12 +136,28 @@ def check(code: int) -> None:
     if code == OK:
         return
     msg = load().lmx_last_error().decode(errors="replace")
-    if code == ERR_ARG:
+    if code in (ERR_ARG, ERR_INVALID_LATTICE):
         raise InvalidArgException(code, msg)
     if code in (ERR_OVERFLOW_CAP, ERR_OVERFLOW_UNPLACEABLE, ERR_OVERFLOW_OUT_OF_RANGE,
                 ERR_REACTION_UNDERFLOW):
         raise OverflowException(code, msg)
     raise LmxError(code, msg)
+
+
+def torch_interop() -> None:
+    """Starts PyTorch's HIP runtime before this library's.  PyTorch ships its
+    own HIP runtime, which starts only if it is initialised first in the
+    process; call this before the first solver or device query when torch
+    tensors are to share the GPU with the solver (device-side hooks)."""
+    if _lib is not None:
+        import torch
+
+        if torch.cuda.is_initialized():
+            return
+        raise LmxError(ERR_STATE, "torch_interop() must be called before the first solver or device query")
+    import torch
+
+    torch.cuda.init()
 
 
 def device_count() -> int:

@@ -16,7 +16,7 @@ constexpr uint32_t kZeroOrder = 1, kFirstOrder = 2, kSecondOrder = 3, kSecondOrd
 
 // Sticky device error codes (mirrored by lmx_status).
 constexpr uint32_t kErrOverflowCap = 3, kErrUnplaceable = 4, kErrOutOfRange = 5,
-                   kErrReactionUnderflow = 6;
+                   kErrReactionUnderflow = 6, kErrInvalidLattice = 9;
 
 // Per-solver control block in device memory.  The step counter lives here so
 // that the step sequence does not depend on host-side arguments.

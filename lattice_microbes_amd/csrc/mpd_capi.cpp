@@ -292,6 +292,8 @@ int check_device_error(lmx_solver *s)
         return fail(LMX_ERR_OVERFLOW_OUT_OF_RANGE, "Overflow search left the lattice at timestep " + std::to_string(hdr[3]));
     case lmx::kErrReactionUnderflow:
         return fail(LMX_ERR_REACTION_UNDERFLOW, "Reaction selected without its reactants at timestep " + std::to_string(hdr[3]));
+    case lmx::kErrInvalidLattice:
+        return fail(LMX_ERR_INVALID_LATTICE, "latticeData: an invalid species or site type was written on the device before timestep " + std::to_string(hdr[3]));
     }
     return fail(LMX_ERR_HIP, "unknown device error " + std::to_string(hdr[2]));
 }
@@ -800,6 +802,29 @@ int lmx_set_lattice_native(lmx_solver *s, const uint32_t *native, size_t words)
     LMX_HIP(hipStreamSynchronize(s->stream));
     LMX_HIP(hipMemcpy(s->d_lat[s->cur], native, words * sizeof(uint32_t), hipMemcpyHostToDevice));
     return LMX_OK;
+}
+
+int lmx_device_lattice(lmx_solver *s, void **particles, void **sites)
+{
+    int rc = require_ready(s);
+    if (rc) return rc;
+    if (!particles || !sites) return fail(LMX_ERR_ARG, "null argument");
+    LMX_HIP(hipStreamSynchronize(s->stream));
+    if ((rc = check_device_error(s))) return rc;
+    *particles = s->d_lat[s->cur];
+    *sites = s->d_sites;
+    return LMX_OK;
+}
+
+int lmx_validate_lattice(lmx_solver *s)
+{
+    int rc = require_ready(s);
+    if (rc) return rc;
+    const lmx::Params p = make_params(s);
+    hipError_t e = lmx::launch_validate(p, s->d_lat[s->cur], s->stream);
+    if (e != hipSuccess) return hip_fail(e, "validate launch");
+    LMX_HIP(hipStreamSynchronize(s->stream));
+    return check_device_error(s);
 }
 
 int lmx_set_lattice(lmx_solver *s, const uint8_t *rm, size_t size)

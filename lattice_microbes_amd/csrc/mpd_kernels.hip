@@ -278,6 +278,28 @@ __global__ __launch_bounds__(256) void count_kernel(const uint32_t *__restrict__
         if (k >= 1 && k <= nS && hist[k]) atomicAdd(&counts[k - 1], hist[k]);
 }
 
+// Range check of a lattice changed on the device by a caller: particle
+// types at most nS, site types below nST (the kernels index their tables with
+// them).  A violation is a sticky error, reported by the next synchronize.
+__global__ __launch_bounds__(256) void validate_kernel(Params p, const uint32_t *__restrict__ lat, size_t nwords)
+{
+    bool bad = false;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t wi = (size_t)blockIdx.x * blockDim.x + threadIdx.x; wi < nwords; wi += stride) {
+        const uint32_t v = lat[wi];
+#pragma unroll
+        for (int b = 0; b < 4; b++) bad |= ((v >> (8 * b)) & 0xFFu) > p.nS;
+    }
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < p.N; i += stride) bad |= p.sites[i] >= p.nST;
+    if (bad) set_error(p.ctl, kErrInvalidLattice, p.ctl->timestep);
+}
+
+hipError_t launch_validate(const Params &p, const uint32_t *lat, hipStream_t st)
+{
+    hipLaunchKernelGGL(validate_kernel, dim3(1024), dim3(256), 0, st, p, lat, (size_t)p.N * p.W);
+    return hipGetLastError();
+}
+
 __global__ void exp_kernel(const double *x, double *y, size_t n)
 {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;

@@ -50,9 +50,11 @@ class Job:
 
 
 def footprint(spec: ModelSpec) -> int:
-    """Device bytes of a job: two lattice buffers, site types, tables."""
+    """Device bytes of a job: two lattice buffers, site types, the reaction
+    list (N/8 entries of index, P particle bytes and type), tables."""
     tables = 4 * (spec.nST * spec.nST * spec.nS + spec.nST * spec.nS * (spec.nS + 3)) + 64 * spec.R
-    return 2 * spec.N * spec.P + spec.N + tables + (1 << 20)
+    rlist = max(64 * 1024, spec.N // 8) * (spec.P + 5)
+    return 2 * spec.N * spec.P + spec.N + rlist + tables + (1 << 20)
 
 
 def lattice_digest(solver: MpdRdmeSolver) -> str:

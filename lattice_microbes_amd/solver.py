@@ -319,15 +319,21 @@ class MpdRdmeSolver:
         hi = int(float(hook_iv)) if hook_enabled else 0
         rec = _Recorder(self, output)
         lattice = HookLattice(self)
+        device = bool(getattr(self, "deviceHooks", False))
+        dlat = DeviceHookLattice(self) if device else None
 
-        lattice.copyFromGPU()
-        self.onBeginTrajectory(lattice)
-        lattice.copyToGPU()
+        if device:
+            self._device_call(self.onBeginTrajectory, None, dlat, validate=True)
+            lattice.copyFromGPU()
+        else:
+            lattice.copyFromGPU()
+            self.onBeginTrajectory(lattice)
+            lattice.copyToGPU()
         time, t = 0.0, 0
         rec.species_counts(time, lattice)
         rec.lattice(time, lattice)
         rec.init_max_counts(lattice)
-        self._hook_check(time, lattice, rec)
+        self._hook_check(time, lattice, dlat, rec)
 
         next_c, next_l, next_h = wi, lwi, hi
         end = _last_step(self.tau, maxTime)
@@ -339,29 +345,59 @@ class MpdRdmeSolver:
             self.runTimesteps(stop - t, synchronize=True)
             t = stop
             time = t * self.tau
-            # The lattice comes to the host for a hook or a lattice frame; counts
-            # alone are taken on the GPU (the same numbers as counting the
-            # host copy, which then equals the device lattice).
-            host = t >= next_l or (hook_enabled and t >= next_h)
-            if host:
+            # The lattice comes to the host for a host hook or a lattice frame;
+            # counts alone are taken on the GPU (the same numbers as counting
+            # the host copy, which then equals the device lattice).
+            hook_due = hook_enabled and t >= next_h
+            host = t >= next_l or (hook_due and not device)
+            if host and not device:
                 lattice.copyFromGPU()
-            if hook_enabled and t >= next_h:
-                self._hook_check(time, lattice, rec)
+            if hook_due:
+                host |= self._hook_check(time, lattice, dlat, rec)
                 next_h += hi
             if t >= next_l:
+                if device:
+                    lattice.copyFromGPU()
                 rec.lattice(time, lattice)
                 next_l += lwi
             if t >= next_c:
                 rec.species_counts(time, lattice if host else None)
                 next_c += wi
+        if device:
+            self._device_call(self.onEndTrajectory, None, dlat, validate=False)
         lattice.copyFromGPU()
-        self.onEndTrajectory(lattice)
+        if not device:
+            self.onEndTrajectory(lattice)
         return rec.finish()
 
-    def _hook_check(self, time: float, lattice: "HookLattice", rec: "_Recorder") -> None:
+    def _device_call(self, fn, time, dlat: "DeviceHookLattice", validate: bool) -> int:
+        """Calls a device-side hook with fresh device views, waits for the
+        device work it queued, and checks the lattice it may have changed."""
+        dlat.refresh()
+        code = fn(dlat) if time is None else fn(time, dlat)
+        dlat.synchronize()
+        if validate or code in (1, 2):
+            _lib.check(self._lib.lmx_validate_lattice(self._h))
+        return code
+
+    def _hook_check(self, time: float, lattice: "HookLattice", dlat: Optional["DeviceHookLattice"],
+                    rec: "_Recorder") -> bool:
+        """hookCheckSimulation (MpdRdmeSolver.cu:958-985).  With device hooks
+        the changes are already on the device (codes 1 and 2 need no copy) and
+        the sites of codes 2 and 3 come to the host to be recorded.  Returns
+        whether the host copy was refreshed."""
+        if dlat is not None:
+            code = self._device_call(self.hookSimulation, time, dlat, validate=False)
+            if code not in (0, 1, 2, 3):
+                raise _lib.LmxError(_lib.ERR_ARG, f"Unknown hook return value {code}")
+            if code in (2, 3):
+                lattice.copyFromGPU()
+                rec.sites(time, lattice)
+                return True
+            return False
         code = self.hookSimulation(time, lattice)
         if code == 0:
-            return
+            return False
         if code == 1:
             lattice.copyToGPU()
         elif code == 2:
@@ -371,6 +407,7 @@ class MpdRdmeSolver:
             rec.sites(time, lattice)
         else:
             raise _lib.LmxError(_lib.ERR_ARG, f"Unknown hook return value {code}")
+        return False
 
 
 def _last_step(tau: float, maxTime: float) -> int:
@@ -406,6 +443,79 @@ class HookLattice(ByteLattice):
     def copyToGPU(self) -> None:
         self._solver.setLatticeNative(self._words)
         self._solver.setSites(self.sites_row_major())
+
+
+class _DeviceArray:
+    """__cuda_array_interface__ (version 3) of device memory the solver owns."""
+
+    def __init__(self, ptr: int, shape, typestr: str):
+        self.__cuda_array_interface__ = {"shape": tuple(int(v) for v in shape), "typestr": typestr,
+                                         "data": (int(ptr), False), "version": 3, "strides": None}
+
+
+class DeviceHookLattice:
+    """What a device-side hook sees (a solver class with ``deviceHooks =
+    True``): the solver's current device buffers as torch tensors, with no
+    copies.  ``getParticleLatticeView()`` is the (W, Z, Y, X, 4) uint8 view of
+    the particle planes, ``getSiteLatticeView()`` the (Z, Y, X) site types;
+    both are valid during the hook call only (the solver alternates between
+    two particle buffers).  Changes made with torch operations, on any stream,
+    are complete before the next step (the solver waits for the device), and
+    are checked on the device (particle types at most numberSpecies, site
+    types below numberSiteTypes)."""
+
+    def __init__(self, solver: MpdRdmeSolver):
+        import torch
+
+        if not torch.cuda.is_initialized():
+            raise _lib.LmxError(_lib.ERR_STATE, "device hooks need lattice_microbes_amd.torch_interop() before the "
+                                                "first solver of the process")
+        self._torch = torch
+        self._solver = solver
+        self._dev = torch.device("cuda", solver.device)
+        self._particles = self._sites = None
+
+    def refresh(self) -> None:
+        s = self._solver
+        pp, sp = c_void_p(), c_void_p()
+        _lib.check(s._lib.lmx_device_lattice(s._h, byref(pp), byref(sp)))
+        X, Y, Z = s.shape
+        W = s.particlesPerSite // 4
+        torch = self._torch
+        self._particles = torch.as_tensor(_DeviceArray(pp.value, (W, Z, Y, X, 4), "|u1"), device=self._dev)
+        self._sites = torch.as_tensor(_DeviceArray(sp.value, (Z, Y, X), "|u1"), device=self._dev)
+
+    def synchronize(self) -> None:
+        self._torch.cuda.synchronize(self._dev)
+        self._particles = self._sites = None
+
+    def getParticleLatticeView(self):
+        return self._particles
+
+    def getSiteLatticeView(self):
+        return self._sites
+
+    def getXSize(self) -> int:
+        return self._solver.shape[0]
+
+    def getYSize(self) -> int:
+        return self._solver.shape[1]
+
+    def getZSize(self) -> int:
+        return self._solver.shape[2]
+
+    def getSize(self):
+        return tuple(self._solver.shape)
+
+    def getMaxOccupancy(self) -> int:
+        return self._solver.particlesPerSite
+
+    def getSpacing(self) -> float:
+        return self._solver.latticeSpacing
+
+    def getParticleCounts(self) -> Dict[int, int]:
+        c = self._torch.bincount(self._particles.reshape(-1).to(self._torch.int64), minlength=256).cpu().numpy()
+        return {int(p): int(c[p]) for p in np.flatnonzero(c) if p != 0}
 
 
 class _Recorder:

@@ -11,6 +11,16 @@ if ROOT not in sys.path:
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs a HIP device (MI355X)")
     config.addinivalue_line("markers", "slow: long-running statistical test")
+    # GPU runs: start PyTorch's HIP runtime before the library's, so that the
+    # device-hook tests can use torch tensors on the solver's buffers.
+    expr = config.getoption("markexpr") or ""
+    if "gpu" in expr and "not gpu" not in expr:
+        try:
+            import lattice_microbes_amd
+
+            lattice_microbes_amd.torch_interop()
+        except Exception:
+            pass
 
 
 @pytest.fixture(scope="session")

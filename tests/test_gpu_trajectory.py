@@ -15,7 +15,7 @@ import lattice_microbes_amd as lmx
 from lattice_microbes_amd.runner import runSimulation, runSolver
 from lattice_microbes_amd.simfile import SimulationFile
 import oracle
-from trajectory_cases import (HOOK_HI, HOOK_LWI, HOOK_STEPS, HOOK_WI, SEED, EditingHooks, expected_hooked,
+from trajectory_cases import (HOOK_HI, HOOK_LWI, HOOK_STEPS, HOOK_WI, PLAN, SEED, EditingHooks, expected_hooked,
                               expected_plain, small_spec, type_counts)
 
 pytestmark = pytest.mark.gpu
@@ -120,3 +120,75 @@ def test_lm_module_runs_a_jlm_style_file(tmp_path):
         for k, t in enumerate(sorted(frames)):
             np.testing.assert_array_equal(f.getLattice(2, k), frames[t])
         np.testing.assert_array_equal(f.getSpeciesCounts(2), np.array([counts[t] for t in sorted(counts)]))
+
+
+class DeviceEditingSolver(EditingHooks, lmx.MpdRdmeSolver):
+    """The same plan as a device-side hook: the edits are torch operations on
+    the solver's device buffers (no host copies)."""
+    deviceHooks = True
+
+    def hookSimulation(self, time, lattice):
+        step = int(round(time / self.tau))
+        self.calls.append(step)
+        edit, code, rate = PLAN[step]
+        if edit is not None:
+            edit.apply(lattice.getParticleLatticeView(), lattice.getSiteLatticeView())
+        if rate is not None:
+            self.setReactionRate(0, rate)
+        assert lattice.getParticleLatticeView().is_cuda and lattice.getXSize() == 16
+        return code
+
+    def onEndTrajectory(self, lattice):
+        self.ended = lattice.getParticleLatticeView().cpu().numpy()
+        return 0
+
+
+def need_torch_first():
+    import torch
+
+    if not torch.cuda.is_initialized():
+        pytest.skip("torch must start before the library in this process (run with -m gpu)")
+
+
+def test_device_hooks_match_oracle(tmp_path):
+    need_torch_first()
+    spec = small_spec()
+    path = str(tmp_path / "dhook.lm")
+    spec.to_file(path, {"seed": SEED, "maxTime": repr(HOOK_STEPS * spec.tau), "writeInterval": HOOK_WI,
+                        "latticeWriteInterval": HOOK_LWI, "hookInterval": HOOK_HI})
+    solver = DeviceEditingSolver(0)
+    try:
+        assert runSolver(path, 5, solver)
+        assert solver.calls == [0, 4, 8, 12, 16]
+        frames, counts, site_steps, final = expected_hooked(spec, 5)
+        with SimulationFile(path, writable=False) as f:
+            assert len(f.getLatticeTimes(5)) == len(frames)
+            for k, fr in enumerate(frames):
+                np.testing.assert_array_equal(f.getLattice(5, k), fr, err_msg=f"frame {k}")
+            np.testing.assert_array_equal(f.getSpeciesCounts(5), counts)
+        np.testing.assert_array_equal(solver.ended.reshape(-1), final)
+    finally:
+        solver.close()
+
+
+def test_device_hook_invalid_species_is_reported(tmp_path, capsys):
+    class Bad(lmx.MpdRdmeSolver):
+        deviceHooks = True
+
+        def hookSimulation(self, time, lattice):
+            if time > 0:
+                lattice.getParticleLatticeView()[0, 1, 1, 1, 0] = 200   # nS is 4
+                return 1
+            return 0
+
+    need_torch_first()
+    spec = small_spec()
+    path = str(tmp_path / "bad.lm")
+    spec.to_file(path, {"seed": SEED, "maxTime": repr(8 * spec.tau), "writeInterval": 4,
+                        "latticeWriteInterval": 8, "hookInterval": 4})
+    solver = Bad(0)
+    try:
+        assert not runSolver(path, 0, solver)
+    finally:
+        solver.close()
+    assert "invalid species or site type" in capsys.readouterr().err
