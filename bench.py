@@ -16,6 +16,7 @@ on a bounded sample of the same model on this host.
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -298,15 +299,17 @@ def main() -> int:
             traffic = json.load(open(tfile)).get(f"{spec.name}_{spec.shape[0]}x{spec.shape[1]}x{spec.shape[2]}:{dom}")
         except Exception:
             traffic = None
+    # Measured copy peak (16-byte grid-stride copy of 1 GiB, read + write bytes).
+    copy_gbps = ctypes.c_double(0.0)
+    lmx._lib.check(lmx.load_library().lmx_copy_bandwidth(0, 1 << 30, 20, ctypes.byref(copy_gbps)))
     roofline = {"bound": "hbm", "kernel": dom, "achieved": dk["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": dk["GBps"] / HBM_PEAK_GBS, "traffic": traffic,
-                "alg_bytes_per_launch": dk["alg_bytes"], "avg_launch_us": dk["avg_us"]}
+                "alg_bytes_per_launch": dk["alg_bytes"], "avg_launch_us": dk["avg_us"],
+                "copy_peak_measured": copy_gbps.value, "frac_of_copy_peak": dk["GBps"] / copy_gbps.value}
     # Bytes of the step as run: three sweeps plus the reaction list (written
     # and read once), or plus the full reaction read.
-    if listed:
-        step_alg = 3 * sweep_bytes_per_site(P) * N + 2 * list_bytes
-    else:
-        step_alg = (3 * sweep_bytes_per_site(P) + (reaction_bytes_per_site(P) if spec.R else 0)) * N
+    unfused_alg = (3 * sweep_bytes_per_site(P) + (reaction_bytes_per_site(P) if spec.R else 0)) * N
+    step_alg = 3 * sweep_bytes_per_site(P) * N + 2 * list_bytes if listed else unfused_alg
     step_ms = total_ms / K
     out = {
         "metric": "site-steps/s & % HBM roofline; ensemble sims/hr, p50/p99 latency vs plain",
@@ -328,7 +331,9 @@ def main() -> int:
                    "reaction_phase": "list" if listed else "full pass", "occupied_sites_end": occupied},
         "roofline": roofline,
         "step_roofline": {"alg_bytes_per_step": step_alg, "achieved_GBps": step_alg / (step_ms / 1e3) / 1e9,
-                          "frac": step_alg / (step_ms / 1e3) / 1e9 / HBM_PEAK_GBS},
+                          "frac": step_alg / (step_ms / 1e3) / 1e9 / HBM_PEAK_GBS,
+                          "unfused_equivalent_bytes_per_step": unfused_alg,
+                          "unfused_equivalent_frac": unfused_alg / (step_ms / 1e3) / 1e9 / HBM_PEAK_GBS},
         "kernels": kernels,
         "overflow": {"entries": ovf_entries, "steps_with_overflow": ovf_steps},
         "final_particles": int(counts.sum()),

@@ -192,6 +192,10 @@ int lmx_get_table_f32(lmx_solver *s, int which, float *out, size_t n, size_t *wr
 
 /* Evaluate the device's reaction-probability function (float)(1-exp(-(double)a))
  * and the device exp itself on n inputs (numerics tests). */
+/* Measured copy peak: reps copies of a bytes-long device buffer by a 16-byte
+ * grid-stride copy kernel, as (read + write) GB/s (the STREAM-copy figure the
+ * HBM roofline is also quoted against).  Allocates its own buffers. */
+int lmx_copy_bandwidth(int device, size_t bytes, int reps, double *gbps);
 int lmx_device_exp(int device, const double *x, double *y, size_t n);
 int lmx_device_reaction_probability(int device, const float *a, float *prob, size_t n);
 

@@ -74,6 +74,7 @@ SIGNATURES = {
     "lmx_set_sites": (c_int, [c_void_p, _u8p, c_size_t]),
     "lmx_device_lattice": (c_int, [c_void_p, ctypes.POINTER(c_void_p), ctypes.POINTER(c_void_p)]),
     "lmx_validate_lattice": (c_int, [c_void_p]),
+    "lmx_copy_bandwidth": (c_int, [c_int, c_size_t, c_int, ctypes.POINTER(c_double)]),
     "lmx_get_species_counts": (c_int, [c_void_p, _u32p, c_uint32]),
     "lmx_get_overflow_stats": (c_int, [c_void_p, _u64p, _u64p]),
     "lmx_time_steps": (c_int, [c_void_p, c_uint32, _dp, _dp, _u64p, c_uint32]),

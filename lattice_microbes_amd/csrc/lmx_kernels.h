@@ -38,6 +38,7 @@ hipError_t launch_batch_sweeps(const Member *m, const uint32_t *start_x, const u
                                const uint32_t *start_z, const uint32_t *start_r, uint32_t n, const uint32_t *total,
                                uint32_t W, hipStream_t st);
 hipError_t launch_batch_resolve(Member *m, uint32_t n, hipStream_t st);
+hipError_t launch_copy(const void *src, void *dst, size_t bytes, int num_cu, hipStream_t st);
 hipError_t launch_validate(const Params &p, const uint32_t *lat, hipStream_t st);
 hipError_t launch_counts(const uint32_t *lat, size_t nwords, uint32_t nS, uint32_t *counts, hipStream_t st);
 hipError_t launch_exp(const double *x, double *y, size_t n, hipStream_t st);

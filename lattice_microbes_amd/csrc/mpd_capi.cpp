@@ -964,6 +964,39 @@ int lmx_get_table_f32(lmx_solver *s, int which, float *out, size_t n, size_t *wr
     return LMX_OK;
 }
 
+int lmx_copy_bandwidth(int device, size_t bytes, int reps, double *gbps)
+{
+    if (!gbps || bytes < (1u << 20) || reps < 1) return fail(LMX_ERR_ARG, "bytes must be at least 1 MiB and reps positive");
+    LMX_HIP(hipSetDevice(device));
+    int num_cu = 256;
+    hipDeviceGetAttribute(&num_cu, hipDeviceAttributeMultiprocessorCount, device);
+    bytes &= ~(size_t)15;
+    void *a = nullptr, *b = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipMalloc(&a, bytes);
+    if (e == hipSuccess) e = hipMalloc(&b, bytes);
+    if (e == hipSuccess) e = hipMemset(a, 1, bytes);
+    if (e == hipSuccess) e = hipStreamCreate(&st);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = lmx::launch_copy(a, b, bytes, num_cu, st);  // warm-up
+    if (e == hipSuccess) e = hipEventRecord(e0, st);
+    for (int r = 0; r < reps && e == hipSuccess; r++) e = lmx::launch_copy(r & 1 ? b : a, r & 1 ? a : b, bytes, num_cu, st);
+    if (e == hipSuccess) e = hipEventRecord(e1, st);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    float ms = 0.0f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    if (st) hipStreamDestroy(st);
+    if (a) hipFree(a);
+    if (b) hipFree(b);
+    if (e != hipSuccess) return hip_fail(e, "copy bandwidth");
+    *gbps = 2.0 * (double)bytes * reps / (ms * 1e-3) / 1e9;
+    return LMX_OK;
+}
+
 int lmx_device_exp(int device, const double *x, double *y, size_t n)
 {
     if (!x || !y) return fail(LMX_ERR_ARG, "null argument");

@@ -11,6 +11,8 @@
 // Every per-particle decision depends only on (site index, slot, timestep,
 // substep) through the counter-based hash, so the results do not depend on
 // how sites are distributed over threads.
+#include <algorithm>
+
 #include "lmx_batch.h"
 #include "lmx_device.h"
 #include "lmx_kernels.h"
@@ -297,6 +299,23 @@ __global__ __launch_bounds__(256) void validate_kernel(Params p, const uint32_t 
 hipError_t launch_validate(const Params &p, const uint32_t *lat, hipStream_t st)
 {
     hipLaunchKernelGGL(validate_kernel, dim3(1024), dim3(256), 0, st, p, lat, (size_t)p.N * p.W);
+    return hipGetLastError();
+}
+
+// STREAM copy: 16-byte loads and stores, grid-stride; the measured copy peak
+// that the sweeps' HBM rates are quoted against beside the 8 TB/s figure.
+__global__ __launch_bounds__(256) void copy_kernel(const uint4 *__restrict__ src, uint4 *__restrict__ dst, size_t n)
+{
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
+hipError_t launch_copy(const void *src, void *dst, size_t bytes, int num_cu, hipStream_t st)
+{
+    const size_t n = bytes / sizeof(uint4);
+    const unsigned nb = (unsigned)std::min<size_t>((n + 255) / 256, (size_t)(num_cu > 0 ? num_cu : 256) * 32);
+    hipLaunchKernelGGL(copy_kernel, dim3(nb), dim3(256), 0, st, reinterpret_cast<const uint4 *>(src),
+                       reinterpret_cast<uint4 *>(dst), n);
     return hipGetLastError();
 }
 
