@@ -95,7 +95,14 @@ int lmx_build_model(lmx_solver *s, uint32_t numberSpecies, uint32_t numberReacti
  * (m^2/s), RL [R][nST], lattice and sites in row-major layout (rowMajor must
  * be 1, as the reference requires).  Any X, Y, Z >= 1 is accepted (the
  * reference's X%32, Y%8, Z%8 limits of :201-217 are launch-shape limits, not
- * semantics); particlesPerSite in {4, 8, 16}; bytesPerParticle == 1. */
+ * semantics); particlesPerSite in {4, 8, 16}.
+ * bytesPerParticle 1: the byte lattice (MpdRdmeSolver), at most 255 species;
+ * latticeData holds X*Y*Z*P bytes.  bytesPerParticle 4: the uint32 lattice
+ * of IntMpdRdmeSolver (IntMpdRdmeSolver.cu:155-170, word_diffusion_1d_dev.cu,
+ * word_reaction_dev.cu), one particle per 32-bit word, the same update rules
+ * and RNG keys, species beyond 255; latticeData holds X*Y*Z*P uint32 values
+ * and the native layout is P planes of N words.  numberSiteTypes *
+ * numberSpecies^2 must stay below 2^32. */
 int lmx_build_diffusion_model(lmx_solver *s, uint32_t numberSiteTypes, const double *DF,
                               const uint32_t *RL, uint32_t X, uint32_t Y, uint32_t Z,
                               uint32_t particlesPerSite, uint32_t bytesPerParticle,
@@ -147,7 +154,10 @@ int lmx_get_timestep(lmx_solver *s, uint32_t *timestep);
 int lmx_set_timestep(lmx_solver *s, uint32_t timestep);
 
 /* CudaByteLattice::copyFromGPU + ByteLattice::copyNativeToRowMajorByte
- * (CudaByteLattice.cu:128-141, ByteLattice.cpp:371-403). */
+ * (CudaByteLattice.cu:128-141, ByteLattice.cpp:371-403; CudaIntLattice and
+ * IntLattice for bytesPerParticle 4).  size is the buffer's size in bytes,
+ * X*Y*Z*P*bytesPerParticle; native buffers hold N*W words (W = P/4, or P for
+ * the uint32 lattice). */
 int lmx_get_lattice(lmx_solver *s, uint8_t *rowMajor, size_t size);
 int lmx_get_lattice_native(lmx_solver *s, uint32_t *native, size_t words);
 int lmx_get_sites(lmx_solver *s, uint8_t *rowMajor, size_t size);
@@ -166,7 +176,8 @@ int lmx_set_sites(lmx_solver *s, const uint8_t *rowMajor, size_t size);
 int lmx_device_lattice(lmx_solver *s, void **particles, void **sites);
 int lmx_validate_lattice(lmx_solver *s);
 
-/* ByteLattice::getParticleCounts (ByteLattice.cpp:495-520), computed on the
+/* ByteLattice::getParticleCounts (ByteLattice.cpp:495-520;
+ * IntLattice::getParticleCounts, IntLattice.cpp:419-435), computed on the
  * device; counts[nS] for species 1..nS. */
 int lmx_get_species_counts(lmx_solver *s, uint32_t *counts, uint32_t n);
 

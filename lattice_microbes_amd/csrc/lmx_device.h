@@ -30,6 +30,7 @@ struct Control {
     unsigned long long ovf_total;
     unsigned long long ovf_steps;
     unsigned long long ovf_keys[kMaxOverflows];
+    uint32_t ovf_part[kMaxOverflows];   // the particle of list entry k
     // The reaction list a fused z sweep fills: entries per shard (may exceed
     // the capacity) and whether any shard overflowed.  The resolver clears
     // both at the end of each step.
@@ -50,18 +51,21 @@ struct ReactionList {
 };
 
 // Overflow entry key: canonical order is phase, then site index, then
-// emission order within the site; the particle rides in the low byte.
-__host__ __device__ inline unsigned long long overflow_key(uint32_t phase, uint32_t site,
-                                                           uint32_t emission, uint32_t particle)
+// emission order within the site; the low 11 bits hold the entry's position
+// in the step's list, where Control::ovf_part keeps its particle (a byte, or
+// a 32-bit word of the uint32 lattice).
+__host__ __device__ inline unsigned long long overflow_key(uint32_t phase, uint32_t site, uint32_t emission,
+                                                           uint32_t entry)
 {
     return ((unsigned long long)phase << 62) | ((unsigned long long)site << 24) |
-           ((unsigned long long)(emission & 0xFFFFu) << 8) | (unsigned long long)(particle & 0xFFu);
+           ((unsigned long long)(emission & 0x1FFFu) << 11) | (unsigned long long)(entry & 0x7FFu);
 }
 
 // Everything a kernel needs about one replicate.
 struct Params {
     uint32_t X, Y, Z, XY, N;
     uint32_t P, W, pcb;         // slots, 32-bit words per site, RNG key shift
+    uint32_t word_particles;    // 1: one particle per 32-bit word (W == P), else 4 per word
     uint32_t nS, nST, R;
     uint32_t boundary;          // 0 periodic, 1 reflecting
     uint32_t seed32;
@@ -213,7 +217,53 @@ __device__ inline void record_overflow(Control *ctl, uint32_t phase, uint32_t si
                                        uint32_t particle)
 {
     const uint32_t k = atomicAdd(&ctl->ovf_count, 1u);
-    if (k < kMaxOverflows) ctl->ovf_keys[k] = overflow_key(phase, site, emission, particle);
+    if (k < kMaxOverflows) {
+        ctl->ovf_part[k] = particle;
+        ctl->ovf_keys[k] = overflow_key(phase, site, emission, k);
+    }
 }
+
+// One particle per 32-bit word (the uint32 lattice of IntMpdRdmeSolver):
+// WordBuf appends particles in order and keeps the first P.
+template <int P>
+struct WordBuf {
+    uint32_t w[P];
+    uint32_t n;
+    __device__ inline void clear()
+    {
+#pragma unroll
+        for (int k = 0; k < P; k++) w[k] = 0;
+        n = 0;
+    }
+    __device__ inline bool push(uint32_t particle)
+    {
+        const uint32_t k = n;
+        n = k + 1;
+        if (k >= (uint32_t)P) return false;
+#pragma unroll
+        for (int q = 0; q < P; q++)
+            if (k == (uint32_t)q) w[q] = particle;
+        return true;
+    }
+};
+
+// Slot layouts of a site's W words.  ByteSites<W>: slot j is byte j % 4 of
+// word j / 4 (P = 4W, byte_diffusion_1d_dev.cu).  WordSites<P>: slot j is
+// word j (W = P, word_diffusion_1d_dev.cu).
+template <int W_>
+struct ByteSites {
+    static constexpr int W = W_, P = 4 * W_, kPerWord = 4;
+    using Buf = SlotBuf<W_>;
+    __device__ static inline uint32_t part(uint32_t word, int q) { return (word >> (8 * q)) & 0xFFu; }
+    __device__ static inline uint32_t slot(const uint32_t *w, int j) { return slot_of(w, j); }
+};
+
+template <int P_>
+struct WordSites {
+    static constexpr int W = P_, P = P_, kPerWord = 1;
+    using Buf = WordBuf<P_>;
+    __device__ static inline uint32_t part(uint32_t word, int) { return word; }
+    __device__ static inline uint32_t slot(const uint32_t *w, int j) { return w[j]; }
+};
 
 }  // namespace lmx

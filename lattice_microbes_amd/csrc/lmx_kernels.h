@@ -40,7 +40,8 @@ hipError_t launch_batch_sweeps(const Member *m, const uint32_t *start_x, const u
 hipError_t launch_batch_resolve(Member *m, uint32_t n, hipStream_t st);
 hipError_t launch_copy(const void *src, void *dst, size_t bytes, int num_cu, hipStream_t st);
 hipError_t launch_validate(const Params &p, const uint32_t *lat, hipStream_t st);
-hipError_t launch_counts(const uint32_t *lat, size_t nwords, uint32_t nS, uint32_t *counts, hipStream_t st);
+hipError_t launch_counts(const uint32_t *lat, size_t nwords, uint32_t nS, bool words, uint32_t *counts,
+                         hipStream_t st);
 hipError_t launch_exp(const double *x, double *y, size_t n, hipStream_t st);
 hipError_t launch_prob(const float *a, float *y, size_t n, hipStream_t st);
 

@@ -16,27 +16,30 @@ namespace lmx {
 // the two exclusive, so a site only evaluates the decisions that can move a
 // particle into it.  wo/wm/wp are the W particle words of the site and its
 // neighbours; has_m/has_p are false at a reflecting edge.  Slots beyond P are
-// recorded as overflow entries of this site and phase.
+// recorded as overflow entries of this site and phase.  L is the slot layout
+// (ByteSites<W> or WordSites<P>); the RNG key of slot j is (site << pcb) | j
+// in both, as word_diffusion_1d_dev.cu:106 keys word w of the uint32 lattice.
 // ---------------------------------------------------------------------------
-template <int W>
-__device__ inline void pull_site(const Params &p, uint32_t i, uint32_t im, uint32_t ip, const uint32_t *wo,
-                                 const uint32_t *wm, const uint32_t *wp, uint32_t st, uint32_t stm, uint32_t stp,
-                                 bool has_m, bool has_p, unsigned long long h, uint32_t phase, uint32_t *out)
+template <class L>
+__device__ inline void pull_site_l(const Params &p, uint32_t i, uint32_t im, uint32_t ip, const uint32_t *wo,
+                                   const uint32_t *wm, const uint32_t *wp, uint32_t st, uint32_t stm, uint32_t stp,
+                                   bool has_m, bool has_p, unsigned long long h, uint32_t phase, uint32_t *out)
 {
-    SlotBuf<W> b;
+    constexpr int W = L::W, Q = L::kPerWord;
+    typename L::Buf b;
     b.clear();
 #pragma unroll
     for (int w = 0; w < W; w++) {
         if (wo[w] == 0) continue;
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
-            const uint32_t part = (wo[w] >> (8 * q)) & 0xFFu;
+        for (int q = 0; q < Q; q++) {
+            const uint32_t part = L::part(wo[w], q);
             if (part == 0) continue;
-            const float u = uniform01(i, p.pcb, 4 * w + q, h);
+            const float u = uniform01(i, p.pcb, Q * w + q, h);
             const float qm = has_m ? lookup_T(p, part, st, stm) : 0.0f;
             const float qp = has_p ? lookup_T(p, part, st, stp) : 0.0f;
             const bool leaves = (u < qm) || (u >= 0.5f && u < (qp + 0.5f));
-            if (!leaves && !b.push(part)) record_overflow(p.ctl, phase, i, b.n - 1 - 4 * W, part);
+            if (!leaves && !b.push(part)) record_overflow(p.ctl, phase, i, b.n - 1 - L::P, part);
         }
     }
     if (has_m) {
@@ -44,13 +47,13 @@ __device__ inline void pull_site(const Params &p, uint32_t i, uint32_t im, uint3
         for (int w = 0; w < W; w++) {
             if (wm[w] == 0) continue;
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const uint32_t part = (wm[w] >> (8 * q)) & 0xFFu;
+            for (int q = 0; q < Q; q++) {
+                const uint32_t part = L::part(wm[w], q);
                 if (part == 0) continue;
-                const float u = uniform01(im, p.pcb, 4 * w + q, h);
+                const float u = uniform01(im, p.pcb, Q * w + q, h);
                 const float qp = lookup_T(p, part, stm, st);
                 if (u >= 0.5f && u < (qp + 0.5f) && !b.push(part))
-                    record_overflow(p.ctl, phase, i, b.n - 1 - 4 * W, part);
+                    record_overflow(p.ctl, phase, i, b.n - 1 - L::P, part);
             }
         }
     }
@@ -59,12 +62,12 @@ __device__ inline void pull_site(const Params &p, uint32_t i, uint32_t im, uint3
         for (int w = 0; w < W; w++) {
             if (wp[w] == 0) continue;
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const uint32_t part = (wp[w] >> (8 * q)) & 0xFFu;
+            for (int q = 0; q < Q; q++) {
+                const uint32_t part = L::part(wp[w], q);
                 if (part == 0) continue;
-                const float u = uniform01(ip, p.pcb, 4 * w + q, h);
+                const float u = uniform01(ip, p.pcb, Q * w + q, h);
                 const float qm = lookup_T(p, part, stp, st);
-                if (u < qm && !b.push(part)) record_overflow(p.ctl, phase, i, b.n - 1 - 4 * W, part);
+                if (u < qm && !b.push(part)) record_overflow(p.ctl, phase, i, b.n - 1 - L::P, part);
             }
         }
     }
@@ -72,16 +75,24 @@ __device__ inline void pull_site(const Params &p, uint32_t i, uint32_t im, uint3
     for (int w = 0; w < W; w++) out[w] = b.w[w];
 }
 
-// calculateReactionPropensity, global-R branch (byte_reaction_dev.cu:77-100).
 template <int W>
+__device__ inline void pull_site(const Params &p, uint32_t i, uint32_t im, uint32_t ip, const uint32_t *wo,
+                                 const uint32_t *wm, const uint32_t *wp, uint32_t st, uint32_t stm, uint32_t stp,
+                                 bool has_m, bool has_p, unsigned long long h, uint32_t phase, uint32_t *out)
+{
+    pull_site_l<ByteSites<W>>(p, i, im, ip, wo, wm, wp, st, stm, stp, has_m, has_p, h, phase, out);
+}
+
+// calculateReactionPropensity, global-R branch (byte_reaction_dev.cu:77-100).
+template <class L>
 __device__ inline float reaction_propensity(const Params &p, uint32_t st, const uint32_t *w, uint32_t r)
 {
     if (!p.RL[r * p.nST + st]) return 0.0f;
     const uint32_t d1 = p.D1[r], d2 = p.D2[r];
     float n1 = 0.0f, n2 = 0.0f;
 #pragma unroll
-    for (int j = 0; j < 4 * W; j++) {
-        const uint32_t s = slot_of(w, j);
+    for (int j = 0; j < L::P; j++) {
+        const uint32_t s = L::slot(w, j);
         if (s > 0) {
             n1 += (s == d1) ? 1.0f : 0.0f;
             n2 += (s == d2) ? 1.0f : 0.0f;
@@ -103,23 +114,23 @@ __device__ inline float reaction_propensity(const Params &p, uint32_t st, const 
 // true if the site's particles changed.  Needs h3 = timestep hash of substep 3.
 // ---------------------------------------------------------------------------
 // Total propensity in the reference's summation order (:1617-1639).
-template <int W>
+template <class L>
 __device__ inline float total_propensity(const Params &p, uint32_t st, const uint32_t *w)
 {
     const uint32_t nS = p.nS;
     float a = p.qp0[st];
     uint32_t any = 0;
 #pragma unroll
-    for (int k = 0; k < W; k++) any |= w[k];
+    for (int k = 0; k < L::W; k++) any |= w[k];
     if (any != 0) {
 #pragma unroll
-        for (int m = 0; m < 4 * W; m++) {
-            const uint32_t p1 = slot_of(w, m);
+        for (int m = 0; m < L::P; m++) {
+            const uint32_t p1 = L::slot(w, m);
             if (p1 > 0) {
                 a += p.qp1[st * nS + (p1 - 1)];
 #pragma unroll
-                for (int n = m + 1; n < 4 * W; n++) {
-                    const uint32_t p2 = slot_of(w, n);
+                for (int n = m + 1; n < L::P; n++) {
+                    const uint32_t p2 = L::slot(w, n);
                     if (p2 > 0) a += p.qp2[(st * nS + (p1 - 1)) * nS + (p2 - 1)];
                 }
             }
@@ -128,36 +139,42 @@ __device__ inline float total_propensity(const Params &p, uint32_t st, const uin
     return a;
 }
 
-template <int W>
+template <class L>
 __device__ inline bool react_apply(const Params &p, uint32_t i, uint32_t st, uint32_t *w, float a, float prob,
                                    unsigned long long h3);
+
+template <class L>
+__device__ inline bool react_site_l(const Params &p, uint32_t i, uint32_t st, uint32_t *w, unsigned long long h3)
+{
+    const float a = total_propensity<L>(p, st, w);
+    if (a == 0.0f) return false;
+    return react_apply<L>(p, i, st, w, a, reaction_probability(a), h3);
+}
 
 template <int W>
 __device__ inline bool react_site(const Params &p, uint32_t i, uint32_t st, uint32_t *w, unsigned long long h3)
 {
-    const float a = total_propensity<W>(p, st, w);
-    if (a == 0.0f) return false;
-    return react_apply<W>(p, i, st, w, a, reaction_probability(a), h3);
+    return react_site_l<ByteSites<W>>(p, i, st, w, h3);
 }
 
 // Same result as react_site.  A site holding at most one particle takes its
 // total propensity as the same float sum and its probability from the table
 // precomputed on the device with the same exp (prob1[st][species],
 // prob0[st]), so the double-precision exp runs only for multi-particle sites.
-template <int W>
-__device__ inline bool react_site_table(const Params &p, uint32_t i, uint32_t st, uint32_t *w,
-                                        unsigned long long h3, const float2 *tab)
+template <class L>
+__device__ inline bool react_site_table_l(const Params &p, uint32_t i, uint32_t st, uint32_t *w,
+                                          unsigned long long h3, const float2 *tab)
 {
     uint32_t n = 0, first = 0;
 #pragma unroll
-    for (int j = 4 * W - 1; j >= 0; j--) {
-        const uint32_t s = slot_of(w, j);
+    for (int j = L::P - 1; j >= 0; j--) {
+        const uint32_t s = L::slot(w, j);
         if (s) {
             n++;
             first = s;
         }
     }
-    if (n > 1) return react_site<W>(p, i, st, w, h3);
+    if (n > 1) return react_site_l<L>(p, i, st, w, h3);
     float a, prob;
     if (n == 1) {
         // tab (prob1, or its copy in LDS) holds (qp0[st] + qp1[st][s], its
@@ -170,13 +187,20 @@ __device__ inline bool react_site_table(const Params &p, uint32_t i, uint32_t st
         prob = p.prob0[st];
     }
     if (a == 0.0f) return false;
-    return react_apply<W>(p, i, st, w, a, prob, h3);
+    return react_apply<L>(p, i, st, w, a, prob, h3);
+}
+
+template <int W>
+__device__ inline bool react_site_table(const Params &p, uint32_t i, uint32_t st, uint32_t *w,
+                                        unsigned long long h3, const float2 *tab)
+{
+    return react_site_table_l<ByteSites<W>>(p, i, st, w, h3, tab);
 }
 
 // checkForReaction, determineReactionIndex and evaluateReaction
 // (byte_reaction_dev.cu:139-257) for a site with total propensity a and
 // reaction probability prob.
-template <int W>
+template <class L>
 __device__ inline bool react_apply(const Params &p, uint32_t i, uint32_t st, uint32_t *w, float a, float prob,
                                    unsigned long long h3)
 {
@@ -188,7 +212,7 @@ __device__ inline bool react_apply(const Params &p, uint32_t i, uint32_t st, uin
     // is absent has propensity 0 (or -0) and changes nothing, so only the
     // reactions listed under a species present at the site, plus the
     // zero-order ones, are visited: a merge of those ascending lists.
-    constexpr int P = 4 * W;
+    constexpr int P = L::P, W = L::W;
     constexpr uint32_t kNone = 0xFFFFFFFFu;
     float rp = uniform01(i, 1, 1, h3) * a;
     uint32_t idx = 0;
@@ -196,10 +220,10 @@ __device__ inline bool react_apply(const Params &p, uint32_t i, uint32_t st, uin
         uint32_t cur[P + 1], end[P + 1], head[P + 1];
 #pragma unroll
         for (int j = 0; j < P; j++) {
-            const uint32_t s = slot_of(w, j);
+            const uint32_t s = L::slot(w, j);
             bool first = s != 0;
 #pragma unroll
-            for (int q = 0; q < j; q++) first = first && (slot_of(w, q) != s);
+            for (int q = 0; q < j; q++) first = first && (L::slot(w, q) != s);
             const uint32_t sl = first ? s - 1 : 0u;
             cur[j] = first ? p.rx_off[sl] : 0u;
             end[j] = first ? p.rx_off[sl + 1] : 0u;
@@ -224,7 +248,7 @@ __device__ inline bool react_apply(const Params &p, uint32_t i, uint32_t st, uin
                     cur[j]++;
                     head[j] = cur[j] < end[j] ? p.rx_list[cur[j]] : kNone;
                 }
-            const float pr = reaction_propensity<W>(p, st, w, best);
+            const float pr = reaction_propensity<L>(p, st, w, best);
             if (pr > 0.0f) {
                 if (rp > 0.0f) idx = best;
                 rp -= pr;
@@ -235,11 +259,11 @@ __device__ inline bool react_apply(const Params &p, uint32_t i, uint32_t st, uin
     // the m-th particle of species s is consumed iff fewer than -S[s] earlier
     // particles of s were seen.  S[idx] comes from its nonzero entries.
     const uint32_t k0 = p.sto_off[idx], k1 = p.sto_off[idx + 1];
-    SlotBuf<W> out;
+    typename L::Buf out;
     out.clear();
 #pragma unroll
     for (int m = 0; m < P; m++) {
-        const uint32_t s = slot_of(w, m);
+        const uint32_t s = L::slot(w, m);
         if (s == 0) continue;
         int sv = 0;
         for (uint32_t k = k0; k < k1; k++)
@@ -249,7 +273,7 @@ __device__ inline bool react_apply(const Params &p, uint32_t i, uint32_t st, uin
         } else {
             uint32_t earlier = 0;
 #pragma unroll
-            for (int q = 0; q < m; q++) earlier += (slot_of(w, q) == s) ? 1u : 0u;
+            for (int q = 0; q < m; q++) earlier += (L::slot(w, q) == s) ? 1u : 0u;
             if ((int)earlier >= -sv) out.push(s);
         }
     }
@@ -262,7 +286,7 @@ __device__ inline bool react_apply(const Params &p, uint32_t i, uint32_t st, uin
         if (sv < 0) {
             uint32_t present = 0;
 #pragma unroll
-            for (int q = 0; q < P; q++) present += (slot_of(w, q) == s + 1) ? 1u : 0u;
+            for (int q = 0; q < P; q++) present += (L::slot(w, q) == s + 1) ? 1u : 0u;
             if ((int)present < -sv) underflow = true;
             continue;
         }

@@ -82,6 +82,8 @@ struct lmx_solver {
 
     // Diffusion model and lattice.
     uint32_t nST = 0, X = 0, Y = 0, Z = 0, P = 0, W = 0;
+    uint32_t bpp = 1;  // bytes per particle: 1 (byte lattice) or 4 (uint32 lattice, one particle per word)
+    uint32_t counts_cap = 256;  // entries of d_counts
     size_t N = 0;
     double lambda = 0.0;
     std::vector<double> DF;
@@ -198,6 +200,7 @@ lmx::Params make_params(const lmx_solver *s)
     p.XY = s->X * s->Y;
     p.N = (uint32_t)s->N;
     p.P = s->P; p.W = s->W;
+    p.word_particles = s->bpp == 4 ? 1u : 0u;
     p.pcb = s->P <= 4 ? 2u : (s->P <= 8 ? 3u : 4u);  // MPD_PARTICLE_COUNT_BITS
     p.nS = s->nS; p.nST = s->nST; p.R = s->R;
     p.boundary = (uint32_t)s->boundary;
@@ -228,11 +231,16 @@ int require_ready(lmx_solver *s)
     return select_device(s);
 }
 
-// Row-major (x,y,z,p) -> native planes, inserting each nonzero byte at the
-// first empty slot of its site (RDMESolver::setLatticeData + addParticle).
-int row_major_to_native(const lmx_solver *s, const uint8_t *rm, std::vector<uint32_t> &native)
+// Row-major (x,y,z,p) -> native planes, inserting each nonzero particle at
+// the first empty slot of its site (RDMESolver::setLatticeData + addParticle;
+// IntLattice::setFromRowMajorData for the uint32 lattice).  rm holds N*P
+// elements of bpp bytes.
+int row_major_to_native(const lmx_solver *s, const void *rm, std::vector<uint32_t> &native)
 {
     const size_t N = s->N;
+    const bool words = s->bpp == 4;
+    const uint8_t *rb = static_cast<const uint8_t *>(rm);
+    const uint32_t *rw = static_cast<const uint32_t *>(rm);
     native.assign(N * s->W, 0u);
     std::vector<uint8_t> fill(N, 0);
     size_t idx = 0;
@@ -241,25 +249,34 @@ int row_major_to_native(const lmx_solver *s, const uint8_t *rm, std::vector<uint
             for (uint32_t z = 0; z < s->Z; z++) {
                 const size_t i = (size_t)x + (size_t)s->X * y + (size_t)s->X * s->Y * z;
                 for (uint32_t l = 0; l < s->P; l++, idx++) {
-                    const uint8_t v = rm[idx];
+                    const uint32_t v = words ? rw[idx] : rb[idx];
                     if (v == 0) continue;
                     if (v > s->nS) return fail(LMX_ERR_ARG, "latticeData: an invalid species was found: " + std::to_string(v));
                     const uint32_t j = fill[i]++;
-                    native[(size_t)(j >> 2) * N + i] |= (uint32_t)v << (8 * (j & 3));
+                    if (words)
+                        native[(size_t)j * N + i] = v;
+                    else
+                        native[(size_t)(j >> 2) * N + i] |= v << (8 * (j & 3));
                 }
             }
     return LMX_OK;
 }
 
-void native_to_row_major(const lmx_solver *s, const uint32_t *native, uint8_t *rm)
+void native_to_row_major(const lmx_solver *s, const uint32_t *native, void *rm)
 {
     const size_t N = s->N;
     for (uint32_t x = 0; x < s->X; x++)
         for (uint32_t y = 0; y < s->Y; y++)
             for (uint32_t z = 0; z < s->Z; z++) {
                 const size_t i = (size_t)x + (size_t)s->X * y + (size_t)s->X * s->Y * z;
-                uint8_t *d = rm + (((size_t)x * s->Y + y) * s->Z + z) * s->P;
-                for (uint32_t j = 0; j < s->P; j++) d[j] = (uint8_t)(native[(size_t)(j >> 2) * N + i] >> (8 * (j & 3)));
+                const size_t o = (((size_t)x * s->Y + y) * s->Z + z) * s->P;
+                if (s->bpp == 4) {
+                    uint32_t *d = static_cast<uint32_t *>(rm) + o;
+                    for (uint32_t j = 0; j < s->P; j++) d[j] = native[(size_t)j * N + i];
+                } else {
+                    uint8_t *d = static_cast<uint8_t *>(rm) + o;
+                    for (uint32_t j = 0; j < s->P; j++) d[j] = (uint8_t)(native[(size_t)(j >> 2) * N + i] >> (8 * (j & 3)));
+                }
             }
 }
 
@@ -502,7 +519,6 @@ int lmx_build_model(lmx_solver *s, uint32_t nS, uint32_t R, const uint32_t *init
     if (!s) return fail(LMX_ERR_ARG, "null solver");
     if (!s->initialized) return fail(LMX_ERR_STATE, "initialize must be called before buildModel");
     if (R > 0 && kCols == 0) return fail(LMX_ERR_ARG, "K: must have at least 1 column");
-    if (nS > 255) return fail(LMX_ERR_ARG, "a byte lattice holds at most 255 species");
     if (R > 16384) return fail(LMX_ERR_ARG, "The number of reaction table entries exceeds the maximum supported by the solver (16384).");
     if ((R > 0 || nS > 0) && (!types || !K || !S || !D) && R > 0) return fail(LMX_ERR_ARG, "null model array");
     s->nS = nS;
@@ -569,7 +585,11 @@ int lmx_build_diffusion_model(lmx_solver *s, uint32_t nST, const double *DF, con
     if (!s) return fail(LMX_ERR_ARG, "null solver");
     if (!s->model_built) return fail(LMX_ERR_STATE, "buildModel must be called before buildDiffusionModel");
     if (!rowMajor) return fail(LMX_ERR_ARG, "rowMajorData: columns major lattice data is not currently supported");
-    if (bytesPerParticle != 1) return fail(LMX_ERR_ARG, "bytes_per_particle must be 1 for the byte lattice");
+    if (bytesPerParticle != 1 && bytesPerParticle != 4)
+        return fail(LMX_ERR_ARG, "bytes_per_particle must be 1 (byte lattice) or 4 (uint32 lattice)");
+    if (bytesPerParticle == 1 && s->nS > 255) return fail(LMX_ERR_ARG, "a byte lattice holds at most 255 species");
+    if ((unsigned long long)nST * s->nS * s->nS >= (1ull << 32))
+        return fail(LMX_ERR_ARG, "numberSiteTypes * numberSpecies^2 must be below 2^32 (second-order propensity table)");
     if (P != 4 && P != 8 && P != 16) return fail(LMX_ERR_ARG, "particlesPerSite must be 4, 8 or 16");
     if (nST == 0 || nST > 256) return fail(LMX_ERR_ARG, "numberSiteTypes must be in 1..256");
     if (X == 0 || Y == 0 || Z == 0) return fail(LMX_ERR_ARG, "incorrect lattice dimensions");
@@ -581,7 +601,9 @@ int lmx_build_diffusion_model(lmx_solver *s, uint32_t nST, const double *DF, con
     LMX_HIP(hipStreamSynchronize(s->stream));
 
     s->nST = nST;
-    s->X = X; s->Y = Y; s->Z = Z; s->P = P; s->W = P / 4;
+    s->X = X; s->Y = Y; s->Z = Z; s->P = P;
+    s->bpp = bytesPerParticle;
+    s->W = bytesPerParticle == 4 ? P : P / 4;
     s->N = (size_t)N;
     s->lambda = lambda;
     const size_t nT = (size_t)s->nS * nST * nST;
@@ -605,14 +627,21 @@ int lmx_build_diffusion_model(lmx_solver *s, uint32_t nST, const double *DF, con
     hipError_t e = hipMalloc((void **)&s->d_lat[0], lat_bytes);
     if (e == hipSuccess) e = hipMalloc((void **)&s->d_lat[1], lat_bytes);
     if (e == hipSuccess) e = hipMalloc((void **)&s->d_sites, s->N);
+    if (e == hipSuccess && s->nS > s->counts_cap) {
+        hipFree(s->d_counts);
+        s->d_counts = nullptr;
+        e = hipMalloc((void **)&s->d_counts, (size_t)s->nS * sizeof(uint32_t));
+        s->counts_cap = e == hipSuccess ? s->nS : 0u;
+    }
     // The reaction list: kListShards shards of cap entries (index, W words,
     // type byte).  Any capacity is correct (an overflowing shard makes the
     // reaction phase a full pass); N/8 entries in total hold every occupied
     // site up to 12.5% occupancy.
     const size_t rcap = std::max<size_t>(64, (s->N / 8 + lmx::kListShards - 1) / lmx::kListShards);
     const size_t rtotal = rcap * lmx::kListShards;
-    if (e == hipSuccess) e = hipMalloc((void **)&s->d_rl, rtotal * (1 + s->W) * sizeof(uint32_t) + rtotal);
-    if (e == hipSuccess) {
+    // (the byte lattice's vector kernels only)
+    if (e == hipSuccess && s->bpp == 1) e = hipMalloc((void **)&s->d_rl, rtotal * (1 + s->W) * sizeof(uint32_t) + rtotal);
+    if (e == hipSuccess && s->d_rl) {
         s->rlist.idx = s->d_rl;
         s->rlist.words = s->d_rl + rtotal;
         s->rlist.type = reinterpret_cast<uint8_t *>(s->d_rl + rtotal * (1 + s->W));
@@ -767,7 +796,7 @@ int lmx_get_lattice(lmx_solver *s, uint8_t *rm, size_t size)
 {
     int rc = require_ready(s);
     if (rc) return rc;
-    if (!rm || size != s->N * s->P) return fail(LMX_ERR_ARG, "bufferSize: the buffer size was not equal to the lattice data size");
+    if (!rm || size != s->N * s->P * s->bpp) return fail(LMX_ERR_ARG, "bufferSize: the buffer size was not equal to the lattice data size");
     std::vector<uint32_t> native(s->N * s->W);
     if ((rc = lmx_get_lattice_native(s, native.data(), native.size()))) return rc;
     native_to_row_major(s, native.data(), rm);
@@ -795,10 +824,15 @@ int lmx_set_lattice_native(lmx_solver *s, const uint32_t *native, size_t words)
     int rc = require_ready(s);
     if (rc) return rc;
     if (!native || words != s->N * s->W) return fail(LMX_ERR_ARG, "the buffer size was not equal to the lattice data size");
-    for (size_t i = 0; i < words; i++)
+    for (size_t i = 0; i < words; i++) {
+        if (s->bpp == 4) {
+            if (native[i] > s->nS) return fail(LMX_ERR_ARG, "latticeData: an invalid species was found");
+            continue;
+        }
         for (int b = 0; b < 4; b++)
             if (((native[i] >> (8 * b)) & 0xFFu) > s->nS)
                 return fail(LMX_ERR_ARG, "latticeData: an invalid species was found");
+    }
     LMX_HIP(hipStreamSynchronize(s->stream));
     LMX_HIP(hipMemcpy(s->d_lat[s->cur], native, words * sizeof(uint32_t), hipMemcpyHostToDevice));
     return LMX_OK;
@@ -831,7 +865,7 @@ int lmx_set_lattice(lmx_solver *s, const uint8_t *rm, size_t size)
 {
     int rc = require_ready(s);
     if (rc) return rc;
-    if (!rm || size != s->N * s->P) return fail(LMX_ERR_ARG, "bufferSize: the buffer size was not equal to the lattice data size");
+    if (!rm || size != s->N * s->P * s->bpp) return fail(LMX_ERR_ARG, "bufferSize: the buffer size was not equal to the lattice data size");
     std::vector<uint32_t> native;
     if ((rc = row_major_to_native(s, rm, native))) return rc;
     LMX_HIP(hipStreamSynchronize(s->stream));
@@ -856,12 +890,12 @@ int lmx_get_species_counts(lmx_solver *s, uint32_t *counts, uint32_t n)
     int rc = require_ready(s);
     if (rc) return rc;
     if (!counts || n < s->nS) return fail(LMX_ERR_ARG, "counts buffer smaller than the number of species");
-    LMX_HIP(hipMemsetAsync(s->d_counts, 0, 256 * sizeof(uint32_t), s->stream));
-    hipError_t e = lmx::launch_counts(s->d_lat[s->cur], s->N * s->W, s->nS, s->d_counts, s->stream);
+    LMX_HIP(hipMemsetAsync(s->d_counts, 0, s->counts_cap * sizeof(uint32_t), s->stream));
+    hipError_t e = lmx::launch_counts(s->d_lat[s->cur], s->N * s->W, s->nS, s->bpp == 4, s->d_counts, s->stream);
     if (e != hipSuccess) return hip_fail(e, "count launch");
     LMX_HIP(hipStreamSynchronize(s->stream));
-    std::vector<uint32_t> h(256);
-    LMX_HIP(hipMemcpy(h.data(), s->d_counts, 256 * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    std::vector<uint32_t> h(s->counts_cap);
+    LMX_HIP(hipMemcpy(h.data(), s->d_counts, s->counts_cap * sizeof(uint32_t), hipMemcpyDeviceToHost));
     for (uint32_t i = 0; i < n; i++) counts[i] = i < s->nS ? h[i] : 0u;
     return LMX_OK;
 }
@@ -1204,6 +1238,7 @@ int lmx_batch_add(lmx_batch *b, lmx_solver *s)
     int rc = require_ready(s);
     if (rc) return rc;
     const lmx::Params p = make_params(s);
+    if (s->bpp != 1) return fail(LMX_ERR_ARG, "batched solvers need the byte lattice");
     if (!lmx::fast_path_ok(p)) return fail(LMX_ERR_ARG, "batched solvers need X % 4 == 0");
     if (!b->members.empty() && s->W != b->W) return fail(LMX_ERR_ARG, "batched solvers need the same particles per site");
     b->W = s->W;

@@ -1630,7 +1630,8 @@ hipError_t launch_fused(const Params &p, const uint32_t *src, uint32_t *dst, uin
     return hipGetLastError();
 }
 
-bool fast_path_ok(const Params &p) { return (p.X & 3u) == 0 && p.N >= 4; }
+// The vector kernels hold four particles per word (the byte lattice).
+bool fast_path_ok(const Params &p) { return !p.word_particles && (p.X & 3u) == 0 && p.N >= 4; }
 
 hipError_t launch_sweep_fast(const Params &p, const uint32_t *src, uint32_t *dst, int axis, uint32_t seg,
                              hipStream_t st)

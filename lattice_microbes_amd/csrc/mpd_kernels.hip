@@ -6,7 +6,8 @@
 //                   (precomp_reaction_kernel, MpdRdmeSolver.cu:1592-1675)
 //  resolve_kernel   canonical-order overflow placement on the device (the
 //                   classic host handler, MpdRdmeSolver.cu:1103-1189)
-//  count_kernel     species counts (ByteLattice::getParticleCounts)
+//  count_kernel     species counts (ByteLattice::getParticleCounts; the
+//                   uint32 lattice's in count_words_kernel)
 //
 // Every per-particle decision depends only on (site index, slot, timestep,
 // substep) through the counter-based hash, so the results do not depend on
@@ -24,11 +25,12 @@ namespace lmx {
 // Generic sweep: one site per thread, any lattice shape (the vector kernels in
 // mpd_fast.hip need X % 4 == 0).
 // ---------------------------------------------------------------------------
-template <int W>
+template <class L>
 __global__ __launch_bounds__(256) void sweep_kernel(Params p, const uint32_t *__restrict__ src,
                                                     uint32_t *__restrict__ dst, uint32_t stride,
                                                     uint32_t len, uint32_t substep)
 {
+    constexpr int W = L::W;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= p.N) return;
     const unsigned long long h = timestep_hash(p.seed32, p.ctl->timestep, substep);
@@ -55,15 +57,16 @@ __global__ __launch_bounds__(256) void sweep_kernel(Params p, const uint32_t *__
     const uint32_t sti = p.sites[i];
     const uint32_t stm = has_m ? p.sites[im] : 0u;
     const uint32_t stp = has_p ? p.sites[ip] : 0u;
-    pull_site<W>(p, i, im, ip, wi, wm, wp, sti, stm, stp, has_m, has_p, h, substep, out);
+    pull_site_l<L>(p, i, im, ip, wi, wm, wp, sti, stm, stp, has_m, has_p, h, substep, out);
 #pragma unroll
     for (int k = 0; k < W; k++) dst[(size_t)k * p.N + i] = out[k];
 }
 
 // Generic reaction phase: one site per thread, in place.
-template <int W>
+template <class L>
 __global__ __launch_bounds__(256) void reaction_kernel(Params p, uint32_t *__restrict__ lat)
 {
+    constexpr int W = L::W;
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= p.N) return;
     const uint32_t st = p.sites[i];
@@ -71,7 +74,7 @@ __global__ __launch_bounds__(256) void reaction_kernel(Params p, uint32_t *__res
 #pragma unroll
     for (int k = 0; k < W; k++) w[k] = lat[(size_t)k * p.N + i];
     const unsigned long long h = timestep_hash(p.seed32, p.ctl->timestep, 3);
-    if (react_site<W>(p, i, st, w, h)) {
+    if (react_site_l<L>(p, i, st, w, h)) {
 #pragma unroll
         for (int k = 0; k < W; k++) lat[(size_t)k * p.N + i] = w[k];
     }
@@ -95,11 +98,17 @@ __device__ inline void store_agent(uint32_t *ptr, uint32_t v)
     __hip_atomic_store(ptr, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Particles up to the first empty slot (Lattice::getOccupancy).
 __device__ inline uint32_t site_occupancy(const Params &p, const uint32_t *lat, uint32_t s)
 {
     uint32_t occ = 0;
     for (uint32_t k = 0; k < p.W; k++) {
         const uint32_t word = load_agent(lat + (size_t)k * p.N + s);
+        if (p.word_particles) {
+            if (word == 0) return occ;
+            occ++;
+            continue;
+        }
         for (uint32_t b = 0; b < 4; b++) {
             if (((word >> (8 * b)) & 0xFFu) == 0) return occ;
             occ++;
@@ -167,7 +176,7 @@ __device__ void resolve_body(const Params &p, uint32_t *lat)
     for (uint32_t e = 0; e < n && !failed; e++) {
         const unsigned long long key = keys[e];
         const uint32_t site = (uint32_t)((key >> 24) & 0xFFFFFFFFull);
-        const uint32_t particle = (uint32_t)(key & 0xFFull);
+        const uint32_t particle = ctl->ovf_part[key & 0x7FFull];
         const uint32_t xc = site % X, yc = (site / X) % Y, zc = site / p.XY;
         const uint32_t st0 = p.sites[site];
         int found = 0x7fffffff;
@@ -220,9 +229,13 @@ __device__ void resolve_body(const Params &p, uint32_t *lat)
                     const uint32_t z = z0 + rem % nz;
                     const uint32_t s2 = x + X * y + p.XY * z;
                     const uint32_t occ = site_occupancy(p, lat, s2);
-                    uint32_t *wp = lat + (size_t)(occ >> 2) * p.N + s2;
-                    const uint32_t old = load_agent(wp);
-                    store_agent(wp, old | (particle << (8 * (occ & 3))));
+                    if (p.word_particles) {
+                        store_agent(lat + (size_t)occ * p.N + s2, particle);
+                    } else {
+                        uint32_t *wp = lat + (size_t)(occ >> 2) * p.N + s2;
+                        const uint32_t old = load_agent(wp);
+                        store_agent(wp, old | (particle << (8 * (occ & 3))));
+                    }
                 }
             }
         }
@@ -280,6 +293,18 @@ __global__ __launch_bounds__(256) void count_kernel(const uint32_t *__restrict__
         if (k >= 1 && k <= nS && hist[k]) atomicAdd(&counts[k - 1], hist[k]);
 }
 
+// Species counts of the uint32 lattice (IntLattice::getParticleCounts,
+// IntLattice.cpp:419-435): every nonzero word is one particle.
+__global__ __launch_bounds__(256) void count_words_kernel(const uint32_t *__restrict__ lat, size_t nwords,
+                                                          uint32_t nS, uint32_t *__restrict__ counts)
+{
+    for (size_t wi = (size_t)blockIdx.x * blockDim.x + threadIdx.x; wi < nwords;
+         wi += (size_t)gridDim.x * blockDim.x) {
+        const uint32_t s = lat[wi];
+        if (s >= 1 && s <= nS) atomicAdd(&counts[s - 1], 1u);
+    }
+}
+
 // Range check of a lattice changed on the device by a caller: particle
 // types at most nS, site types below nST (the kernels index their tables with
 // them).  A violation is a sticky error, reported by the next synchronize.
@@ -289,6 +314,10 @@ __global__ __launch_bounds__(256) void validate_kernel(Params p, const uint32_t 
     const size_t stride = (size_t)gridDim.x * blockDim.x;
     for (size_t wi = (size_t)blockIdx.x * blockDim.x + threadIdx.x; wi < nwords; wi += stride) {
         const uint32_t v = lat[wi];
+        if (p.word_particles) {
+            bad |= v > p.nS;
+            continue;
+        }
 #pragma unroll
         for (int b = 0; b < 4; b++) bad |= ((v >> (8 * b)) & 0xFFu) > p.nS;
     }
@@ -336,15 +365,26 @@ __global__ void prob_kernel(const float *a, float *y, size_t n)
 // ---------------------------------------------------------------------------
 static inline unsigned blocks_for(size_t n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }
 
+template <class L>
+static void sweep_launch(const Params &p, const uint32_t *src, uint32_t *dst, uint32_t stride, uint32_t len,
+                         int axis, hipStream_t st)
+{
+    hipLaunchKernelGGL(sweep_kernel<L>, dim3(blocks_for(p.N, 256)), dim3(256), 0, st, p, src, dst, stride, len,
+                       (uint32_t)axis);
+}
+
 hipError_t launch_sweep(const Params &p, const uint32_t *src, uint32_t *dst, int axis, hipStream_t st)
 {
     const uint32_t stride = axis == 0 ? 1u : (axis == 1 ? p.X : p.XY);
     const uint32_t len = axis == 0 ? p.X : (axis == 1 ? p.Y : p.Z);
-    const dim3 grid(blocks_for(p.N, 256)), block(256);
-    switch (p.W) {
-    case 1: hipLaunchKernelGGL(sweep_kernel<1>, grid, block, 0, st, p, src, dst, stride, len, (uint32_t)axis); break;
-    case 2: hipLaunchKernelGGL(sweep_kernel<2>, grid, block, 0, st, p, src, dst, stride, len, (uint32_t)axis); break;
-    case 4: hipLaunchKernelGGL(sweep_kernel<4>, grid, block, 0, st, p, src, dst, stride, len, (uint32_t)axis); break;
+    const int key = p.word_particles ? 100 + (int)p.P : (int)p.W;
+    switch (key) {
+    case 1: sweep_launch<ByteSites<1>>(p, src, dst, stride, len, axis, st); break;
+    case 2: sweep_launch<ByteSites<2>>(p, src, dst, stride, len, axis, st); break;
+    case 4: sweep_launch<ByteSites<4>>(p, src, dst, stride, len, axis, st); break;
+    case 104: sweep_launch<WordSites<4>>(p, src, dst, stride, len, axis, st); break;
+    case 108: sweep_launch<WordSites<8>>(p, src, dst, stride, len, axis, st); break;
+    case 116: sweep_launch<WordSites<16>>(p, src, dst, stride, len, axis, st); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -353,10 +393,14 @@ hipError_t launch_sweep(const Params &p, const uint32_t *src, uint32_t *dst, int
 hipError_t launch_reaction(const Params &p, uint32_t *lat, hipStream_t st)
 {
     const dim3 grid(blocks_for(p.N, 256)), block(256);
-    switch (p.W) {
-    case 1: hipLaunchKernelGGL(reaction_kernel<1>, grid, block, 0, st, p, lat); break;
-    case 2: hipLaunchKernelGGL(reaction_kernel<2>, grid, block, 0, st, p, lat); break;
-    case 4: hipLaunchKernelGGL(reaction_kernel<4>, grid, block, 0, st, p, lat); break;
+    const int key = p.word_particles ? 100 + (int)p.P : (int)p.W;
+    switch (key) {
+    case 1: hipLaunchKernelGGL(reaction_kernel<ByteSites<1>>, grid, block, 0, st, p, lat); break;
+    case 2: hipLaunchKernelGGL(reaction_kernel<ByteSites<2>>, grid, block, 0, st, p, lat); break;
+    case 4: hipLaunchKernelGGL(reaction_kernel<ByteSites<4>>, grid, block, 0, st, p, lat); break;
+    case 104: hipLaunchKernelGGL(reaction_kernel<WordSites<4>>, grid, block, 0, st, p, lat); break;
+    case 108: hipLaunchKernelGGL(reaction_kernel<WordSites<8>>, grid, block, 0, st, p, lat); break;
+    case 116: hipLaunchKernelGGL(reaction_kernel<WordSites<16>>, grid, block, 0, st, p, lat); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -374,12 +418,16 @@ hipError_t launch_batch_resolve(Member *m, uint32_t n, hipStream_t st)
     return hipGetLastError();
 }
 
-hipError_t launch_counts(const uint32_t *lat, size_t nwords, uint32_t nS, uint32_t *counts, hipStream_t st)
+hipError_t launch_counts(const uint32_t *lat, size_t nwords, uint32_t nS, bool words, uint32_t *counts,
+                         hipStream_t st)
 {
     unsigned g = blocks_for(nwords, 256);
     if (g > 4096) g = 4096;
     if (g == 0) g = 1;
-    hipLaunchKernelGGL(count_kernel, dim3(g), dim3(256), 0, st, lat, nwords, nS, counts);
+    if (words)
+        hipLaunchKernelGGL(count_words_kernel, dim3(g), dim3(256), 0, st, lat, nwords, nS, counts);
+    else
+        hipLaunchKernelGGL(count_kernel, dim3(g), dim3(256), 0, st, lat, nwords, nS, counts);
     return hipGetLastError();
 }
 

@@ -39,7 +39,7 @@ class ModelSpec:
     D: np.ndarray                # [nS, R] uint32
     DF: np.ndarray               # [nST, nST, nS] float64
     RL: np.ndarray               # [R, nST] uint32
-    lattice: np.ndarray          # [X, Y, Z, P] uint8
+    lattice: np.ndarray          # [X, Y, Z, P] uint8 (uint32 for more than 255 species)
     sites: np.ndarray            # [X, Y, Z] uint8
     boundary: str = "periodic"
     species_names: List[str] = field(default_factory=list)
@@ -73,8 +73,9 @@ class ModelSpec:
         solver.initialize(replicate, prm, boundary=self.boundary)
         solver.buildModel(self.nS, self.R, self.initial_counts(), self.types, self.K, self.S, self.D, 1)
         X, Y, Z = self.shape
-        solver.buildDiffusionModel(self.nST, self.DF, self.RL, X, Y, Z, self.P, 1, self.spacing,
-                                   self.lattice, self.sites, True)
+        solver.buildDiffusionModel(self.nST, self.DF, self.RL, X, Y, Z, self.P,
+                                   getattr(solver, "bytesPerParticle", 1), self.spacing, self.lattice, self.sites,
+                                   True)
         return solver
 
     # ---------------------------------------------------------------- files

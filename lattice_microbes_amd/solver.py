@@ -19,6 +19,7 @@ from . import _lib
 from .lattice import ByteLattice
 
 SOLVER_NAME = "lm::rdme::MpdRdmeSolver"
+INT_SOLVER_NAME = "lm::rdme::IntMpdRdmeSolver"
 
 
 def _ptr(a: np.ndarray, ctype):
@@ -38,6 +39,10 @@ def timestep_seed(seed32: int, timestep: int, substep: int) -> int:
 
 class MpdRdmeSolver:
     """Drop-in for the reference MPD-RDME solver, running on one MI355X."""
+
+    # The byte lattice: one byte per particle, at most 255 species
+    # (MpdRdmeSolver::allocateLattice, MpdRdmeSolver.cu:189-199).
+    bytesPerParticle = 1
 
     def __init__(self, device: int = 0):
         self._lib = _lib.load()
@@ -133,10 +138,17 @@ class MpdRdmeSolver:
                             latticeSpacing: float, latticeData, latticeSitesData,
                             rowMajorData: bool = True) -> None:
         """RDMESolver/MpdRdmeSolver::buildDiffusionModel.  latticeData is the
-        row-major (x,y,z,p) byte array, latticeSitesData row-major (x,y,z)."""
+        row-major (x,y,z,p) particle array (uint8, or uint32 for the uint32
+        lattice), latticeSitesData row-major (x,y,z)."""
+        if int(bytes_per_particle) != self.bytesPerParticle:
+            raise _lib.InvalidArgException(
+                _lib.ERR_ARG, f"{type(self).__name__} works only with lattices of {self.bytesPerParticle} "
+                              f"bytes per particle, not {int(bytes_per_particle)}")
+        if self.bytesPerParticle == 1 and self.numberSpecies > 255:
+            raise _lib.InvalidArgException(_lib.ERR_ARG, "a byte lattice holds at most 255 species")
         DFa = np.ascontiguousarray(np.asarray(DF, dtype=np.float64).reshape(-1))
         RLa = np.ascontiguousarray(np.asarray(RL, dtype=np.uint32).reshape(-1))
-        lat = np.ascontiguousarray(np.asarray(latticeData, dtype=np.uint8).reshape(-1))
+        lat = self._particle_array(latticeData)
         sites = np.ascontiguousarray(np.asarray(latticeSitesData, dtype=np.uint8).reshape(-1))
         X, Y, Z, P = int(latticeXSize), int(latticeYSize), int(latticeZSize), int(particlesPerSite)
         if lat.size != X * Y * Z * P:
@@ -147,7 +159,7 @@ class MpdRdmeSolver:
             RLa = np.zeros(1, dtype=np.uint32)
         _lib.check(self._lib.lmx_build_diffusion_model(
             self._h, int(numberSiteTypes), _ptr(DFa, c_double), _ptr(RLa, c_uint32), X, Y, Z, P,
-            int(bytes_per_particle), float(latticeSpacing), _ptr(lat, ctypes.c_uint8),
+            int(bytes_per_particle), float(latticeSpacing), _ptr(lat.view(np.uint8), ctypes.c_uint8),
             _ptr(sites, ctypes.c_uint8), 1 if rowMajorData else 0))
         self.numberSiteTypes = int(numberSiteTypes)
         self.shape = (X, Y, Z)
@@ -198,17 +210,33 @@ class MpdRdmeSolver:
         return total.value, per
 
     # ------------------------------------------------------------------ lattice
+    @property
+    def particleDtype(self):
+        return np.uint32 if self.bytesPerParticle == 4 else np.uint8
+
+    @property
+    def wordsPerSite(self) -> int:
+        return self.particlesPerSite if self.bytesPerParticle == 4 else self.particlesPerSite // 4
+
+    def _particle_array(self, data) -> np.ndarray:
+        a = np.asarray(data)
+        if a.dtype != self.particleDtype and a.size and (a.min() < 0 or a.max() > np.iinfo(self.particleDtype).max):
+            raise _lib.InvalidArgException(_lib.ERR_ARG, "latticeData: an invalid species was found")
+        return np.ascontiguousarray(a.astype(self.particleDtype, copy=False).reshape(-1))
+
     def getLattice(self) -> np.ndarray:
-        """Particles as row-major (X, Y, Z, P) uint8."""
+        """Particles as row-major (X, Y, Z, P): uint8, or uint32 for the
+        uint32 lattice."""
         X, Y, Z = self.shape
-        out = np.empty((X, Y, Z, self.particlesPerSite), dtype=np.uint8)
-        _lib.check(self._lib.lmx_get_lattice(self._h, _ptr(out, ctypes.c_uint8), out.size))
+        out = np.empty((X, Y, Z, self.particlesPerSite), dtype=self.particleDtype)
+        _lib.check(self._lib.lmx_get_lattice(self._h, _ptr(out.view(np.uint8), ctypes.c_uint8), out.nbytes))
         return out
 
     def getLatticeNative(self) -> np.ndarray:
-        """Particles in the native layout: (P/4, Z, Y, X) uint32 planes."""
+        """Particles in the native layout: (W, Z, Y, X) uint32 planes, W = P/4
+        (four byte slots per word) or P (one particle per word)."""
         X, Y, Z = self.shape
-        out = np.empty((self.particlesPerSite // 4, Z, Y, X), dtype=np.uint32)
+        out = np.empty((self.wordsPerSite, Z, Y, X), dtype=np.uint32)
         _lib.check(self._lib.lmx_get_lattice_native(self._h, _ptr(out, c_uint32), out.size))
         return out
 
@@ -219,8 +247,8 @@ class MpdRdmeSolver:
         return out
 
     def setLattice(self, latticeData) -> None:
-        lat = np.ascontiguousarray(np.asarray(latticeData, dtype=np.uint8).reshape(-1))
-        _lib.check(self._lib.lmx_set_lattice(self._h, _ptr(lat, ctypes.c_uint8), lat.size))
+        lat = self._particle_array(latticeData)
+        _lib.check(self._lib.lmx_set_lattice(self._h, _ptr(lat.view(np.uint8), ctypes.c_uint8), lat.nbytes))
 
     def setLatticeNative(self, native) -> None:
         lat = np.ascontiguousarray(np.asarray(native, dtype=np.uint32).reshape(-1))
@@ -592,10 +620,27 @@ class _Recorder:
                 "max_particle_counts": self.max_particles.copy(), "max_site_counts": self.max_sites.copy()}
 
 
+class IntMpdRdmeSolver(MpdRdmeSolver):
+    """Drop-in for lm::rdme::IntMpdRdmeSolver (IntMpdRdmeSolver.cu): the
+    same MPD-RDME step on the uint32 lattice, one particle per 32-bit word
+    (P planes of N words), so that models may have more than 255 species
+    (allocateLattice, :155-170, requires 4 bytes per particle).  The update
+    rules and RNG keys are those of the byte lattice
+    (word_diffusion_1d_dev.cu:106 keys word w as the byte kernels key slot w),
+    so both solvers give the same trajectory for a model that fits a byte.
+    Trajectories (generateTrajectory, hooks) run on the byte lattice only."""
+
+    bytesPerParticle = 4
+
+    def generateTrajectory(self, output=None):
+        raise NotImplementedError("trajectories with hooks and output run on the byte lattice (MpdRdmeSolver)")
+
+
 class MESolverFactory:
     """String-based selection (MESolverFactory.cpp:75-96,115-190)."""
 
-    _solvers: Dict[str, Callable[..., MpdRdmeSolver]] = {SOLVER_NAME: MpdRdmeSolver}
+    _solvers: Dict[str, Callable[..., MpdRdmeSolver]] = {SOLVER_NAME: MpdRdmeSolver,
+                                                          INT_SOLVER_NAME: IntMpdRdmeSolver}
 
     def __init__(self):
         self.name: Optional[str] = None
