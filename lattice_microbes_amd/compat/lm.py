@@ -9,9 +9,10 @@ select the MI355X solver unchanged:
 * ``ReactionModel``, ``DiffusionModel``, ``SpatialModel``: the protobuf
   messages with their generated accessors (src/protobuf/*.proto);
 * ``SimulationFile``: create, parameters, models, initial lattice, output;
-* ``ByteLattice``, ``getCompiledLatticeMaxOccupancy``;
+* ``ByteLattice``, ``IntLattice``, ``getCompiledLatticeMaxOccupancy``;
 * ``MpdRdmeSolver`` (an overridable ``hookSimulation(time, lattice)``),
-  ``RDMESolver`` (the same class: the only RDME solver here);
+  ``RDMESolver`` (the same class), ``IntMpdRdmeSolver`` (the uint32
+  lattice);
 * ``runSolver``, ``runSimulation``.
 
 Spatial geometry (regions, obstacles) and the other solvers of the reference
@@ -27,8 +28,9 @@ import numpy as np
 from lattice_microbes_amd import runner as _runner
 from lattice_microbes_amd import simfile as _sf
 from lattice_microbes_amd import solver as _solver
-from lattice_microbes_amd.lattice import ByteLattice
+from lattice_microbes_amd.lattice import ByteLattice, IntLattice, Lattice
 from lattice_microbes_amd.solver import HookLattice as CudaByteLattice  # what a hook receives
+from lattice_microbes_amd.solver import IntHookLattice as CudaIntLattice
 
 BUILD_CONFIGURATION = "lattice_microbes_amd (gfx950)"
 
@@ -227,17 +229,18 @@ class SimulationFile(_sf.SimulationFile):
         return dm
 
     def setDiffusionModelLattice(self, dm, lattice, sites=None) -> None:
-        """With an ``lm.ByteLattice``, as jLM and pyLM call it; or with
-        row-major lattice and site arrays."""
+        """With an ``lm.ByteLattice`` or ``lm.IntLattice``, as jLM and pyLM
+        call it (SimulationFile.cpp:800-900); or with row-major lattice and site
+        arrays."""
         data = dm._to_data() if isinstance(dm, DiffusionModel) else dm
-        if isinstance(lattice, ByteLattice):
+        if isinstance(lattice, Lattice):
             lattice, sites = lattice.row_major(), lattice.sites_row_major()
         super().setDiffusionModelLattice(data, lattice, sites)
 
     def getDiffusionModelLattice(self, dm=None, lattice=None):
         lat, sites = super().getDiffusionModelLattice()
-        if isinstance(lattice, ByteLattice):
-            lattice.setFromRowMajorByteData(lat, sites)
+        if isinstance(lattice, Lattice):
+            lattice.setFromRowMajorData(lat, sites)
             return lattice
         return lat, sites
 
@@ -262,6 +265,13 @@ class MpdRdmeSolver(_solver.MpdRdmeSolver):
 
 
 RDMESolver = MpdRdmeSolver
+
+
+class IntMpdRdmeSolver(_solver.IntMpdRdmeSolver):
+    """lm.IntMpdRdmeSolver: the uint32 lattice (4 bytes per particle)."""
+
+    def __init__(self, device: int = 0):
+        super().__init__(device)
 
 
 def runSolver(simulationFilename: str, replicate: int, solver, cudaDevices: List[int] = (0,),

@@ -78,6 +78,11 @@ class ModelSpec:
                                    True)
         return solver
 
+    @property
+    def bytes_per_particle(self) -> int:
+        """4 for a uint32 lattice (IntMpdRdmeSolver), else 1."""
+        return 4 if self.lattice.dtype == np.uint32 else 1
+
     # ---------------------------------------------------------------- files
     def reaction_model(self):
         """The ReactionModel message of this model (one rate constant each)."""
@@ -95,7 +100,7 @@ class ModelSpec:
         X, Y, Z = self.shape
         return DiffusionModel(number_species=self.nS, number_reactions=self.R, number_site_types=self.nST,
                               lattice_spacing=self.spacing, lattice_x_size=X, lattice_y_size=Y, lattice_z_size=Z,
-                              particles_per_site=self.P, bytes_per_particle=1,
+                              particles_per_site=self.P, bytes_per_particle=self.bytes_per_particle,
                               diffusion_matrix=self.DF.astype(np.float64),
                               reaction_location_matrix=self.RL.astype(np.uint32))
 
@@ -126,8 +131,6 @@ class ModelSpec:
             rm = f.getReactionModel()
             dm = f.getDiffusionModel()
             lattice, sites = f.getDiffusionModelLattice(dm)
-        if lattice.dtype != np.uint8:
-            raise ValueError("only byte lattices (bytes_per_particle 1) are supported by the MPD-RDME solver")
         R = rm.number_reactions
         K = np.array([ks[0] if ks else 0.0 for ks in rm.rate_constants], dtype=np.float64)
         return cls(name=name or path, shape=dm.shape, spacing=dm.lattice_spacing, tau=tau,

@@ -16,7 +16,7 @@ from typing import Callable, Dict, List, Optional
 import numpy as np
 
 from . import _lib
-from .lattice import ByteLattice
+from .lattice import ByteLattice, IntLattice
 
 SOLVER_NAME = "lm::rdme::MpdRdmeSolver"
 INT_SOLVER_NAME = "lm::rdme::IntMpdRdmeSolver"
@@ -346,7 +346,7 @@ class MpdRdmeSolver:
         hook_enabled = hook_iv not in ("", None)
         hi = int(float(hook_iv)) if hook_enabled else 0
         rec = _Recorder(self, output)
-        lattice = HookLattice(self)
+        lattice = IntHookLattice(self) if self.bytesPerParticle == 4 else HookLattice(self)
         device = bool(getattr(self, "deviceHooks", False))
         dlat = DeviceHookLattice(self) if device else None
 
@@ -452,12 +452,9 @@ def _last_step(tau: float, maxTime: float) -> int:
     return k
 
 
-class HookLattice(ByteLattice):
-    """The lattice a hook sees, in place of ``lm::rdme::CudaByteLattice``: a
-    host copy of the solver's particles and site types (``ByteLattice``
-    accessors and views over it).  ``copyFromGPU``/``copyToGPU`` move it
-    between the host and the solver; the trajectory loop copies it back when a
-    hook returns 1 or 2."""
+class _HookCopies:
+    """copyFromGPU/copyToGPU of a hook lattice: a host copy of the solver's
+    particles and site types in the native planes."""
 
     def __init__(self, solver: MpdRdmeSolver):
         X, Y, Z = solver.shape
@@ -473,6 +470,18 @@ class HookLattice(ByteLattice):
         self._solver.setSites(self.sites_row_major())
 
 
+class HookLattice(_HookCopies, ByteLattice):
+    """The lattice a hook sees, in place of ``lm::rdme::CudaByteLattice``: a
+    host copy of the solver's particles and site types (``ByteLattice``
+    accessors and views over it).  ``copyFromGPU``/``copyToGPU`` move it
+    between the host and the solver; the trajectory loop copies it back when a
+    hook returns 1 or 2."""
+
+
+class IntHookLattice(_HookCopies, IntLattice):
+    """The same for the uint32 lattice, in place of ``lm::rdme::CudaIntLattice``."""
+
+
 class _DeviceArray:
     """__cuda_array_interface__ (version 3) of device memory the solver owns."""
 
@@ -485,7 +494,10 @@ class DeviceHookLattice:
     """What a device-side hook sees (a solver class with ``deviceHooks =
     True``): the solver's current device buffers as torch tensors, with no
     copies.  ``getParticleLatticeView()`` is the (W, Z, Y, X, 4) uint8 view of
-    the particle planes, ``getSiteLatticeView()`` the (Z, Y, X) site types;
+    the byte lattice's particle planes, or the (P, Z, Y, X, 1) int32 view of
+    the uint32 lattice's (int32, the 32-bit integer type torch computes with;
+    the bits are the particle words), ``getSiteLatticeView()`` the (Z, Y, X)
+    site types;
     both are valid during the hook call only (the solver alternates between
     two particle buffers).  Changes made with torch operations, on any stream,
     are complete before the next step (the solver waits for the device), and
@@ -508,9 +520,12 @@ class DeviceHookLattice:
         pp, sp = c_void_p(), c_void_p()
         _lib.check(s._lib.lmx_device_lattice(s._h, byref(pp), byref(sp)))
         X, Y, Z = s.shape
-        W = s.particlesPerSite // 4
         torch = self._torch
-        self._particles = torch.as_tensor(_DeviceArray(pp.value, (W, Z, Y, X, 4), "|u1"), device=self._dev)
+        if s.bytesPerParticle == 4:
+            shape, typestr = (s.particlesPerSite, Z, Y, X, 1), "<i4"
+        else:
+            shape, typestr = (s.particlesPerSite // 4, Z, Y, X, 4), "|u1"
+        self._particles = torch.as_tensor(_DeviceArray(pp.value, shape, typestr), device=self._dev)
         self._sites = torch.as_tensor(_DeviceArray(sp.value, (Z, Y, X), "|u1"), device=self._dev)
 
     def synchronize(self) -> None:
@@ -542,7 +557,8 @@ class DeviceHookLattice:
         return self._solver.latticeSpacing
 
     def getParticleCounts(self) -> Dict[int, int]:
-        c = self._torch.bincount(self._particles.reshape(-1).to(self._torch.int64), minlength=256).cpu().numpy()
+        c = self._torch.bincount(self._particles.reshape(-1).to(self._torch.int64),
+                                 minlength=self._solver.numberSpecies + 1).cpu().numpy()
         return {int(p): int(c[p]) for p in np.flatnonzero(c) if p != 0}
 
 
@@ -564,7 +580,8 @@ class _Recorder:
         self.max_sites = np.zeros(solver.numberSiteTypes, np.uint32)
 
     def _particle_counts(self, lattice: HookLattice) -> np.ndarray:
-        c = np.bincount(lattice.getParticleLatticeView().reshape(-1), minlength=256)[: self.max_particles.size]
+        c = np.bincount(lattice.getParticleLatticeView().reshape(-1),
+                        minlength=self.max_particles.size)[: self.max_particles.size]
         c[0] = 0
         return c.astype(np.uint32)
 
@@ -586,7 +603,7 @@ class _Recorder:
             self.counts.append(self.solver.getSpeciesCounts())
         else:
             n = self.solver.numberSpecies
-            c = np.bincount(lattice.getParticleLatticeView().reshape(-1), minlength=256)
+            c = np.bincount(lattice.getParticleLatticeView().reshape(-1), minlength=n + 1)
             self.counts.append(c[1:n + 1].astype(np.uint32))
 
     def lattice(self, time: float, lattice: HookLattice) -> None:
@@ -628,12 +645,10 @@ class IntMpdRdmeSolver(MpdRdmeSolver):
     rules and RNG keys are those of the byte lattice
     (word_diffusion_1d_dev.cu:106 keys word w as the byte kernels key slot w),
     so both solvers give the same trajectory for a model that fits a byte.
-    Trajectories (generateTrajectory, hooks) run on the byte lattice only."""
+    Hooks receive an ``IntHookLattice`` (``IntLattice`` accessors); lattice
+    frames are written with 4 bytes per particle."""
 
     bytesPerParticle = 4
-
-    def generateTrajectory(self, output=None):
-        raise NotImplementedError("trajectories with hooks and output run on the byte lattice (MpdRdmeSolver)")
 
 
 class MESolverFactory:

@@ -132,7 +132,7 @@ class DeviceEditingSolver(EditingHooks, lmx.MpdRdmeSolver):
         self.calls.append(step)
         edit, code, rate = PLAN[step]
         if edit is not None:
-            edit.apply(lattice.getParticleLatticeView(), lattice.getSiteLatticeView())
+            edit.apply(lattice.getParticleLatticeView(), lattice.getSiteLatticeView(), self.species_offset)
         if rate is not None:
             self.setReactionRate(0, rate)
         assert lattice.getParticleLatticeView().is_cuda and lattice.getXSize() == 16
@@ -192,3 +192,79 @@ def test_device_hook_invalid_species_is_reported(tmp_path, capsys):
     finally:
         solver.close()
     assert "invalid species or site type" in capsys.readouterr().err
+
+
+# ---------------------------------------------------------------------------
+# The uint32 lattice (IntMpdRdmeSolver): the same trajectories with every
+# species id raised by OFF (past a byte), against the byte lattice's expected
+# records raised the same way.
+# ---------------------------------------------------------------------------
+OFF = 300
+
+
+def raised(a, off=OFF):
+    a = np.asarray(a).astype(np.uint32)
+    a[a > 0] += off
+    return a
+
+
+def int_spec():
+    from test_gpu_int_lattice import shifted
+
+    return shifted(small_spec(), OFF)
+
+
+def test_int_solver_run_simulation(tmp_path):
+    spec, ispec = small_spec(), int_spec()
+    path = str(tmp_path / "int.lm")
+    steps, wi, lwi = 23, 5, 7
+    ispec.to_file(path, {"seed": SEED, "maxTime": repr(steps * spec.tau), "writeInterval": wi,
+                         "latticeWriteInterval": lwi})
+    assert runSimulation(path, 3, solverClass=lmx.INT_SOLVER_NAME)
+    counts, frames, maxp = expected_plain(spec, 3, steps, wi, lwi)
+    with SimulationFile(path, writable=False) as f:
+        c = f.getSpeciesCounts(3)
+        assert c.shape[1] == spec.nS + OFF and not c[:, :OFF].any()
+        np.testing.assert_array_equal(c[:, OFF:], np.array([counts[t] for t in sorted(counts)]))
+        for k, t in enumerate(sorted(frames)):
+            fr = f.getLattice(3, k)
+            assert fr.dtype == np.uint32
+            np.testing.assert_array_equal(fr, raised(frames[t]), err_msg=f"frame at step {t}")
+        mp = f.getReplicateDataset(3, "MaxParticleCounts")
+        np.testing.assert_array_equal(mp[OFF + 1:], maxp[1:])
+        assert not mp[:OFF + 1].any()
+
+
+class IntEditingSolver(EditingHooks, lmx.IntMpdRdmeSolver):
+    species_offset = OFF
+
+
+class IntDeviceEditingSolver(DeviceEditingSolver, lmx.IntMpdRdmeSolver):
+    species_offset = OFF
+
+
+@pytest.mark.parametrize("device", [False, True], ids=["host-hooks", "device-hooks"])
+def test_int_solver_hooks(tmp_path, device):
+    if device:
+        need_torch_first()
+    spec, ispec = small_spec(), int_spec()
+    path = str(tmp_path / "ihook.lm")
+    ispec.to_file(path, {"seed": SEED, "maxTime": repr(HOOK_STEPS * spec.tau), "writeInterval": HOOK_WI,
+                         "latticeWriteInterval": HOOK_LWI, "hookInterval": HOOK_HI})
+    solver = IntDeviceEditingSolver(0) if device else IntEditingSolver(0)
+    try:
+        assert runSolver(path, 5, solver)
+        assert solver.calls == [0, 4, 8, 12, 16]
+        frames, counts, site_steps, final = expected_hooked(spec, 5)
+        with SimulationFile(path, writable=False) as f:
+            assert len(f.getLatticeTimes(5)) == len(frames)
+            for k, fr in enumerate(frames):
+                np.testing.assert_array_equal(f.getLattice(5, k), raised(fr), err_msg=f"frame {k}")
+            np.testing.assert_array_equal(f.getSpeciesCounts(5)[:, OFF:], counts)
+        # final: the byte lattice's native bytes in slot order per word plane
+        X, Y, Z = spec.shape
+        fb = final.reshape(spec.P // 4, Z, Y, X, 4).transpose(0, 4, 1, 2, 3).reshape(spec.P, Z, Y, X)
+        np.testing.assert_array_equal(np.asarray(solver.ended).reshape(spec.P, Z, Y, X).astype(np.uint32),
+                                      raised(fb))
+    finally:
+        solver.close()

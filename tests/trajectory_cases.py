@@ -46,9 +46,9 @@ class Edit:
     def __init__(self, x, y, z, species, site_type=None):
         self.x, self.y, self.z, self.s, self.st = x, y, z, species, site_type
 
-    def apply(self, view, sites):
+    def apply(self, view, sites, off=0):
         view[:, self.z, self.y, self.x, :] = 0
-        view[0, self.z, self.y, self.x, 0] = self.s
+        view[0, self.z, self.y, self.x, 0] = self.s + off
         if self.st is not None:
             sites[self.z, self.y, self.x] = self.st
 
@@ -66,11 +66,14 @@ HOOK_STEPS, HOOK_WI, HOOK_LWI, HOOK_HI = 18, 2, 4, 4
 
 
 class EditingHooks:
-    """Mixin overriding the solver's hooks with BEGIN_EDIT and PLAN."""
+    """Mixin overriding the solver's hooks with BEGIN_EDIT and PLAN (species
+    ids raised by ``species_offset``)."""
+
+    species_offset = 0
 
     def onBeginTrajectory(self, lattice):
         self.calls = []
-        BEGIN_EDIT.apply(lattice.getParticleLatticeView(), lattice.getSiteLatticeView())
+        BEGIN_EDIT.apply(lattice.getParticleLatticeView(), lattice.getSiteLatticeView(), self.species_offset)
         return 0
 
     def hookSimulation(self, time, lattice):
@@ -78,13 +81,13 @@ class EditingHooks:
         self.calls.append(step)
         edit, code, rate = PLAN[step]
         if edit is not None:
-            edit.apply(lattice.getParticleLatticeView(), lattice.getSiteLatticeView())
+            edit.apply(lattice.getParticleLatticeView(), lattice.getSiteLatticeView(), self.species_offset)
         if rate is not None:
             self.setReactionRate(0, rate)
         assert lattice.getXSize() == 16 and lattice.getMaxOccupancy() == 8
         assert lattice.getParticleCounts() == {int(k): int(v) for k, v in
-                                               enumerate(np.bincount(lattice.getParticleLatticeView().reshape(-1),
-                                                                     minlength=256)) if k and v}
+                                               enumerate(np.bincount(lattice.getParticleLatticeView().reshape(-1)))
+                                               if k and v}
         return code
 
     def onEndTrajectory(self, lattice):
