@@ -299,7 +299,8 @@ def main() -> int:
             traffic = json.load(open(tfile)).get(f"{spec.name}_{spec.shape[0]}x{spec.shape[1]}x{spec.shape[2]}:{dom}")
         except Exception:
             traffic = None
-    # Measured copy peak (16-byte grid-stride copy of 1 GiB, read + write bytes).
+    # Measured copy peak (1 GiB copy, one 16-byte load and store per thread;
+    # read + write bytes).
     copy_gbps = ctypes.c_double(0.0)
     lmx._lib.check(lmx.load_library().lmx_copy_bandwidth(0, 1 << 30, 20, ctypes.byref(copy_gbps)))
     roofline = {"bound": "hbm", "kernel": dom, "achieved": dk["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",

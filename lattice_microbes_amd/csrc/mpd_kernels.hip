@@ -331,19 +331,24 @@ hipError_t launch_validate(const Params &p, const uint32_t *lat, hipStream_t st)
     return hipGetLastError();
 }
 
-// STREAM copy: 16-byte loads and stores, grid-stride; the measured copy peak
-// that the sweeps' HBM rates are quoted against beside the 8 TB/s figure.
+// STREAM copy: one 16-byte load and store per thread, one pass (no
+// grid-stride loop: on gfx950 a flat grid of n/256 blocks measures ~6.2 TB/s
+// where a grid-stride loop over 8-64 blocks per CU stays near 4.5-4.8 TB/s);
+// the measured copy peak that the sweeps' HBM rates are quoted against beside
+// the 8 TB/s figure.
 __global__ __launch_bounds__(256) void copy_kernel(const uint4 *__restrict__ src, uint4 *__restrict__ dst, size_t n)
 {
-    const size_t stride = (size_t)gridDim.x * blockDim.x;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
 }
 
 hipError_t launch_copy(const void *src, void *dst, size_t bytes, int num_cu, hipStream_t st)
 {
+    (void)num_cu;
     const size_t n = bytes / sizeof(uint4);
-    const unsigned nb = (unsigned)std::min<size_t>((n + 255) / 256, (size_t)(num_cu > 0 ? num_cu : 256) * 32);
-    hipLaunchKernelGGL(copy_kernel, dim3(nb), dim3(256), 0, st, reinterpret_cast<const uint4 *>(src),
+    const size_t nb = (n + 255) / 256;
+    if (nb == 0 || nb > 0x7fffffffu) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(copy_kernel, dim3((unsigned)nb), dim3(256), 0, st, reinterpret_cast<const uint4 *>(src),
                        reinterpret_cast<uint4 *>(dst), n);
     return hipGetLastError();
 }
