@@ -125,3 +125,25 @@ def test_uint32_lattice_arguments(gpu):
     f = lmx.MESolverFactory()
     f.setSolver(lmx.INT_SOLVER_NAME)
     assert isinstance(f.instantiate(0), lmx.IntMpdRdmeSolver)
+
+
+def test_uint32_lattice_whole_cell_full_size(gpu):
+    """At the whole-cell size (256x256x512, P=8, 100 species renumbered past
+    a byte), the uint32 lattice ends on the byte lattice's state after the
+    same steps."""
+    spec = models.whole_cell(seed=1)
+    off = 300
+    b = lmx.MpdRdmeSolver(0)
+    spec.load_into(b, seed=3)
+    b.runTimesteps(12)
+    ref = b.getLatticeNative()
+    b.close()
+    w = lmx.IntMpdRdmeSolver(0)
+    shifted(spec, off).load_into(w, seed=3)
+    w.runTimesteps(12)
+    got = w.getLatticeNative().astype(np.int64)
+    w.close()
+    got[got > 0] -= off
+    P = spec.P
+    as_bytes = got.astype(np.uint8).reshape(P // 4, 4, *got.shape[1:]).transpose(0, 2, 3, 4, 1)
+    np.testing.assert_array_equal(np.ascontiguousarray(as_bytes).view(np.uint32)[..., 0], ref)
