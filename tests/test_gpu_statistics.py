@@ -131,3 +131,54 @@ def test_transient_mean_matches_next_subvolume_ensemble():
     f = c_mpd.var(ddof=1) / c_nsm.var(ddof=1)
     lo, hi = stats.f.ppf([0.005, 0.995], R_mpd - 1, R_nsm - 1)
     assert lo <= f <= hi, f"variance ratio {f:.3f} outside [{lo:.3f}, {hi:.3f}]"
+
+
+def test_tracked_particles_diffuse_as_mpd_random_walk(gpu):
+    """Every particle has its own species id (the uint32 lattice), so each
+    path is tracked.  Inside a periodic lattice of one site type a sweep moves
+    a particle by -1 or +1 along its axis with probability q each
+    (makeDiffusionChoices, byte_diffusion_1d_dev.cu:86-121), one sweep per
+    axis per step, so after K steps each axis displacement has mean 0 and
+    variance 2qK, independently per axis.  Nine checks (three means, three
+    variances, three correlations), each at 99.9% (normal for means and
+    correlations, chi-square for variances), so 99% jointly; a 10% error in q
+    fails one of them with probability ~0.9, a 10% directional bias always."""
+    from scipy import stats
+
+    from lattice_microbes_amd.models import ModelSpec
+    from helpers import SPACING
+
+    L3, n, q, K = 96, 3000, 0.2, 60
+    z999 = stats.norm.ppf(0.9995)
+    rng = np.random.default_rng(17)
+    grid = np.stack(np.meshgrid(*(np.arange(0, L3, 6),) * 3, indexing="ij"), -1).reshape(-1, 3)
+    start = grid[rng.choice(len(grid), n, replace=False)]
+    P = 8
+    lattice = np.zeros((L3, L3, L3, P), np.uint32)
+    lattice[start[:, 0], start[:, 1], start[:, 2], 0] = np.arange(1, n + 1, dtype=np.uint32)
+    s = ModelSpec("tracked", (L3, L3, L3), SPACING, TAU, P, n, 1, np.zeros(0, np.uint32), np.zeros(0),
+                  np.zeros((n, 0), np.int32), np.zeros((n, 0), np.uint32), q_to_D(np.full((1, 1, n), q)),
+                  np.zeros((0, 1), np.uint32), lattice, np.zeros((L3, L3, L3), np.uint8), "periodic")
+    w = lmx.IntMpdRdmeSolver(0)
+    try:
+        s.load_into(w, seed=29)
+        w.runTimesteps(K)
+        lat = w.getLattice()
+    finally:
+        w.close()
+    x, y, z, sl = np.nonzero(lat)
+    ids = lat[x, y, z, sl].astype(np.int64)
+    assert np.array_equal(np.sort(ids), np.arange(1, n + 1))   # every particle once
+    end = np.zeros((n, 3), np.int64)
+    end[ids - 1] = np.stack([x, y, z], -1)
+    d = (end - start + L3 // 2) % L3 - L3 // 2                  # periodic displacement
+    var = 2 * q * K
+    for a in range(3):
+        m = d[:, a].mean()
+        assert abs(m) < z999 * math.sqrt(var / n), (a, m)
+        s2 = d[:, a].var(ddof=1)
+        lo, hi = stats.chi2.ppf([0.0005, 0.9995], n - 1) * var / (n - 1)
+        assert lo < s2 < hi, (a, s2, lo, hi)
+    c = np.corrcoef(d.T)
+    for a, b in ((0, 1), (0, 2), (1, 2)):
+        assert abs(c[a, b]) < z999 / math.sqrt(n), (a, b, c[a, b])
