@@ -147,3 +147,16 @@ def test_uint32_lattice_whole_cell_full_size(gpu):
     P = spec.P
     as_bytes = got.astype(np.uint8).reshape(P // 4, 4, *got.shape[1:]).transpose(0, 2, 3, 4, 1)
     np.testing.assert_array_equal(np.ascontiguousarray(as_bytes).view(np.uint32)[..., 0], ref)
+
+
+def test_uint32_lattice_is_not_batched(gpu):
+    spec = models.random_model(shape=(8, 6, 5), P=8, nS=4, nST=2, R=3, density=0.2, seed=2)
+    w = lmx.IntMpdRdmeSolver(0)
+    spec.load_into(w, seed=1)
+    ens = lmx.Ensemble(0)
+    try:
+        with pytest.raises(InvalidArgException, match="byte lattice"):
+            ens.add(w)
+    finally:
+        ens.close()
+        w.close()
