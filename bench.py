@@ -186,6 +186,48 @@ def ensemble_bench(nrep: int, steps: int, seed: int):
     }
 
 
+def uint32_bench(spec, steps: int, seed: int, copy_gbps: float):
+    """The same model on the uint32 lattice (IntMpdRdmeSolver) with every
+    species id raised by 300 past a byte: per-step time and HBM rate of its
+    one-site-per-thread kernels (2*4P+1 B/site per sweep, 4P+1 B/site for the
+    reaction phase)."""
+    import lattice_microbes_amd as lmx
+    from lattice_microbes_amd.models import ModelSpec
+
+    off = 300
+    nS = spec.nS + off
+    S = np.zeros((nS, spec.R), np.int32)
+    S[off:] = spec.S
+    D = np.zeros((nS, spec.R), np.uint32)
+    D[off:] = spec.D
+    DF = np.zeros((spec.nST, spec.nST, nS))
+    DF[..., off:] = spec.DF
+    lat = spec.lattice.astype(np.uint32)
+    lat[lat > 0] += off
+    wspec = ModelSpec(name=spec.name, shape=spec.shape, spacing=spec.spacing, tau=spec.tau, P=spec.P, nS=nS,
+                      nST=spec.nST, types=spec.types, K=spec.K, S=S, D=D, DF=DF, RL=spec.RL, lattice=lat,
+                      sites=spec.sites, boundary=spec.boundary)
+    del lat
+    w = lmx.IntMpdRdmeSolver(0)
+    try:
+        wspec.load_into(w, replicate=0, seed=seed)
+        w.runTimesteps(3)
+        total_ms, per = w.timeSteps(steps, per_kernel=True)
+    finally:
+        w.close()
+    N, P = spec.N, spec.P
+    sweep_b, react_b = (8 * P + 1) * N, (4 * P + 1) * N
+    step_b = 3 * sweep_b + (react_b if spec.R else 0)
+    ms = total_ms / steps
+    gbps = step_b / (ms / 1e3) / 1e9
+    return {"workload": f"{spec.name} {spec.shape[0]}x{spec.shape[1]}x{spec.shape[2]}, species ids +{off} "
+                        f"({nS} species), 4 bytes per particle",
+            "steps": steps, "ms_per_step": ms, "site_steps_per_s": N * steps / (total_ms / 1e3),
+            "kernels_us": {k: v[0] / max(v[1], 1) * 1e3 for k, v in per.items()},
+            "alg_bytes_per_step": step_b, "achieved_GBps": gbps, "frac": gbps / HBM_PEAK_GBS,
+            "frac_of_copy_peak": gbps / copy_gbps if copy_gbps else None}
+
+
 def serving_bench(njobs: int, seed: int):
     """Continuous batching of a Poisson request stream (20 req/s) mixing the
     bimolecular 64^3, capsule 128x128x256 and whole-cell 256x256x128 (30
@@ -223,6 +265,7 @@ def main() -> int:
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--no-uint32", action="store_true", help="skip the uint32-lattice measurement")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--ensemble-replicates", type=int, default=256)
     ap.add_argument("--ensemble-steps", type=int, default=20)
@@ -340,6 +383,9 @@ def main() -> int:
         "final_particles": int(counts.sum()),
     }
     solver.close()
+    if not args.no_uint32 and args.path == "sweeps":
+        log("uint32 lattice")
+        out["uint32_lattice"] = uint32_bench(spec, 30, args.seed, copy_gbps.value)
     if not args.no_ensemble:
         log(f"ensemble: {args.ensemble_replicates} replicates")
         out["ensemble"] = ensemble_bench(args.ensemble_replicates, args.ensemble_steps, args.seed)
