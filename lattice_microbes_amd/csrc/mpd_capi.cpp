@@ -36,6 +36,22 @@ int hip_fail(hipError_t e, const char *what)
     return fail(LMX_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
 }
 
+// Entry points that allocate host memory run their body through guarded():
+// a C++ exception must not cross the C ABI.
+template <typename F>
+int guarded(F &&body)
+{
+    try {
+        return body();
+    } catch (const std::bad_alloc &) {
+        g_error = "host memory allocation failed";
+        return LMX_ERR_NOMEM;
+    } catch (const std::exception &e) {
+        g_error = std::string("internal error: ") + e.what();
+        return LMX_ERR_STATE;
+    }
+}
+
 #define LMX_HIP(call)                                  \
     do {                                               \
         hipError_t e_ = (call);                        \
@@ -446,28 +462,30 @@ int lmx_device_count(int *count)
 
 int lmx_create(int device, lmx_solver **out)
 {
-    if (!out) return fail(LMX_ERR_ARG, "null output handle");
-    *out = nullptr;
-    int n = 0;
-    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return fail(LMX_ERR_HIP, "no HIP device is available");
-    if (device < 0 || device >= n) return fail(LMX_ERR_ARG, "device index out of range");
-    lmx_solver *s = new lmx_solver();
-    s->device = device;
-    hipError_t e = hipSetDevice(device);
-    if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipMalloc((void **)&s->d_ctl, sizeof(lmx::Control));
-    if (e == hipSuccess) e = hipMemset(s->d_ctl, 0, sizeof(lmx::Control));
-    if (e == hipSuccess) e = hipMalloc((void **)&s->d_counts, 256 * sizeof(uint32_t));
-    int ncu = 0;
-    if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device);
-    if (ncu > 0) s->num_cu = (uint32_t)ncu;
-    if (e != hipSuccess) {
-        int rc = hip_fail(e, "lmx_create");
-        lmx_destroy(s);
-        return rc;
-    }
-    *out = s;
-    return LMX_OK;
+    return guarded([&]() -> int {
+        if (!out) return fail(LMX_ERR_ARG, "null output handle");
+        *out = nullptr;
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return fail(LMX_ERR_HIP, "no HIP device is available");
+        if (device < 0 || device >= n) return fail(LMX_ERR_ARG, "device index out of range");
+        lmx_solver *s = new lmx_solver();
+        s->device = device;
+        hipError_t e = hipSetDevice(device);
+        if (e == hipSuccess) e = hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipMalloc((void **)&s->d_ctl, sizeof(lmx::Control));
+        if (e == hipSuccess) e = hipMemset(s->d_ctl, 0, sizeof(lmx::Control));
+        if (e == hipSuccess) e = hipMalloc((void **)&s->d_counts, 256 * sizeof(uint32_t));
+        int ncu = 0;
+        if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, device);
+        if (ncu > 0) s->num_cu = (uint32_t)ncu;
+        if (e != hipSuccess) {
+            int rc = hip_fail(e, "lmx_create");
+            lmx_destroy(s);
+            return rc;
+        }
+        *out = s;
+        return LMX_OK;
+    });
 }
 
 int lmx_destroy(lmx_solver *s)
@@ -516,193 +534,199 @@ int lmx_get_seed(lmx_solver *s, uint32_t *seed32)
 int lmx_build_model(lmx_solver *s, uint32_t nS, uint32_t R, const uint32_t *initialCounts, const uint32_t *types,
                     const double *K, const int32_t *S, const uint32_t *D, uint32_t kCols)
 {
-    if (!s) return fail(LMX_ERR_ARG, "null solver");
-    if (!s->initialized) return fail(LMX_ERR_STATE, "initialize must be called before buildModel");
-    if (R > 0 && kCols == 0) return fail(LMX_ERR_ARG, "K: must have at least 1 column");
-    if (R > 16384) return fail(LMX_ERR_ARG, "The number of reaction table entries exceeds the maximum supported by the solver (16384).");
-    if ((R > 0 || nS > 0) && (!types || !K || !S || !D) && R > 0) return fail(LMX_ERR_ARG, "null model array");
-    s->nS = nS;
-    s->R = R;
-    s->initialCounts.assign(initialCounts ? initialCounts : nullptr, initialCounts ? initialCounts + nS : nullptr);
-    s->types.assign(types, types + R);
-    s->k.assign(R, 0.0);
-    s->si.assign(R, 0);
-    s->s2i.assign(R, 0);
-    s->order.assign(R, 0);
-    s->D1.assign(R, 0);
-    s->D2.assign(R, 0);
-    s->S_sm.assign(S, S + (size_t)nS * R);
-    s->S_rm.assign((size_t)nS * R, 0);
-    for (uint32_t i = 0; i < R; i++) {
-        uint32_t deps[2] = {0, 0}, nd = 0;
-        for (uint32_t j = 0; j < nS; j++)
-            if (D[(size_t)j * R + i] == 1) {
-                if (nd < 2) deps[nd] = j;
-                nd++;
+    return guarded([&]() -> int {
+        if (!s) return fail(LMX_ERR_ARG, "null solver");
+        if (!s->initialized) return fail(LMX_ERR_STATE, "initialize must be called before buildModel");
+        if (R > 0 && kCols == 0) return fail(LMX_ERR_ARG, "K: must have at least 1 column");
+        if (R > 16384) return fail(LMX_ERR_ARG, "The number of reaction table entries exceeds the maximum supported by the solver (16384).");
+        if ((R > 0 || nS > 0) && (!types || !K || !S || !D) && R > 0) return fail(LMX_ERR_ARG, "null model array");
+        s->nS = nS;
+        s->R = R;
+        s->initialCounts.assign(initialCounts ? initialCounts : nullptr, initialCounts ? initialCounts + nS : nullptr);
+        s->types.assign(types, types + R);
+        s->k.assign(R, 0.0);
+        s->si.assign(R, 0);
+        s->s2i.assign(R, 0);
+        s->order.assign(R, 0);
+        s->D1.assign(R, 0);
+        s->D2.assign(R, 0);
+        s->S_sm.assign(S, S + (size_t)nS * R);
+        s->S_rm.assign((size_t)nS * R, 0);
+        for (uint32_t i = 0; i < R; i++) {
+            uint32_t deps[2] = {0, 0}, nd = 0;
+            for (uint32_t j = 0; j < nS; j++)
+                if (D[(size_t)j * R + i] == 1) {
+                    if (nd < 2) deps[nd] = j;
+                    nd++;
+                }
+            s->k[i] = K[(size_t)i * kCols];
+            switch (types[i]) {
+            case 0:
+                if (nd != 0) return fail(LMX_ERR_ARG, "D: zeroth order reaction cannot have any dependencies");
+                s->order[i] = lmx::kZeroOrder;
+                break;
+            case 1:
+                if (nd != 1) return fail(LMX_ERR_ARG, "D: first order reaction had invalid number of dependencies");
+                s->order[i] = lmx::kFirstOrder;
+                s->si[i] = deps[0];
+                s->D1[i] = deps[0] + 1;
+                break;
+            case 2:
+                if (nd != 2) return fail(LMX_ERR_ARG, "D: second order reaction had invalid number of dependencies");
+                s->order[i] = lmx::kSecondOrder;
+                s->si[i] = deps[0];
+                s->s2i[i] = deps[1];
+                s->D1[i] = deps[0] + 1;
+                s->D2[i] = deps[1] + 1;
+                break;
+            case 3:
+                if (nd != 1) return fail(LMX_ERR_ARG, "D: second order self reaction had invalid number of dependencies");
+                s->order[i] = lmx::kSecondOrderSelf;
+                s->si[i] = deps[0];
+                s->D1[i] = deps[0] + 1;
+                break;
+            default:
+                return fail(LMX_ERR_ARG, "reactionTypeA: the reaction type was not supported by the solver");
             }
-        s->k[i] = K[(size_t)i * kCols];
-        switch (types[i]) {
-        case 0:
-            if (nd != 0) return fail(LMX_ERR_ARG, "D: zeroth order reaction cannot have any dependencies");
-            s->order[i] = lmx::kZeroOrder;
-            break;
-        case 1:
-            if (nd != 1) return fail(LMX_ERR_ARG, "D: first order reaction had invalid number of dependencies");
-            s->order[i] = lmx::kFirstOrder;
-            s->si[i] = deps[0];
-            s->D1[i] = deps[0] + 1;
-            break;
-        case 2:
-            if (nd != 2) return fail(LMX_ERR_ARG, "D: second order reaction had invalid number of dependencies");
-            s->order[i] = lmx::kSecondOrder;
-            s->si[i] = deps[0];
-            s->s2i[i] = deps[1];
-            s->D1[i] = deps[0] + 1;
-            s->D2[i] = deps[1] + 1;
-            break;
-        case 3:
-            if (nd != 1) return fail(LMX_ERR_ARG, "D: second order self reaction had invalid number of dependencies");
-            s->order[i] = lmx::kSecondOrderSelf;
-            s->si[i] = deps[0];
-            s->D1[i] = deps[0] + 1;
-            break;
-        default:
-            return fail(LMX_ERR_ARG, "reactionTypeA: the reaction type was not supported by the solver");
         }
-    }
-    // S transposed to reaction-major int8 (MpdRdmeSolver.cu:317-325).
-    for (uint32_t rx = 0; rx < R; rx++)
-        for (uint32_t p = 0; p < nS; p++) s->S_rm[(size_t)rx * nS + p] = (int8_t)S[(size_t)R * p + rx];
-    s->model_built = true;
-    s->diffusion_built = false;
-    return LMX_OK;
+        // S transposed to reaction-major int8 (MpdRdmeSolver.cu:317-325).
+        for (uint32_t rx = 0; rx < R; rx++)
+            for (uint32_t p = 0; p < nS; p++) s->S_rm[(size_t)rx * nS + p] = (int8_t)S[(size_t)R * p + rx];
+        s->model_built = true;
+        s->diffusion_built = false;
+        return LMX_OK;
+    });
 }
 
 int lmx_build_diffusion_model(lmx_solver *s, uint32_t nST, const double *DF, const uint32_t *RL, uint32_t X,
                               uint32_t Y, uint32_t Z, uint32_t P, uint32_t bytesPerParticle, double lambda,
                               const uint8_t *lattice, const uint8_t *sites, int rowMajor)
 {
-    if (!s) return fail(LMX_ERR_ARG, "null solver");
-    if (!s->model_built) return fail(LMX_ERR_STATE, "buildModel must be called before buildDiffusionModel");
-    if (!rowMajor) return fail(LMX_ERR_ARG, "rowMajorData: columns major lattice data is not currently supported");
-    if (bytesPerParticle != 1 && bytesPerParticle != 4)
-        return fail(LMX_ERR_ARG, "bytes_per_particle must be 1 (byte lattice) or 4 (uint32 lattice)");
-    if (bytesPerParticle == 1 && s->nS > 255) return fail(LMX_ERR_ARG, "a byte lattice holds at most 255 species");
-    if ((unsigned long long)nST * s->nS * s->nS >= (1ull << 32))
-        return fail(LMX_ERR_ARG, "numberSiteTypes * numberSpecies^2 must be below 2^32 (second-order propensity table)");
-    if (P != 4 && P != 8 && P != 16) return fail(LMX_ERR_ARG, "particlesPerSite must be 4, 8 or 16");
-    if (nST == 0 || nST > 256) return fail(LMX_ERR_ARG, "numberSiteTypes must be in 1..256");
-    if (X == 0 || Y == 0 || Z == 0) return fail(LMX_ERR_ARG, "incorrect lattice dimensions");
-    const unsigned long long N = (unsigned long long)X * Y * Z;
-    if (N >= (1ull << 32)) return fail(LMX_ERR_ARG, "lattice has 2^32 or more sites");
-    if (!DF || !lattice || !sites || (s->R > 0 && !RL)) return fail(LMX_ERR_ARG, "null model array");
-    int rc = select_device(s);
-    if (rc) return rc;
-    LMX_HIP(hipStreamSynchronize(s->stream));
+    return guarded([&]() -> int {
+        if (!s) return fail(LMX_ERR_ARG, "null solver");
+        if (!s->model_built) return fail(LMX_ERR_STATE, "buildModel must be called before buildDiffusionModel");
+        if (!rowMajor) return fail(LMX_ERR_ARG, "rowMajorData: columns major lattice data is not currently supported");
+        if (bytesPerParticle != 1 && bytesPerParticle != 4)
+            return fail(LMX_ERR_ARG, "bytes_per_particle must be 1 (byte lattice) or 4 (uint32 lattice)");
+        if (bytesPerParticle == 1 && s->nS > 255) return fail(LMX_ERR_ARG, "a byte lattice holds at most 255 species");
+        if ((unsigned long long)nST * s->nS * s->nS >= (1ull << 32))
+            return fail(LMX_ERR_ARG, "numberSiteTypes * numberSpecies^2 must be below 2^32 (second-order propensity table)");
+        if (P != 4 && P != 8 && P != 16) return fail(LMX_ERR_ARG, "particlesPerSite must be 4, 8 or 16");
+        if (nST == 0 || nST > 256) return fail(LMX_ERR_ARG, "numberSiteTypes must be in 1..256");
+        if (X == 0 || Y == 0 || Z == 0) return fail(LMX_ERR_ARG, "incorrect lattice dimensions");
+        const unsigned long long N = (unsigned long long)X * Y * Z;
+        if (N >= (1ull << 32)) return fail(LMX_ERR_ARG, "lattice has 2^32 or more sites");
+        if (!DF || !lattice || !sites || (s->R > 0 && !RL)) return fail(LMX_ERR_ARG, "null model array");
+        int rc = select_device(s);
+        if (rc) return rc;
+        LMX_HIP(hipStreamSynchronize(s->stream));
 
-    s->nST = nST;
-    s->X = X; s->Y = Y; s->Z = Z; s->P = P;
-    s->bpp = bytesPerParticle;
-    s->W = bytesPerParticle == 4 ? P : P / 4;
-    s->N = (size_t)N;
-    s->lambda = lambda;
-    const size_t nT = (size_t)s->nS * nST * nST;
-    s->DF.assign(DF, DF + nT);
-    s->RL.assign(RL ? RL : nullptr, RL ? RL + (size_t)s->R * nST : nullptr);
-    // Transition probabilities (MpdRdmeSolver.cu:408-421).
-    s->T.assign(nT, 0.0f);
-    for (size_t i = 0; i < nT; i++) {
-        const float q = (float)(DF[i] * s->tau / pow(lambda, 2));
-        if (q > 0.50f) return fail(LMX_ERR_ARG, "D: The specified diffusion coefficient is too high for the diffusion model.");
-        s->T[i] = q;
-    }
-    std::vector<uint32_t> native;
-    if ((rc = row_major_to_native(s, lattice, native))) return rc;
-    std::vector<uint8_t> nsites;
-    if ((rc = sites_row_major_to_native(s, sites, nsites))) return rc;
-    compute_propensities(s);
-
-    free_device(s);
-    const size_t lat_bytes = s->N * s->W * sizeof(uint32_t);
-    hipError_t e = hipMalloc((void **)&s->d_lat[0], lat_bytes);
-    if (e == hipSuccess) e = hipMalloc((void **)&s->d_lat[1], lat_bytes);
-    if (e == hipSuccess) e = hipMalloc((void **)&s->d_sites, s->N);
-    if (e == hipSuccess && s->nS > s->counts_cap) {
-        hipFree(s->d_counts);
-        s->d_counts = nullptr;
-        e = hipMalloc((void **)&s->d_counts, (size_t)s->nS * sizeof(uint32_t));
-        s->counts_cap = e == hipSuccess ? s->nS : 0u;
-    }
-    // The reaction list: kListShards shards of cap entries (index, W words,
-    // type byte).  Any capacity is correct (an overflowing shard makes the
-    // reaction phase a full pass); N/8 entries in total hold every occupied
-    // site up to 12.5% occupancy.
-    const size_t rcap = std::max<size_t>(64, (s->N / 8 + lmx::kListShards - 1) / lmx::kListShards);
-    const size_t rtotal = rcap * lmx::kListShards;
-    // (the byte lattice's vector kernels only)
-    if (e == hipSuccess && s->bpp == 1) e = hipMalloc((void **)&s->d_rl, rtotal * (1 + s->W) * sizeof(uint32_t) + rtotal);
-    if (e == hipSuccess && s->d_rl) {
-        s->rlist.idx = s->d_rl;
-        s->rlist.words = s->d_rl + rtotal;
-        s->rlist.type = reinterpret_cast<uint8_t *>(s->d_rl + rtotal * (1 + s->W));
-        s->rlist.cap = (uint32_t)rcap;
-    }
-    if (e != hipSuccess) {
-        free_device(s);
-        return fail(LMX_ERR_NOMEM, std::string("Failed to allocate the device lattice: ") + hipGetErrorString(e));
-    }
-    s->cur = 0;
-    LMX_HIP(hipMemcpy(s->d_lat[0], native.data(), lat_bytes, hipMemcpyHostToDevice));
-    LMX_HIP(hipMemset(s->d_lat[1], 0, lat_bytes));
-    LMX_HIP(hipMemcpy(s->d_sites, nsites.data(), s->N, hipMemcpyHostToDevice));
-    if ((rc = dev_upload(&s->d_T, s->T))) return rc;
-    if ((rc = dev_upload(&s->d_order, s->order))) return rc;
-    if ((rc = dev_upload(&s->d_D1, s->D1))) return rc;
-    if ((rc = dev_upload(&s->d_D2, s->D2))) return rc;
-    if ((rc = dev_upload(&s->d_S, s->S_rm))) return rc;
-    std::vector<uint8_t> rl8(s->RL.size());
-    for (size_t i = 0; i < rl8.size(); i++) rl8[i] = (uint8_t)s->RL[i];  // tmpRL (MpdRdmeSolver.cu:424-428)
-    if ((rc = dev_upload(&s->d_RL, rl8))) return rc;
-    // Reactions listed by first reactant (zero-order ones under nS), and the
-    // nonzero stoichiometry of each reaction, both in ascending order.
-    {
-        std::vector<uint32_t> rx_off(s->nS + 2, 0), rx_list(s->R), sto_off(s->R + 1, 0), sto_sp;
-        std::vector<int32_t> sto_val;
-        auto bucket = [&](uint32_t r) { return s->order[r] == lmx::kZeroOrder ? s->nS : s->D1[r] - 1; };
-        for (uint32_t r = 0; r < s->R; r++) rx_off[bucket(r) + 1]++;
-        for (uint32_t b = 0; b <= s->nS; b++) rx_off[b + 1] += rx_off[b];
-        std::vector<uint32_t> fill(rx_off.begin(), rx_off.end() - 1);
-        for (uint32_t r = 0; r < s->R; r++) rx_list[fill[bucket(r)]++] = r;
-        for (uint32_t r = 0; r < s->R; r++) {
-            for (uint32_t sp = 0; sp < s->nS; sp++) {
-                const int8_t v = s->S_rm[(size_t)r * s->nS + sp];
-                if (v != 0) {
-                    sto_sp.push_back(sp);
-                    sto_val.push_back(v);
-                }
-            }
-            sto_off[r + 1] = (uint32_t)sto_sp.size();
+        s->nST = nST;
+        s->X = X; s->Y = Y; s->Z = Z; s->P = P;
+        s->bpp = bytesPerParticle;
+        s->W = bytesPerParticle == 4 ? P : P / 4;
+        s->N = (size_t)N;
+        s->lambda = lambda;
+        const size_t nT = (size_t)s->nS * nST * nST;
+        s->DF.assign(DF, DF + nT);
+        s->RL.assign(RL ? RL : nullptr, RL ? RL + (size_t)s->R * nST : nullptr);
+        // Transition probabilities (MpdRdmeSolver.cu:408-421).
+        s->T.assign(nT, 0.0f);
+        for (size_t i = 0; i < nT; i++) {
+            const float q = (float)(DF[i] * s->tau / pow(lambda, 2));
+            if (q > 0.50f) return fail(LMX_ERR_ARG, "D: The specified diffusion coefficient is too high for the diffusion model.");
+            s->T[i] = q;
         }
-        if ((rc = dev_upload(&s->d_rx_off, rx_off))) return rc;
-        if ((rc = dev_upload(&s->d_rx_list, rx_list))) return rc;
-        if ((rc = dev_upload(&s->d_sto_off, sto_off))) return rc;
-        if ((rc = dev_upload(&s->d_sto_sp, sto_sp))) return rc;
-        if ((rc = dev_upload(&s->d_sto_val, sto_val))) return rc;
-    }
-    if ((rc = upload_tables(s))) return rc;
-    LMX_HIP(hipMemset(s->d_ctl, 0, sizeof(lmx::Control)));
-    s->timestep = 0;
-    s->diffusion_built = true;
-    return LMX_OK;
+        std::vector<uint32_t> native;
+        if ((rc = row_major_to_native(s, lattice, native))) return rc;
+        std::vector<uint8_t> nsites;
+        if ((rc = sites_row_major_to_native(s, sites, nsites))) return rc;
+        compute_propensities(s);
+
+        free_device(s);
+        const size_t lat_bytes = s->N * s->W * sizeof(uint32_t);
+        hipError_t e = hipMalloc((void **)&s->d_lat[0], lat_bytes);
+        if (e == hipSuccess) e = hipMalloc((void **)&s->d_lat[1], lat_bytes);
+        if (e == hipSuccess) e = hipMalloc((void **)&s->d_sites, s->N);
+        if (e == hipSuccess && s->nS > s->counts_cap) {
+            hipFree(s->d_counts);
+            s->d_counts = nullptr;
+            e = hipMalloc((void **)&s->d_counts, (size_t)s->nS * sizeof(uint32_t));
+            s->counts_cap = e == hipSuccess ? s->nS : 0u;
+        }
+        // The reaction list: kListShards shards of cap entries (index, W words,
+        // type byte).  Any capacity is correct (an overflowing shard makes the
+        // reaction phase a full pass); N/8 entries in total hold every occupied
+        // site up to 12.5% occupancy.
+        const size_t rcap = std::max<size_t>(64, (s->N / 8 + lmx::kListShards - 1) / lmx::kListShards);
+        const size_t rtotal = rcap * lmx::kListShards;
+        // (the byte lattice's vector kernels only)
+        if (e == hipSuccess && s->bpp == 1) e = hipMalloc((void **)&s->d_rl, rtotal * (1 + s->W) * sizeof(uint32_t) + rtotal);
+        if (e == hipSuccess && s->d_rl) {
+            s->rlist.idx = s->d_rl;
+            s->rlist.words = s->d_rl + rtotal;
+            s->rlist.type = reinterpret_cast<uint8_t *>(s->d_rl + rtotal * (1 + s->W));
+            s->rlist.cap = (uint32_t)rcap;
+        }
+        if (e != hipSuccess) {
+            free_device(s);
+            return fail(LMX_ERR_NOMEM, std::string("Failed to allocate the device lattice: ") + hipGetErrorString(e));
+        }
+        s->cur = 0;
+        LMX_HIP(hipMemcpy(s->d_lat[0], native.data(), lat_bytes, hipMemcpyHostToDevice));
+        LMX_HIP(hipMemset(s->d_lat[1], 0, lat_bytes));
+        LMX_HIP(hipMemcpy(s->d_sites, nsites.data(), s->N, hipMemcpyHostToDevice));
+        if ((rc = dev_upload(&s->d_T, s->T))) return rc;
+        if ((rc = dev_upload(&s->d_order, s->order))) return rc;
+        if ((rc = dev_upload(&s->d_D1, s->D1))) return rc;
+        if ((rc = dev_upload(&s->d_D2, s->D2))) return rc;
+        if ((rc = dev_upload(&s->d_S, s->S_rm))) return rc;
+        std::vector<uint8_t> rl8(s->RL.size());
+        for (size_t i = 0; i < rl8.size(); i++) rl8[i] = (uint8_t)s->RL[i];  // tmpRL (MpdRdmeSolver.cu:424-428)
+        if ((rc = dev_upload(&s->d_RL, rl8))) return rc;
+        // Reactions listed by first reactant (zero-order ones under nS), and the
+        // nonzero stoichiometry of each reaction, both in ascending order.
+        {
+            std::vector<uint32_t> rx_off(s->nS + 2, 0), rx_list(s->R), sto_off(s->R + 1, 0), sto_sp;
+            std::vector<int32_t> sto_val;
+            auto bucket = [&](uint32_t r) { return s->order[r] == lmx::kZeroOrder ? s->nS : s->D1[r] - 1; };
+            for (uint32_t r = 0; r < s->R; r++) rx_off[bucket(r) + 1]++;
+            for (uint32_t b = 0; b <= s->nS; b++) rx_off[b + 1] += rx_off[b];
+            std::vector<uint32_t> fill(rx_off.begin(), rx_off.end() - 1);
+            for (uint32_t r = 0; r < s->R; r++) rx_list[fill[bucket(r)]++] = r;
+            for (uint32_t r = 0; r < s->R; r++) {
+                for (uint32_t sp = 0; sp < s->nS; sp++) {
+                    const int8_t v = s->S_rm[(size_t)r * s->nS + sp];
+                    if (v != 0) {
+                        sto_sp.push_back(sp);
+                        sto_val.push_back(v);
+                    }
+                }
+                sto_off[r + 1] = (uint32_t)sto_sp.size();
+            }
+            if ((rc = dev_upload(&s->d_rx_off, rx_off))) return rc;
+            if ((rc = dev_upload(&s->d_rx_list, rx_list))) return rc;
+            if ((rc = dev_upload(&s->d_sto_off, sto_off))) return rc;
+            if ((rc = dev_upload(&s->d_sto_sp, sto_sp))) return rc;
+            if ((rc = dev_upload(&s->d_sto_val, sto_val))) return rc;
+        }
+        if ((rc = upload_tables(s))) return rc;
+        LMX_HIP(hipMemset(s->d_ctl, 0, sizeof(lmx::Control)));
+        s->timestep = 0;
+        s->diffusion_built = true;
+        return LMX_OK;
+    });
 }
 
 int lmx_set_reaction_rate(lmx_solver *s, uint32_t rxid, float rate)
 {
-    if (!s) return fail(LMX_ERR_ARG, "null solver");
-    if (!s->model_built || rxid >= s->R) return fail(LMX_ERR_ARG, "reaction index is out of bounds");
-    s->k[rxid] = rate;
-    if (s->diffusion_built) compute_propensities(s);
-    return LMX_OK;
+    return guarded([&]() -> int {
+        if (!s) return fail(LMX_ERR_ARG, "null solver");
+        if (!s->model_built || rxid >= s->R) return fail(LMX_ERR_ARG, "reaction index is out of bounds");
+        s->k[rxid] = rate;
+        if (s->diffusion_built) compute_propensities(s);
+        return LMX_OK;
+    });
 }
 
 int lmx_set_path(lmx_solver *s, int path)
@@ -794,29 +818,33 @@ int lmx_get_lattice_native(lmx_solver *s, uint32_t *native, size_t words)
 
 int lmx_get_lattice(lmx_solver *s, uint8_t *rm, size_t size)
 {
-    int rc = require_ready(s);
-    if (rc) return rc;
-    if (!rm || size != s->N * s->P * s->bpp) return fail(LMX_ERR_ARG, "bufferSize: the buffer size was not equal to the lattice data size");
-    std::vector<uint32_t> native(s->N * s->W);
-    if ((rc = lmx_get_lattice_native(s, native.data(), native.size()))) return rc;
-    native_to_row_major(s, native.data(), rm);
-    return LMX_OK;
+    return guarded([&]() -> int {
+        int rc = require_ready(s);
+        if (rc) return rc;
+        if (!rm || size != s->N * s->P * s->bpp) return fail(LMX_ERR_ARG, "bufferSize: the buffer size was not equal to the lattice data size");
+        std::vector<uint32_t> native(s->N * s->W);
+        if ((rc = lmx_get_lattice_native(s, native.data(), native.size()))) return rc;
+        native_to_row_major(s, native.data(), rm);
+        return LMX_OK;
+    });
 }
 
 int lmx_get_sites(lmx_solver *s, uint8_t *rm, size_t size)
 {
-    int rc = require_ready(s);
-    if (rc) return rc;
-    if (!rm || size != s->N) return fail(LMX_ERR_ARG, "bufferSize: the buffer size was not equal to the lattice site size");
-    std::vector<uint8_t> native(s->N);
-    LMX_HIP(hipStreamSynchronize(s->stream));
-    LMX_HIP(hipMemcpy(native.data(), s->d_sites, s->N, hipMemcpyDeviceToHost));
-    size_t idx = 0;
-    for (uint32_t x = 0; x < s->X; x++)
-        for (uint32_t y = 0; y < s->Y; y++)
-            for (uint32_t z = 0; z < s->Z; z++, idx++)
-                rm[idx] = native[(size_t)x + (size_t)s->X * y + (size_t)s->X * s->Y * z];
-    return LMX_OK;
+    return guarded([&]() -> int {
+        int rc = require_ready(s);
+        if (rc) return rc;
+        if (!rm || size != s->N) return fail(LMX_ERR_ARG, "bufferSize: the buffer size was not equal to the lattice site size");
+        std::vector<uint8_t> native(s->N);
+        LMX_HIP(hipStreamSynchronize(s->stream));
+        LMX_HIP(hipMemcpy(native.data(), s->d_sites, s->N, hipMemcpyDeviceToHost));
+        size_t idx = 0;
+        for (uint32_t x = 0; x < s->X; x++)
+            for (uint32_t y = 0; y < s->Y; y++)
+                for (uint32_t z = 0; z < s->Z; z++, idx++)
+                    rm[idx] = native[(size_t)x + (size_t)s->X * y + (size_t)s->X * s->Y * z];
+        return LMX_OK;
+    });
 }
 
 int lmx_set_lattice_native(lmx_solver *s, const uint32_t *native, size_t words)
@@ -863,41 +891,47 @@ int lmx_validate_lattice(lmx_solver *s)
 
 int lmx_set_lattice(lmx_solver *s, const uint8_t *rm, size_t size)
 {
-    int rc = require_ready(s);
-    if (rc) return rc;
-    if (!rm || size != s->N * s->P * s->bpp) return fail(LMX_ERR_ARG, "bufferSize: the buffer size was not equal to the lattice data size");
-    std::vector<uint32_t> native;
-    if ((rc = row_major_to_native(s, rm, native))) return rc;
-    LMX_HIP(hipStreamSynchronize(s->stream));
-    LMX_HIP(hipMemcpy(s->d_lat[s->cur], native.data(), native.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
-    return LMX_OK;
+    return guarded([&]() -> int {
+        int rc = require_ready(s);
+        if (rc) return rc;
+        if (!rm || size != s->N * s->P * s->bpp) return fail(LMX_ERR_ARG, "bufferSize: the buffer size was not equal to the lattice data size");
+        std::vector<uint32_t> native;
+        if ((rc = row_major_to_native(s, rm, native))) return rc;
+        LMX_HIP(hipStreamSynchronize(s->stream));
+        LMX_HIP(hipMemcpy(s->d_lat[s->cur], native.data(), native.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+        return LMX_OK;
+    });
 }
 
 int lmx_set_sites(lmx_solver *s, const uint8_t *rm, size_t size)
 {
-    int rc = require_ready(s);
-    if (rc) return rc;
-    if (!rm || size != s->N) return fail(LMX_ERR_ARG, "bufferSize: the buffer size was not equal to the lattice site size");
-    std::vector<uint8_t> native;
-    if ((rc = sites_row_major_to_native(s, rm, native))) return rc;
-    LMX_HIP(hipStreamSynchronize(s->stream));
-    LMX_HIP(hipMemcpy(s->d_sites, native.data(), s->N, hipMemcpyHostToDevice));
-    return LMX_OK;
+    return guarded([&]() -> int {
+        int rc = require_ready(s);
+        if (rc) return rc;
+        if (!rm || size != s->N) return fail(LMX_ERR_ARG, "bufferSize: the buffer size was not equal to the lattice site size");
+        std::vector<uint8_t> native;
+        if ((rc = sites_row_major_to_native(s, rm, native))) return rc;
+        LMX_HIP(hipStreamSynchronize(s->stream));
+        LMX_HIP(hipMemcpy(s->d_sites, native.data(), s->N, hipMemcpyHostToDevice));
+        return LMX_OK;
+    });
 }
 
 int lmx_get_species_counts(lmx_solver *s, uint32_t *counts, uint32_t n)
 {
-    int rc = require_ready(s);
-    if (rc) return rc;
-    if (!counts || n < s->nS) return fail(LMX_ERR_ARG, "counts buffer smaller than the number of species");
-    LMX_HIP(hipMemsetAsync(s->d_counts, 0, s->counts_cap * sizeof(uint32_t), s->stream));
-    hipError_t e = lmx::launch_counts(s->d_lat[s->cur], s->N * s->W, s->nS, s->bpp == 4, s->d_counts, s->stream);
-    if (e != hipSuccess) return hip_fail(e, "count launch");
-    LMX_HIP(hipStreamSynchronize(s->stream));
-    std::vector<uint32_t> h(s->counts_cap);
-    LMX_HIP(hipMemcpy(h.data(), s->d_counts, s->counts_cap * sizeof(uint32_t), hipMemcpyDeviceToHost));
-    for (uint32_t i = 0; i < n; i++) counts[i] = i < s->nS ? h[i] : 0u;
-    return LMX_OK;
+    return guarded([&]() -> int {
+        int rc = require_ready(s);
+        if (rc) return rc;
+        if (!counts || n < s->nS) return fail(LMX_ERR_ARG, "counts buffer smaller than the number of species");
+        LMX_HIP(hipMemsetAsync(s->d_counts, 0, s->counts_cap * sizeof(uint32_t), s->stream));
+        hipError_t e = lmx::launch_counts(s->d_lat[s->cur], s->N * s->W, s->nS, s->bpp == 4, s->d_counts, s->stream);
+        if (e != hipSuccess) return hip_fail(e, "count launch");
+        LMX_HIP(hipStreamSynchronize(s->stream));
+        std::vector<uint32_t> h(s->counts_cap);
+        LMX_HIP(hipMemcpy(h.data(), s->d_counts, s->counts_cap * sizeof(uint32_t), hipMemcpyDeviceToHost));
+        for (uint32_t i = 0; i < n; i++) counts[i] = i < s->nS ? h[i] : 0u;
+        return LMX_OK;
+    });
 }
 
 int lmx_get_overflow_stats(lmx_solver *s, uint64_t *entries, uint64_t *steps)
@@ -916,86 +950,90 @@ int lmx_get_overflow_stats(lmx_solver *s, uint64_t *entries, uint64_t *steps)
 int lmx_time_steps(lmx_solver *s, uint32_t nsteps, double *total_ms, double *kernel_ms, uint64_t *kernel_launches,
                    uint32_t nkernels)
 {
-    int rc = require_ready(s);
-    if (rc) return rc;
-    if (s->tables_dirty) {
-        LMX_HIP(hipStreamSynchronize(s->stream));
-        if ((rc = upload_tables(s))) return rc;
-    }
-    const lmx::Params p = make_params(s);
-    if (kernel_ms)
-        for (uint32_t k = 0; k < nkernels; k++) kernel_ms[k] = 0.0;
-    if (kernel_launches)
-        for (uint32_t k = 0; k < nkernels; k++) kernel_launches[k] = 0;
-    hipEvent_t t0, t1;
-    LMX_HIP(hipEventCreate(&t0));
-    LMX_HIP(hipEventCreate(&t1));
-    const int kMaxLaunches = 8;
-    const uint32_t chunk = 256;
-    std::vector<hipEvent_t> ev;
-    std::vector<int> kinds;
-    if (kernel_ms) {
-        ev.resize((size_t)chunk * (kMaxLaunches + 1));
-        for (auto &e : ev) LMX_HIP(hipEventCreate(&e));
-        kinds.resize((size_t)chunk * kMaxLaunches);
-    }
-    LMX_HIP(hipStreamSynchronize(s->stream));
-    LMX_HIP(hipEventRecord(t0, s->stream));
-    for (uint32_t base = 0; base < nsteps; base += chunk) {
-        const uint32_t m = (nsteps - base) < chunk ? (nsteps - base) : chunk;
-        std::vector<int> nl(m, 0);
-        for (uint32_t i = 0; i < m; i++) {
-            if (kernel_ms)
-                rc = enqueue_step(s, p, &ev[(size_t)i * (kMaxLaunches + 1)], &kinds[(size_t)i * kMaxLaunches], &nl[i]);
-            else
-                rc = enqueue_step(s, p, nullptr, nullptr, nullptr);
-            if (rc) return rc;
+    return guarded([&]() -> int {
+        int rc = require_ready(s);
+        if (rc) return rc;
+        if (s->tables_dirty) {
+            LMX_HIP(hipStreamSynchronize(s->stream));
+            if ((rc = upload_tables(s))) return rc;
         }
+        const lmx::Params p = make_params(s);
+        if (kernel_ms)
+            for (uint32_t k = 0; k < nkernels; k++) kernel_ms[k] = 0.0;
+        if (kernel_launches)
+            for (uint32_t k = 0; k < nkernels; k++) kernel_launches[k] = 0;
+        hipEvent_t t0, t1;
+        LMX_HIP(hipEventCreate(&t0));
+        LMX_HIP(hipEventCreate(&t1));
+        const int kMaxLaunches = 8;
+        const uint32_t chunk = 256;
+        std::vector<hipEvent_t> ev;
+        std::vector<int> kinds;
         if (kernel_ms) {
-            LMX_HIP(hipEventSynchronize(ev[(size_t)(m - 1) * (kMaxLaunches + 1) + nl[m - 1]]));
+            ev.resize((size_t)chunk * (kMaxLaunches + 1));
+            for (auto &e : ev) LMX_HIP(hipEventCreate(&e));
+            kinds.resize((size_t)chunk * kMaxLaunches);
+        }
+        LMX_HIP(hipStreamSynchronize(s->stream));
+        LMX_HIP(hipEventRecord(t0, s->stream));
+        for (uint32_t base = 0; base < nsteps; base += chunk) {
+            const uint32_t m = (nsteps - base) < chunk ? (nsteps - base) : chunk;
+            std::vector<int> nl(m, 0);
             for (uint32_t i = 0; i < m; i++) {
-                hipEvent_t *e = &ev[(size_t)i * (kMaxLaunches + 1)];
-                for (int j = 0; j < nl[i]; j++) {
-                    float ms = 0.0f;
-                    LMX_HIP(hipEventElapsedTime(&ms, e[j], e[j + 1]));
-                    const int kind = kinds[(size_t)i * kMaxLaunches + j];
-                    if ((uint32_t)kind < nkernels) {
-                        kernel_ms[kind] += ms;
-                        if (kernel_launches) kernel_launches[kind]++;
+                if (kernel_ms)
+                    rc = enqueue_step(s, p, &ev[(size_t)i * (kMaxLaunches + 1)], &kinds[(size_t)i * kMaxLaunches], &nl[i]);
+                else
+                    rc = enqueue_step(s, p, nullptr, nullptr, nullptr);
+                if (rc) return rc;
+            }
+            if (kernel_ms) {
+                LMX_HIP(hipEventSynchronize(ev[(size_t)(m - 1) * (kMaxLaunches + 1) + nl[m - 1]]));
+                for (uint32_t i = 0; i < m; i++) {
+                    hipEvent_t *e = &ev[(size_t)i * (kMaxLaunches + 1)];
+                    for (int j = 0; j < nl[i]; j++) {
+                        float ms = 0.0f;
+                        LMX_HIP(hipEventElapsedTime(&ms, e[j], e[j + 1]));
+                        const int kind = kinds[(size_t)i * kMaxLaunches + j];
+                        if ((uint32_t)kind < nkernels) {
+                            kernel_ms[kind] += ms;
+                            if (kernel_launches) kernel_launches[kind]++;
+                        }
                     }
                 }
             }
         }
-    }
-    LMX_HIP(hipEventRecord(t1, s->stream));
-    LMX_HIP(hipEventSynchronize(t1));
-    float ms = 0.0f;
-    LMX_HIP(hipEventElapsedTime(&ms, t0, t1));
-    if (total_ms) *total_ms = ms;
-    for (auto &e : ev) hipEventDestroy(e);
-    hipEventDestroy(t0);
-    hipEventDestroy(t1);
-    return check_device_error(s);
+        LMX_HIP(hipEventRecord(t1, s->stream));
+        LMX_HIP(hipEventSynchronize(t1));
+        float ms = 0.0f;
+        LMX_HIP(hipEventElapsedTime(&ms, t0, t1));
+        if (total_ms) *total_ms = ms;
+        for (auto &e : ev) hipEventDestroy(e);
+        hipEventDestroy(t0);
+        hipEventDestroy(t1);
+        return check_device_error(s);
+    });
 }
 
 int lmx_get_table_f32(lmx_solver *s, int which, float *out, size_t n, size_t *written)
 {
-    if (!s) return fail(LMX_ERR_ARG, "null solver");
-    const std::vector<float> *v = nullptr;
-    switch (which) {
-    case 0: v = &s->T; break;
-    case 1: v = &s->qp0; break;
-    case 2: v = &s->qp1; break;
-    case 3: v = &s->qp2; break;
-    case 4: v = &s->rate; break;
-    default: return fail(LMX_ERR_ARG, "unknown table");
-    }
-    if (written) *written = v->size();
-    if (out) {
-        if (n < v->size()) return fail(LMX_ERR_ARG, "table buffer too small");
-        memcpy(out, v->data(), v->size() * sizeof(float));
-    }
-    return LMX_OK;
+    return guarded([&]() -> int {
+        if (!s) return fail(LMX_ERR_ARG, "null solver");
+        const std::vector<float> *v = nullptr;
+        switch (which) {
+        case 0: v = &s->T; break;
+        case 1: v = &s->qp0; break;
+        case 2: v = &s->qp1; break;
+        case 3: v = &s->qp2; break;
+        case 4: v = &s->rate; break;
+        default: return fail(LMX_ERR_ARG, "unknown table");
+        }
+        if (written) *written = v->size();
+        if (out) {
+            if (n < v->size()) return fail(LMX_ERR_ARG, "table buffer too small");
+            memcpy(out, v->data(), v->size() * sizeof(float));
+        }
+        return LMX_OK;
+    });
 }
 
 int lmx_copy_bandwidth(int device, size_t bytes, int reps, double *gbps)
@@ -1033,36 +1071,40 @@ int lmx_copy_bandwidth(int device, size_t bytes, int reps, double *gbps)
 
 int lmx_device_exp(int device, const double *x, double *y, size_t n)
 {
-    if (!x || !y) return fail(LMX_ERR_ARG, "null argument");
-    LMX_HIP(hipSetDevice(device));
-    double *dx = nullptr, *dy = nullptr;
-    LMX_HIP(hipMalloc((void **)&dx, (n ? n : 1) * sizeof(double)));
-    LMX_HIP(hipMalloc((void **)&dy, (n ? n : 1) * sizeof(double)));
-    LMX_HIP(hipMemcpy(dx, x, n * sizeof(double), hipMemcpyHostToDevice));
-    hipError_t e = lmx::launch_exp(dx, dy, n, nullptr);
-    if (e == hipSuccess) e = hipDeviceSynchronize();
-    if (e == hipSuccess) e = hipMemcpy(y, dy, n * sizeof(double), hipMemcpyDeviceToHost);
-    hipFree(dx);
-    hipFree(dy);
-    if (e != hipSuccess) return hip_fail(e, "lmx_device_exp");
-    return LMX_OK;
+    return guarded([&]() -> int {
+        if (!x || !y) return fail(LMX_ERR_ARG, "null argument");
+        LMX_HIP(hipSetDevice(device));
+        double *dx = nullptr, *dy = nullptr;
+        LMX_HIP(hipMalloc((void **)&dx, (n ? n : 1) * sizeof(double)));
+        LMX_HIP(hipMalloc((void **)&dy, (n ? n : 1) * sizeof(double)));
+        LMX_HIP(hipMemcpy(dx, x, n * sizeof(double), hipMemcpyHostToDevice));
+        hipError_t e = lmx::launch_exp(dx, dy, n, nullptr);
+        if (e == hipSuccess) e = hipDeviceSynchronize();
+        if (e == hipSuccess) e = hipMemcpy(y, dy, n * sizeof(double), hipMemcpyDeviceToHost);
+        hipFree(dx);
+        hipFree(dy);
+        if (e != hipSuccess) return hip_fail(e, "lmx_device_exp");
+        return LMX_OK;
+    });
 }
 
 int lmx_device_reaction_probability(int device, const float *a, float *prob, size_t n)
 {
-    if (!a || !prob) return fail(LMX_ERR_ARG, "null argument");
-    LMX_HIP(hipSetDevice(device));
-    float *da = nullptr, *dp = nullptr;
-    LMX_HIP(hipMalloc((void **)&da, (n ? n : 1) * sizeof(float)));
-    LMX_HIP(hipMalloc((void **)&dp, (n ? n : 1) * sizeof(float)));
-    LMX_HIP(hipMemcpy(da, a, n * sizeof(float), hipMemcpyHostToDevice));
-    hipError_t e = lmx::launch_prob(da, dp, n, nullptr);
-    if (e == hipSuccess) e = hipDeviceSynchronize();
-    if (e == hipSuccess) e = hipMemcpy(prob, dp, n * sizeof(float), hipMemcpyDeviceToHost);
-    hipFree(da);
-    hipFree(dp);
-    if (e != hipSuccess) return hip_fail(e, "lmx_device_reaction_probability");
-    return LMX_OK;
+    return guarded([&]() -> int {
+        if (!a || !prob) return fail(LMX_ERR_ARG, "null argument");
+        LMX_HIP(hipSetDevice(device));
+        float *da = nullptr, *dp = nullptr;
+        LMX_HIP(hipMalloc((void **)&da, (n ? n : 1) * sizeof(float)));
+        LMX_HIP(hipMalloc((void **)&dp, (n ? n : 1) * sizeof(float)));
+        LMX_HIP(hipMemcpy(da, a, n * sizeof(float), hipMemcpyHostToDevice));
+        hipError_t e = lmx::launch_prob(da, dp, n, nullptr);
+        if (e == hipSuccess) e = hipDeviceSynchronize();
+        if (e == hipSuccess) e = hipMemcpy(prob, dp, n * sizeof(float), hipMemcpyDeviceToHost);
+        hipFree(da);
+        hipFree(dp);
+        if (e != hipSuccess) return hip_fail(e, "lmx_device_reaction_probability");
+        return LMX_OK;
+    });
 }
 
 }  // extern "C"
@@ -1194,23 +1236,25 @@ int batch_capture(lmx_batch *b)
 
 int lmx_batch_create(int device, lmx_batch **out)
 {
-    if (!out) return fail(LMX_ERR_ARG, "null output handle");
-    *out = nullptr;
-    int n = 0;
-    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return fail(LMX_ERR_HIP, "no HIP device is available");
-    if (device < 0 || device >= n) return fail(LMX_ERR_ARG, "device index out of range");
-    lmx_batch *b = new lmx_batch();
-    b->device = device;
-    hipError_t e = hipSetDevice(device);
-    if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&b->done, hipEventDisableTiming);
-    if (e != hipSuccess) {
-        int rc = hip_fail(e, "lmx_batch_create");
-        lmx_batch_destroy(b);
-        return rc;
-    }
-    *out = b;
-    return LMX_OK;
+    return guarded([&]() -> int {
+        if (!out) return fail(LMX_ERR_ARG, "null output handle");
+        *out = nullptr;
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return fail(LMX_ERR_HIP, "no HIP device is available");
+        if (device < 0 || device >= n) return fail(LMX_ERR_ARG, "device index out of range");
+        lmx_batch *b = new lmx_batch();
+        b->device = device;
+        hipError_t e = hipSetDevice(device);
+        if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&b->done, hipEventDisableTiming);
+        if (e != hipSuccess) {
+            int rc = hip_fail(e, "lmx_batch_create");
+            lmx_batch_destroy(b);
+            return rc;
+        }
+        *out = b;
+        return LMX_OK;
+    });
 }
 
 int lmx_batch_destroy(lmx_batch *b)
@@ -1232,31 +1276,35 @@ int lmx_batch_destroy(lmx_batch *b)
 
 int lmx_batch_add(lmx_batch *b, lmx_solver *s)
 {
-    if (!b || !s) return fail(LMX_ERR_ARG, "null batch or solver");
-    if (s->batch) return fail(LMX_ERR_ARG, "the solver is already a member of a batch");
-    if (s->device != b->device) return fail(LMX_ERR_ARG, "the solver lives on another device");
-    int rc = require_ready(s);
-    if (rc) return rc;
-    const lmx::Params p = make_params(s);
-    if (s->bpp != 1) return fail(LMX_ERR_ARG, "batched solvers need the byte lattice");
-    if (!lmx::fast_path_ok(p)) return fail(LMX_ERR_ARG, "batched solvers need X % 4 == 0");
-    if (!b->members.empty() && s->W != b->W) return fail(LMX_ERR_ARG, "batched solvers need the same particles per site");
-    b->W = s->W;
-    b->members.push_back(s);
-    s->batch = b;
-    return LMX_OK;
+    return guarded([&]() -> int {
+        if (!b || !s) return fail(LMX_ERR_ARG, "null batch or solver");
+        if (s->batch) return fail(LMX_ERR_ARG, "the solver is already a member of a batch");
+        if (s->device != b->device) return fail(LMX_ERR_ARG, "the solver lives on another device");
+        int rc = require_ready(s);
+        if (rc) return rc;
+        const lmx::Params p = make_params(s);
+        if (s->bpp != 1) return fail(LMX_ERR_ARG, "batched solvers need the byte lattice");
+        if (!lmx::fast_path_ok(p)) return fail(LMX_ERR_ARG, "batched solvers need X % 4 == 0");
+        if (!b->members.empty() && s->W != b->W) return fail(LMX_ERR_ARG, "batched solvers need the same particles per site");
+        b->W = s->W;
+        b->members.push_back(s);
+        s->batch = b;
+        return LMX_OK;
+    });
 }
 
 int lmx_batch_remove(lmx_batch *b, lmx_solver *s)
 {
-    if (!b || !s) return fail(LMX_ERR_ARG, "null batch or solver");
-    auto it = std::find(b->members.begin(), b->members.end(), s);
-    if (it == b->members.end()) return fail(LMX_ERR_ARG, "the solver is not a member of this batch");
-    hipSetDevice(b->device);
-    hipStreamSynchronize(b->stream);
-    b->members.erase(it);
-    s->batch = nullptr;
-    return LMX_OK;
+    return guarded([&]() -> int {
+        if (!b || !s) return fail(LMX_ERR_ARG, "null batch or solver");
+        auto it = std::find(b->members.begin(), b->members.end(), s);
+        if (it == b->members.end()) return fail(LMX_ERR_ARG, "the solver is not a member of this batch");
+        hipSetDevice(b->device);
+        hipStreamSynchronize(b->stream);
+        b->members.erase(it);
+        s->batch = nullptr;
+        return LMX_OK;
+    });
 }
 
 int lmx_batch_size(const lmx_batch *b, uint32_t *n)
@@ -1279,26 +1327,28 @@ int lmx_batch_set_option(lmx_batch *b, int option, int value)
 
 int lmx_batch_run_steps(lmx_batch *b, uint32_t nsteps)
 {
-    if (!b) return fail(LMX_ERR_ARG, "null batch");
-    if (nsteps == 0 || b->members.empty()) return LMX_OK;
-    LMX_HIP(hipSetDevice(b->device));
-    int rc = batch_prepare(b);
-    if (rc) return rc;
-    if (b->use_graph) {
-        if ((rc = batch_capture(b))) return rc;
-        for (uint32_t i = 0; i < nsteps; i++) LMX_HIP(hipGraphLaunch(b->exec, b->stream));
-    } else {
-        for (uint32_t i = 0; i < nsteps; i++)
-            if ((rc = batch_enqueue_step(b))) return rc;
-    }
-    LMX_HIP(hipEventRecord(b->done, b->stream));
-    for (lmx_solver *s : b->members) {
-        if (nsteps & 1u) s->cur = 1 - s->cur;
-        s->timestep += nsteps;
-        // Later work on a member's own stream starts after the batch's.
-        LMX_HIP(hipStreamWaitEvent(s->stream, b->done, 0));
-    }
-    return LMX_OK;
+    return guarded([&]() -> int {
+        if (!b) return fail(LMX_ERR_ARG, "null batch");
+        if (nsteps == 0 || b->members.empty()) return LMX_OK;
+        LMX_HIP(hipSetDevice(b->device));
+        int rc = batch_prepare(b);
+        if (rc) return rc;
+        if (b->use_graph) {
+            if ((rc = batch_capture(b))) return rc;
+            for (uint32_t i = 0; i < nsteps; i++) LMX_HIP(hipGraphLaunch(b->exec, b->stream));
+        } else {
+            for (uint32_t i = 0; i < nsteps; i++)
+                if ((rc = batch_enqueue_step(b))) return rc;
+        }
+        LMX_HIP(hipEventRecord(b->done, b->stream));
+        for (lmx_solver *s : b->members) {
+            if (nsteps & 1u) s->cur = 1 - s->cur;
+            s->timestep += nsteps;
+            // Later work on a member's own stream starts after the batch's.
+            LMX_HIP(hipStreamWaitEvent(s->stream, b->done, 0));
+        }
+        return LMX_OK;
+    });
 }
 
 int lmx_batch_synchronize(lmx_batch *b)
@@ -1315,25 +1365,27 @@ int lmx_batch_synchronize(lmx_batch *b)
 
 int lmx_batch_time_steps(lmx_batch *b, uint32_t nsteps, double *total_ms)
 {
-    if (!b || !total_ms) return fail(LMX_ERR_ARG, "null argument");
-    LMX_HIP(hipSetDevice(b->device));
-    hipEvent_t t0, t1;
-    LMX_HIP(hipEventCreate(&t0));
-    LMX_HIP(hipEventCreate(&t1));
-    int rc = batch_prepare(b);
-    if (!rc && b->use_graph) rc = batch_capture(b);
-    if (!rc) {
-        LMX_HIP(hipStreamSynchronize(b->stream));
-        LMX_HIP(hipEventRecord(t0, b->stream));
-        rc = lmx_batch_run_steps(b, nsteps);
-        LMX_HIP(hipEventRecord(t1, b->stream));
-        LMX_HIP(hipEventSynchronize(t1));
-        float ms = 0.0f;
-        LMX_HIP(hipEventElapsedTime(&ms, t0, t1));
-        *total_ms = ms;
-    }
-    hipEventDestroy(t0);
-    hipEventDestroy(t1);
-    if (rc) return rc;
-    return lmx_batch_synchronize(b);
+    return guarded([&]() -> int {
+        if (!b || !total_ms) return fail(LMX_ERR_ARG, "null argument");
+        LMX_HIP(hipSetDevice(b->device));
+        hipEvent_t t0, t1;
+        LMX_HIP(hipEventCreate(&t0));
+        LMX_HIP(hipEventCreate(&t1));
+        int rc = batch_prepare(b);
+        if (!rc && b->use_graph) rc = batch_capture(b);
+        if (!rc) {
+            LMX_HIP(hipStreamSynchronize(b->stream));
+            LMX_HIP(hipEventRecord(t0, b->stream));
+            rc = lmx_batch_run_steps(b, nsteps);
+            LMX_HIP(hipEventRecord(t1, b->stream));
+            LMX_HIP(hipEventSynchronize(t1));
+            float ms = 0.0f;
+            LMX_HIP(hipEventElapsedTime(&ms, t0, t1));
+            *total_ms = ms;
+        }
+        hipEventDestroy(t0);
+        hipEventDestroy(t1);
+        if (rc) return rc;
+        return lmx_batch_synchronize(b);
+    });
 }

@@ -36,6 +36,7 @@ class OrcModel(Structure):
         ("qp2", POINTER(c_float)), ("rate", POINTER(c_float)),
         ("order", POINTER(c_uint32)), ("D1", POINTER(c_uint32)), ("D2", POINTER(c_uint32)),
         ("S", POINTER(ctypes.c_int8)), ("RL", POINTER(c_uint8)),
+        ("bpp", c_uint32),
     ]
 
 
@@ -87,11 +88,17 @@ def load() -> ctypes.CDLL:
     lib.orc_species_counts.restype = None
     lib.orc_species_counts.argtypes = [c_uint32, c_uint32, u32p, c_uint32, u32p]
     lib.orc_sweep.restype = c_int
-    lib.orc_sweep.argtypes = [POINTER(OrcModel), u32p, u8p, u32p, c_uint32, c_uint64, c_int, u32p, u8p, u32p]
+    lib.orc_lattice_from_row_major_words.restype = c_int
+    lib.orc_lattice_from_row_major_words.argtypes = [c_uint32, c_uint32, c_uint32, c_uint32, c_uint32, u32p, u32p]
+    lib.orc_lattice_to_row_major_words.restype = None
+    lib.orc_lattice_to_row_major_words.argtypes = [c_uint32, c_uint32, c_uint32, c_uint32, u32p, u32p]
+    lib.orc_species_counts_words.restype = None
+    lib.orc_species_counts_words.argtypes = [c_uint32, c_uint32, u32p, c_uint32, u32p]
+    lib.orc_sweep.argtypes = [POINTER(OrcModel), u32p, u8p, u32p, c_uint32, c_uint64, c_int, u32p, u32p, u32p]
     lib.orc_react.restype = c_int
-    lib.orc_react.argtypes = [POINTER(OrcModel), u32p, u8p, c_uint64, u32p, u8p, u32p, POINTER(c_uint64)]
+    lib.orc_react.argtypes = [POINTER(OrcModel), u32p, u8p, c_uint64, u32p, u32p, u32p, POINTER(c_uint64)]
     lib.orc_resolve_overflows.restype = c_int
-    lib.orc_resolve_overflows.argtypes = [POINTER(OrcModel), u32p, u8p, u32p, u8p, c_uint32]
+    lib.orc_resolve_overflows.argtypes = [POINTER(OrcModel), u32p, u8p, u32p, u32p, c_uint32]
     lib.orc_run.restype = c_int
     lib.orc_run.argtypes = [POINTER(OrcModel), u32p, u8p, c_uint32, c_uint32, c_int, POINTER(OrcStats)]
     lib.orc_nsm_create.restype = c_void_p
@@ -129,7 +136,9 @@ def seed32(seed_top: int, replicate: int) -> int:
 
 class Oracle:
     """Tables + state for one model, built from the same arrays the solver
-    receives (a lattice_microbes_amd.models.ModelSpec or compatible)."""
+    receives (a lattice_microbes_amd.models.ModelSpec or compatible).  A
+    uint32 spec lattice selects the uint32 lattice (one particle per word,
+    IntMpdRdmeSolver); otherwise the byte lattice."""
 
     def __init__(self, spec, seed_top: int = 1, replicate: int = 0):
         lib = load()
@@ -139,6 +148,8 @@ class Oracle:
         self.shape = (X, Y, Z)
         self.N = X * Y * Z
         self.P = spec.P
+        self.words = np.asarray(spec.lattice).dtype == np.uint32
+        self.W = self.P if self.words else self.P // 4
         nS, nST, R = spec.nS, spec.nST, spec.R
         types = np.ascontiguousarray(spec.types, dtype=np.uint32)
         K = np.ascontiguousarray(spec.K, dtype=np.float64).reshape(-1)
@@ -179,9 +190,15 @@ class Oracle:
                                           _p(self.qp1, c_float), _p(self.qp2, c_float))
         if rc:
             raise ValueError("oracle propensity tables rejected the model")
-        lat_rm = np.ascontiguousarray(spec.lattice, dtype=np.uint8).reshape(-1)
-        self.lattice = np.zeros(self.N * (self.P // 4), np.uint32)
-        rc = lib.orc_lattice_from_row_major(X, Y, Z, self.P, nS, _p(lat_rm, c_uint8), _p(self.lattice, c_uint32))
+        self.lattice = np.zeros(self.N * self.W, np.uint32)
+        if self.words:
+            lat_rm = np.ascontiguousarray(spec.lattice, dtype=np.uint32).reshape(-1)
+            rc = lib.orc_lattice_from_row_major_words(X, Y, Z, self.P, nS, _p(lat_rm, c_uint32),
+                                                      _p(self.lattice, c_uint32))
+        else:
+            lat_rm = np.ascontiguousarray(spec.lattice, dtype=np.uint8).reshape(-1)
+            rc = lib.orc_lattice_from_row_major(X, Y, Z, self.P, nS, _p(lat_rm, c_uint8),
+                                                _p(self.lattice, c_uint32))
         if rc:
             raise ValueError("oracle rejected the lattice")
         sites_rm = np.ascontiguousarray(spec.sites, dtype=np.uint8).reshape(-1)
@@ -201,6 +218,7 @@ class Oracle:
         m.order, m.D1, m.D2 = _p(self.order, c_uint32), _p(self.D1, c_uint32), _p(self.D2, c_uint32)
         m.S = _p(self.S_rm, ctypes.c_int8)
         m.RL = _p(self.RL8, c_uint8)
+        m.bpp = 4 if self.words else 1
         self.timestep = 0
         self.stats = OrcStats()
 
@@ -217,17 +235,22 @@ class Oracle:
 
     def native(self) -> np.ndarray:
         X, Y, Z = self.shape
-        return self.lattice.reshape(self.P // 4, Z, Y, X).copy()
+        return self.lattice.reshape(self.W, Z, Y, X).copy()
 
     def row_major(self) -> np.ndarray:
         X, Y, Z = self.shape
-        out = np.zeros(self.N * self.P, np.uint8)
-        self.lib.orc_lattice_to_row_major(X, Y, Z, self.P, _p(self.lattice, c_uint32), _p(out, c_uint8))
+        if self.words:
+            out = np.zeros(self.N * self.P, np.uint32)
+            self.lib.orc_lattice_to_row_major_words(X, Y, Z, self.P, _p(self.lattice, c_uint32), _p(out, c_uint32))
+        else:
+            out = np.zeros(self.N * self.P, np.uint8)
+            self.lib.orc_lattice_to_row_major(X, Y, Z, self.P, _p(self.lattice, c_uint32), _p(out, c_uint8))
         return out.reshape(X, Y, Z, self.P)
 
     def species_counts(self) -> np.ndarray:
         out = np.zeros(max(self.spec.nS, 1), np.uint32)
-        self.lib.orc_species_counts(self.N, self.P, _p(self.lattice, c_uint32), self.spec.nS, _p(out, c_uint32))
+        fn = self.lib.orc_species_counts_words if self.words else self.lib.orc_species_counts
+        fn(self.N, self.P, _p(self.lattice, c_uint32), self.spec.nS, _p(out, c_uint32))
         return out[: self.spec.nS]
 
 

@@ -226,32 +226,49 @@ int orc_compute_propensities(uint32_t nS, uint32_t nST, uint32_t R, const uint32
 /* Lattice layout                                                           */
 /* ------------------------------------------------------------------------ */
 
-static inline uint8_t slot_get(const uint32_t *lat, size_t N, size_t i, uint32_t j)
+/* Byte lattice: slot j of site i is byte j%4 of word i of plane j/4
+ * (ByteLattice.cpp:371-435).  uint32 lattice: slot j is word i of plane j
+ * (IntLattice.cpp, one particle per word). */
+static inline uint32_t byte_get(const uint32_t *lat, size_t N, size_t i, uint32_t j)
 {
-    return (uint8_t)(lat[(size_t)(j >> 2) * N + i] >> (8 * (j & 3)));
+    return (lat[(size_t)(j >> 2) * N + i] >> (8 * (j & 3))) & 0xFFu;
 }
 
-static inline void slot_set(uint32_t *lat, size_t N, size_t i, uint32_t j, uint8_t v)
+static inline void byte_set(uint32_t *lat, size_t N, size_t i, uint32_t j, uint32_t v)
 {
     uint32_t *w = &lat[(size_t)(j >> 2) * N + i];
     uint32_t sh = 8 * (j & 3);
-    *w = (*w & ~(0xFFu << sh)) | ((uint32_t)v << sh);
+    *w = (*w & ~(0xFFu << sh)) | ((v & 0xFFu) << sh);
 }
 
-/* ByteLattice::getOccupancy, ByteLattice.cpp:206-222: slots up to the first 0. */
-static uint32_t occupancy(const uint32_t *lat, size_t N, size_t i, uint32_t P)
+static inline int word_mode(const orc_model *m) { return m->bpp == 4; }
+
+static inline uint32_t slot_get(const orc_model *m, const uint32_t *lat, size_t N, size_t i, uint32_t j)
 {
-    for (uint32_t j = 0; j < P; j++)
-        if (slot_get(lat, N, i, j) == 0) return j;
-    return P;
+    return word_mode(m) ? lat[(size_t)j * N + i] : byte_get(lat, N, i, j);
+}
+
+static inline void slot_set(const orc_model *m, uint32_t *lat, size_t N, size_t i, uint32_t j, uint32_t v)
+{
+    if (word_mode(m)) lat[(size_t)j * N + i] = v;
+    else byte_set(lat, N, i, j, v);
+}
+
+/* ByteLattice::getOccupancy, ByteLattice.cpp:206-222 (IntLattice.cpp:172-190):
+ * slots up to the first 0. */
+static uint32_t occupancy(const orc_model *m, const uint32_t *lat, size_t N, size_t i)
+{
+    for (uint32_t j = 0; j < m->P; j++)
+        if (slot_get(m, lat, N, i, j) == 0) return j;
+    return m->P;
 }
 
 /* ByteLattice::addParticle, ByteLattice.cpp:264-285: first empty slot. */
-static int add_particle(uint32_t *lat, size_t N, size_t i, uint32_t P, uint8_t p)
+static int add_particle(const orc_model *m, uint32_t *lat, size_t N, size_t i, uint32_t p)
 {
-    for (uint32_t j = 0; j < P; j++) {
-        if (slot_get(lat, N, i, j) == 0) {
-            slot_set(lat, N, i, j, p);
+    for (uint32_t j = 0; j < m->P; j++) {
+        if (slot_get(m, lat, N, i, j) == 0) {
+            slot_set(m, lat, N, i, j, p);
             return 1;
         }
     }
@@ -259,24 +276,41 @@ static int add_particle(uint32_t *lat, size_t N, size_t i, uint32_t P, uint8_t p
 }
 
 /* RDMESolver::setLatticeData, RDMESolver.cpp:144-167. */
-int orc_lattice_from_row_major(uint32_t X, uint32_t Y, uint32_t Z, uint32_t P, uint32_t nS,
-                               const uint8_t *rm, uint32_t *native)
+static int from_row_major(uint32_t X, uint32_t Y, uint32_t Z, uint32_t P, uint32_t nS, uint32_t bpp,
+                          const void *rm, uint32_t *native)
 {
+    orc_model m;
+    memset(&m, 0, sizeof(m));
+    m.P = P;
+    m.bpp = bpp;
     size_t N = (size_t)X * Y * Z;
-    memset(native, 0, N * (P / 4) * sizeof(uint32_t));
+    memset(native, 0, N * (bpp == 4 ? P : P / 4) * sizeof(uint32_t));
     size_t idx = 0;
     for (uint32_t x = 0; x < X; x++)
         for (uint32_t y = 0; y < Y; y++)
             for (uint32_t z = 0; z < Z; z++)
                 for (uint32_t l = 0; l < P; l++, idx++) {
-                    uint8_t v = rm[idx];
+                    uint32_t v = bpp == 4 ? ((const uint32_t *)rm)[idx] : ((const uint8_t *)rm)[idx];
                     if (v != 0) {
                         if (v > nS) return ORC_ERR_ARG;
                         size_t i = (size_t)x + (size_t)X * y + (size_t)X * Y * z;
-                        if (!add_particle(native, N, i, P, v)) return ORC_ERR_ARG;
+                        if (!add_particle(&m, native, N, i, v)) return ORC_ERR_ARG;
                     }
                 }
     return ORC_OK;
+}
+
+int orc_lattice_from_row_major(uint32_t X, uint32_t Y, uint32_t Z, uint32_t P, uint32_t nS,
+                               const uint8_t *rm, uint32_t *native)
+{
+    return from_row_major(X, Y, Z, P, nS, 1, rm, native);
+}
+
+/* IntLattice::setFromRowMajorData: the same insertion, 32-bit particles. */
+int orc_lattice_from_row_major_words(uint32_t X, uint32_t Y, uint32_t Z, uint32_t P, uint32_t nS,
+                                     const uint32_t *rm, uint32_t *native)
+{
+    return from_row_major(X, Y, Z, P, nS, 4, rm, native);
 }
 
 /* ByteLattice::copyNativeToRowMajorByte, ByteLattice.cpp:371-403. */
@@ -289,7 +323,21 @@ void orc_lattice_to_row_major(uint32_t X, uint32_t Y, uint32_t Z, uint32_t P,
             for (uint32_t z = 0; z < Z; z++) {
                 size_t i = (size_t)x + (size_t)X * y + (size_t)X * Y * z;
                 size_t d = (((size_t)x * Y + y) * Z + z) * P;
-                for (uint32_t j = 0; j < P; j++) rm[d + j] = slot_get(native, N, i, j);
+                for (uint32_t j = 0; j < P; j++) rm[d + j] = (uint8_t)byte_get(native, N, i, j);
+            }
+}
+
+/* IntLattice::copyNativeToRowMajor: word j of site i to (x, y, z, j). */
+void orc_lattice_to_row_major_words(uint32_t X, uint32_t Y, uint32_t Z, uint32_t P,
+                                    const uint32_t *native, uint32_t *rm)
+{
+    size_t N = (size_t)X * Y * Z;
+    for (uint32_t x = 0; x < X; x++)
+        for (uint32_t y = 0; y < Y; y++)
+            for (uint32_t z = 0; z < Z; z++) {
+                size_t i = (size_t)x + (size_t)X * y + (size_t)X * Y * z;
+                size_t d = (((size_t)x * Y + y) * Z + z) * P;
+                for (uint32_t j = 0; j < P; j++) rm[d + j] = native[(size_t)j * N + i];
             }
 }
 
@@ -324,6 +372,17 @@ void orc_species_counts(uint32_t N, uint32_t P, const uint32_t *native, uint32_t
             if (p == 0) break;
             if (p <= nS) counts[p - 1]++;
         }
+    }
+}
+
+/* IntLattice::getParticleCounts, IntLattice.cpp:419-435: every nonzero word. */
+void orc_species_counts_words(uint32_t N, uint32_t P, const uint32_t *native, uint32_t nS, uint32_t *counts)
+{
+    memset(counts, 0, nS * sizeof(uint32_t));
+    size_t nw = (size_t)N * P;
+    for (size_t w = 0; w < nw; w++) {
+        uint32_t p = native[w];
+        if (p >= 1 && p <= nS) counts[p - 1]++;
     }
 }
 
@@ -386,8 +445,8 @@ static inline size_t nbr_plus(const orc_model *m, const axis_t *a, size_t i)
     return m->boundary ? NO_SITE : i - (size_t)(a->len - 1) * a->stride;
 }
 
-static int push_overflow(uint32_t *ovf_site, uint8_t *ovf_particle, uint32_t *ovf_count,
-                         size_t site, uint8_t particle)
+static int push_overflow(uint32_t *ovf_site, uint32_t *ovf_particle, uint32_t *ovf_count,
+                         size_t site, uint32_t particle)
 {
     uint32_t k = (*ovf_count)++;
     if (k < ORC_MAX_OVERFLOWS) {
@@ -404,7 +463,7 @@ static int push_overflow(uint32_t *ovf_site, uint8_t *ovf_particle, uint32_t *ov
  * (performPropagation, :199-270). */
 static int sweep_choices(const orc_model *m, const uint32_t *src, const uint8_t *sites, uint32_t *dst,
                          uint32_t axis, uint64_t hash,
-                         uint32_t *ovf_site, uint8_t *ovf_particle, uint32_t *ovf_count)
+                         uint32_t *ovf_site, uint32_t *ovf_particle, uint32_t *ovf_count)
 {
     axis_t a = make_axis(m, axis);
     size_t N = a.N;
@@ -415,7 +474,7 @@ static int sweep_choices(const orc_model *m, const uint32_t *src, const uint8_t 
         size_t mi = nbr_minus(m, &a, i), pi = nbr_plus(m, &a, i);
         uint32_t st = sites[i];
         for (uint32_t j = 0; j < P; j++) {
-            uint8_t p = slot_get(src, N, i, j);
+            uint32_t p = slot_get(m, src, N, i, j);
             if (p == 0) continue;
             float qm = lookup_T(m, p, st, mi, sites);
             float qp = lookup_T(m, p, st, pi, sites);
@@ -425,19 +484,19 @@ static int sweep_choices(const orc_model *m, const uint32_t *src, const uint8_t 
             choice[i * P + j] = c;
         }
     }
-    uint8_t nw[48];
+    uint32_t nw[48];
     for (size_t i = 0; i < N; i++) {
         uint32_t n = 0;
         size_t mi = nbr_minus(m, &a, i), pi = nbr_plus(m, &a, i);
         for (uint32_t j = 0; j < P; j++)
-            if (choice[i * P + j] == MOVE_STAY) nw[n++] = slot_get(src, N, i, j);
+            if (choice[i * P + j] == MOVE_STAY) nw[n++] = slot_get(m, src, N, i, j);
         if (mi != NO_SITE)
             for (uint32_t j = 0; j < P; j++)
-                if (choice[mi * P + j] == MOVE_PLUS) nw[n++] = slot_get(src, N, mi, j);
+                if (choice[mi * P + j] == MOVE_PLUS) nw[n++] = slot_get(m, src, N, mi, j);
         if (pi != NO_SITE)
             for (uint32_t j = 0; j < P; j++)
-                if (choice[pi * P + j] == MOVE_MINUS) nw[n++] = slot_get(src, N, pi, j);
-        for (uint32_t j = 0; j < P; j++) slot_set(dst, N, i, j, j < n ? nw[j] : 0);
+                if (choice[pi * P + j] == MOVE_MINUS) nw[n++] = slot_get(m, src, N, pi, j);
+        for (uint32_t j = 0; j < P; j++) slot_set(m, dst, N, i, j, j < n ? nw[j] : 0);
         for (uint32_t j = P; j < n; j++) push_overflow(ovf_site, ovf_particle, ovf_count, i, nw[j]);
     }
     free(choice);
@@ -450,18 +509,18 @@ static int sweep_choices(const orc_model *m, const uint32_t *src, const uint8_t 
  * are exclusive, so this agrees with formulation 0 bit for bit. */
 static int sweep_pull(const orc_model *m, const uint32_t *src, const uint8_t *sites, uint32_t *dst,
                       uint32_t axis, uint64_t hash,
-                      uint32_t *ovf_site, uint8_t *ovf_particle, uint32_t *ovf_count)
+                      uint32_t *ovf_site, uint32_t *ovf_particle, uint32_t *ovf_count)
 {
     axis_t a = make_axis(m, axis);
     size_t N = a.N;
     uint32_t P = m->P, pcb = orc_particle_count_bits(P);
-    uint8_t nw[48];
+    uint32_t nw[48];
     for (size_t i = 0; i < N; i++) {
         uint32_t n = 0;
         size_t mi = nbr_minus(m, &a, i), pi = nbr_plus(m, &a, i);
         uint32_t st = sites[i];
         for (uint32_t j = 0; j < P; j++) {
-            uint8_t p = slot_get(src, N, i, j);
+            uint32_t p = slot_get(m, src, N, i, j);
             if (p == 0) continue;
             float u = orc_uniform((uint32_t)i, pcb, j, hash);
             float qm = lookup_T(m, p, st, mi, sites);
@@ -472,7 +531,7 @@ static int sweep_pull(const orc_model *m, const uint32_t *src, const uint8_t *si
         if (mi != NO_SITE) {
             uint32_t stm = sites[mi];
             for (uint32_t j = 0; j < P; j++) {
-                uint8_t p = slot_get(src, N, mi, j);
+                uint32_t p = slot_get(m, src, N, mi, j);
                 if (p == 0) continue;
                 float u = orc_uniform((uint32_t)mi, pcb, j, hash);
                 float qp = lookup_T(m, p, stm, i, sites);
@@ -482,14 +541,14 @@ static int sweep_pull(const orc_model *m, const uint32_t *src, const uint8_t *si
         if (pi != NO_SITE) {
             uint32_t stp = sites[pi];
             for (uint32_t j = 0; j < P; j++) {
-                uint8_t p = slot_get(src, N, pi, j);
+                uint32_t p = slot_get(m, src, N, pi, j);
                 if (p == 0) continue;
                 float u = orc_uniform((uint32_t)pi, pcb, j, hash);
                 float qm = lookup_T(m, p, stp, i, sites);
                 if (u < qm) nw[n++] = p;
             }
         }
-        for (uint32_t j = 0; j < P; j++) slot_set(dst, N, i, j, j < n ? nw[j] : 0);
+        for (uint32_t j = 0; j < P; j++) slot_set(m, dst, N, i, j, j < n ? nw[j] : 0);
         for (uint32_t j = P; j < n; j++) push_overflow(ovf_site, ovf_particle, ovf_count, i, nw[j]);
     }
     return ORC_OK;
@@ -497,7 +556,7 @@ static int sweep_pull(const orc_model *m, const uint32_t *src, const uint8_t *si
 
 int orc_sweep(const orc_model *m, const uint32_t *src, const uint8_t *sites, uint32_t *dst,
               uint32_t axis, uint64_t hash, int formulation,
-              uint32_t *ovf_site, uint8_t *ovf_particle, uint32_t *ovf_count)
+              uint32_t *ovf_site, uint32_t *ovf_particle, uint32_t *ovf_count)
 {
     if (axis > 2 || (m->P != 4 && m->P != 8 && m->P != 16)) return ORC_ERR_ARG;
     if (formulation == 0)
@@ -510,7 +569,7 @@ int orc_sweep(const orc_model *m, const uint32_t *src, const uint8_t *sites, uin
 /* ------------------------------------------------------------------------ */
 
 /* calculateReactionPropensity, global-R branch, byte_reaction_dev.cu:77-100. */
-static float reaction_propensity(const orc_model *m, uint32_t st, const uint8_t *slots, uint32_t r)
+static float reaction_propensity(const orc_model *m, uint32_t st, const uint32_t *slots, uint32_t r)
 {
     if (!m->RL[(size_t)r * m->nST + st]) return 0.0f;
     float n1 = 0.0f, n2 = 0.0f;
@@ -530,16 +589,16 @@ static float reaction_propensity(const orc_model *m, uint32_t st, const uint8_t 
 }
 
 /* precomp_reaction_kernel's total propensity, MpdRdmeSolver.cu:1617-1639. */
-static float total_propensity(const orc_model *m, uint32_t st, const uint8_t *slots)
+static float total_propensity(const orc_model *m, uint32_t st, const uint32_t *slots)
 {
     uint32_t nS = m->nS;
     float a = m->qp0[st];
     for (uint32_t i = 0; i < m->P; i++) {
-        uint8_t p1 = slots[i];
+        uint32_t p1 = slots[i];
         if (p1 > 0) {
             a += m->qp1[(size_t)st * nS + (p1 - 1)];
             for (uint32_t j = i + 1; j < m->P; j++) {
-                uint8_t p2 = slots[j];
+                uint32_t p2 = slots[j];
                 if (p2 > 0)
                     a += m->qp2[(size_t)st * nS * nS + (size_t)(p1 - 1) * nS + (p2 - 1)];
             }
@@ -549,17 +608,17 @@ static float total_propensity(const orc_model *m, uint32_t st, const uint8_t *sl
 }
 
 int orc_react(const orc_model *m, uint32_t *lat, const uint8_t *sites, uint64_t hash,
-              uint32_t *ovf_site, uint8_t *ovf_particle, uint32_t *ovf_count, uint64_t *reactions)
+              uint32_t *ovf_site, uint32_t *ovf_particle, uint32_t *ovf_count, uint64_t *reactions)
 {
     size_t N = (size_t)m->X * m->Y * m->Z;
     uint32_t P = m->P, nS = m->nS;
-    uint8_t slots[16], nw[16];
+    uint32_t slots[16], nw[16];
     int8_t *Sloc = (int8_t *)malloc(nS ? nS : 1);
     if (!Sloc) return ORC_ERR_ARG;
     int err = ORC_OK;
     for (size_t i = 0; i < N; i++) {
         uint32_t st = sites[i];
-        for (uint32_t j = 0; j < P; j++) slots[j] = slot_get(lat, N, i, j);
+        for (uint32_t j = 0; j < P; j++) slots[j] = slot_get(m, lat, N, i, j);
         float a = total_propensity(m, st, slots);
         if (a == 0.0f) continue;
         float prob = orc_reaction_probability(a);
@@ -579,7 +638,7 @@ int orc_react(const orc_model *m, uint32_t *lat, const uint8_t *sites, uint64_t 
         memcpy(Sloc, &m->S[(size_t)idx * nS], nS);
         uint32_t n = 0;
         for (uint32_t j = 0; j < P; j++) {
-            uint8_t p = slots[j];
+            uint32_t p = slots[j];
             if (p > 0) {
                 if (Sloc[p - 1] >= 0) nw[n++] = p;
                 else Sloc[p - 1]++;
@@ -593,11 +652,11 @@ int orc_react(const orc_model *m, uint32_t *lat, const uint8_t *sites, uint64_t 
                 continue;
             }
             for (int j = 0; j < Sloc[s]; j++) {
-                if (n < P) nw[n++] = (uint8_t)(s + 1);
-                else push_overflow(ovf_site, ovf_particle, ovf_count, i, (uint8_t)(s + 1));
+                if (n < P) nw[n++] = s + 1;
+                else push_overflow(ovf_site, ovf_particle, ovf_count, i, s + 1);
             }
         }
-        for (uint32_t j = 0; j < P; j++) slot_set(lat, N, i, j, j < n ? nw[j] : 0);
+        for (uint32_t j = 0; j < P; j++) slot_set(m, lat, N, i, j, j < n ? nw[j] : 0);
         if (reactions) (*reactions)++;
     }
     free(Sloc);
@@ -614,14 +673,14 @@ int orc_react(const orc_model *m, uint32_t *lat, const uint8_t *sites, uint64_t 
  * with the reference's unsigned arithmetic.  A candidate outside the lattice
  * makes the reference's getOccupancy throw: reported as an error here. */
 int orc_resolve_overflows(const orc_model *m, uint32_t *lat, const uint8_t *sites,
-                          const uint32_t *ovf_site, const uint8_t *ovf_particle, uint32_t count)
+                          const uint32_t *ovf_site, const uint32_t *ovf_particle, uint32_t count)
 {
     if (count > ORC_MAX_OVERFLOWS) return ORC_ERR_OVERFLOW_CAP;
     size_t N = (size_t)m->X * m->Y * m->Z;
     uint32_t X = m->X, Y = m->Y, Z = m->Z;
     for (uint32_t e = 0; e < count; e++) {
         uint32_t li = ovf_site[e];
-        uint8_t particle = ovf_particle[e];
+        uint32_t particle = ovf_particle[e];
         uint32_t xc = li % X, yc = (li / X) % Y, zc = li / (X * Y);
         uint8_t st0 = sites[li];
         int placed = 0;
@@ -640,8 +699,8 @@ int orc_resolve_overflows(const orc_model *m, uint32_t *lat, const uint8_t *site
                         if (!(r2 <= maxD2 && (r2 > minD2 || minD == maxD))) continue;
                         if (x >= X || y >= Y || z >= Z) return ORC_ERR_OVERFLOW_OUT_OF_RANGE;
                         size_t s = (size_t)x + (size_t)X * y + (size_t)X * Y * z;
-                        if (occupancy(lat, N, s, m->P) < m->P && sites[s] == st0) {
-                            add_particle(lat, N, s, m->P, particle);
+                        if (occupancy(m, lat, N, s) < m->P && sites[s] == st0) {
+                            add_particle(m, lat, N, s, particle);
                             placed = 1;
                         }
                     }
@@ -661,10 +720,10 @@ int orc_run(const orc_model *m, uint32_t *lat, const uint8_t *sites, uint32_t t0
             uint32_t nsteps, int formulation, orc_stats *stats)
 {
     size_t N = (size_t)m->X * m->Y * m->Z;
-    size_t words = N * (m->P / 4);
+    size_t words = N * (word_mode(m) ? m->P : m->P / 4);
     uint32_t *tmp = (uint32_t *)malloc(words * sizeof(uint32_t));
     uint32_t *ovf_site = (uint32_t *)malloc(ORC_MAX_OVERFLOWS * sizeof(uint32_t));
-    uint8_t *ovf_particle = (uint8_t *)malloc(ORC_MAX_OVERFLOWS);
+    uint32_t *ovf_particle = (uint32_t *)malloc(ORC_MAX_OVERFLOWS * sizeof(uint32_t));
     if (!tmp || !ovf_site || !ovf_particle) {
         free(tmp); free(ovf_site); free(ovf_particle);
         return ORC_ERR_ARG;

@@ -25,6 +25,9 @@
  *       propensities, stoichiometry.
  *   overflow                                    : hand-built full-site
  *       scenarios with placement computed by hand from the search order.
+ *   uint32 lattice (bpp = 4)                    : equal to the byte mode for
+ *       byte-sized models and, with species ids raised by a constant, to the
+ *       compact byte run; IntLattice layout and counts on hand cases.
  */
 #ifndef LMX_ORACLE_H
 #define LMX_ORACLE_H
@@ -71,6 +74,8 @@ typedef struct {
     const uint32_t *D2; /* [R] second reactant species+1 (0: none) */
     const int8_t *S;    /* [R][nS] reaction-major stoichiometry */
     const uint8_t *RL;  /* [R][nST] reaction location matrix */
+    uint32_t bpp;       /* bytes per particle: 1 (or 0) the byte lattice, W = P/4 planes;
+                           4 the uint32 lattice of IntMpdRdmeSolver, W = P planes */
 } orc_model;
 
 typedef struct {
@@ -111,22 +116,28 @@ void orc_lattice_to_row_major(uint32_t X, uint32_t Y, uint32_t Z, uint32_t P,
 int orc_sites_from_row_major(uint32_t X, uint32_t Y, uint32_t Z, uint32_t nST,
                              const uint8_t *row_major, uint8_t *native_out);
 void orc_species_counts(uint32_t N, uint32_t P, const uint32_t *native, uint32_t nS, uint32_t *counts);
+/* The uint32 lattice (IntLattice.cpp): row-major (x,y,z,p) uint32 values. */
+int orc_lattice_from_row_major_words(uint32_t X, uint32_t Y, uint32_t Z, uint32_t P, uint32_t nS,
+                                     const uint32_t *row_major, uint32_t *native_out);
+void orc_lattice_to_row_major_words(uint32_t X, uint32_t Y, uint32_t Z, uint32_t P,
+                                    const uint32_t *native, uint32_t *row_major_out);
+void orc_species_counts_words(uint32_t N, uint32_t P, const uint32_t *native, uint32_t nS, uint32_t *counts);
 
 /* One diffusion sweep along axis (0=x,1=y,2=z): src -> dst, overflow entries appended.
  * formulation 0: choice array (byte_diffusion_1d_dev.cu:86-197 + performPropagation :199-270)
  * formulation 1: per-site pull of only the decisions a site depends on. */
 int orc_sweep(const orc_model *m, const uint32_t *src, const uint8_t *sites, uint32_t *dst,
               uint32_t axis, uint64_t hash, int formulation,
-              uint32_t *ovf_site, uint8_t *ovf_particle, uint32_t *ovf_count);
+              uint32_t *ovf_site, uint32_t *ovf_particle, uint32_t *ovf_count);
 /* Reaction phase in place (MpdRdmeSolver.cu:1592-1675, byte_reaction_dev.cu:62-257). */
 int orc_react(const orc_model *m, uint32_t *lat, const uint8_t *sites, uint64_t hash,
-              uint32_t *ovf_site, uint8_t *ovf_particle, uint32_t *ovf_count, uint64_t *reactions);
+              uint32_t *ovf_site, uint32_t *ovf_particle, uint32_t *ovf_count, uint64_t *reactions);
 /* Classic overflow resolution (MpdRdmeSolver.cu:1103-1189, Lattice.cpp:106-128). */
 int orc_resolve_overflows(const orc_model *m, uint32_t *lat, const uint8_t *sites,
-                          const uint32_t *ovf_site, const uint8_t *ovf_particle, uint32_t count);
+                          const uint32_t *ovf_site, const uint32_t *ovf_particle, uint32_t count);
 
 /* Full timesteps t0 .. t0+nsteps-1 (MpdRdmeSolver.cu:996-1193).  lat is the
- * native lattice (W=P/4 planes of N words), updated in place. */
+ * native lattice (W planes of N words), updated in place. */
 int orc_run(const orc_model *m, uint32_t *lat, const uint8_t *sites, uint32_t t0,
             uint32_t nsteps, int formulation, orc_stats *stats);
 

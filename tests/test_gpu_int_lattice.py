@@ -1,8 +1,10 @@
 """The uint32 lattice (IntMpdRdmeSolver, one particle per 32-bit word).
 
-The reference's word kernels (word_diffusion_1d_dev.cu,
-word_reaction_dev.cu) apply the byte kernels' rules to 32-bit particles with
-the same RNG keys (slot index w under the same MPD_PARTICLE_COUNT_BITS), so:
+Every case is compared word for word with the oracle's uint32 mode (pinned
+against its byte mode in test_oracle_words.py).  The reference's word kernels
+(word_diffusion_1d_dev.cu, word_reaction_dev.cu) apply the byte kernels' rules
+to 32-bit particles with the same RNG keys (slot index w under the same
+MPD_PARTICLE_COUNT_BITS), so also:
 
 * for a model that fits a byte, the uint32 lattice's trajectory equals the
   byte lattice's particle for particle, step by step, including overflow
@@ -17,6 +19,7 @@ import pytest
 
 import lattice_microbes_amd as lmx
 from lattice_microbes_amd import InvalidArgException, models
+import oracle
 
 pytestmark = pytest.mark.gpu
 
@@ -39,14 +42,19 @@ def shifted(spec, off):
 
 
 def run_both(spec, int_spec, off, steps, chunks, seed):
+    """The uint32 lattice against the oracle's uint32 mode word for word, and
+    against the byte lattice with ids lowered by off."""
     b = lmx.MpdRdmeSolver(0)
     spec.load_into(b, seed=seed)
     w = lmx.IntMpdRdmeSolver(0)
     int_spec.load_into(w, seed=seed)
-    assert b.seed == w.seed
+    orc = oracle.Oracle(int_spec, seed_top=seed)
+    assert b.seed == w.seed == orc.seed32 and orc.words
     for c in range(chunks):
         b.runTimesteps(steps // chunks)
         w.runTimesteps(steps // chunks)
+        assert orc.run(steps // chunks) == 0
+        np.testing.assert_array_equal(w.getLatticeNative(), orc.native(), err_msg=f"oracle, chunk {c}")
         lb = b.getLattice().astype(np.uint32)
         lb[lb > 0] += off
         lw = w.getLattice()

@@ -31,12 +31,12 @@ def sweep(o, axis, t, formulation=0):
     lib = o.lib
     dst = np.zeros_like(o.lattice)
     ovs = np.zeros(2048, np.uint32)
-    ovp = np.zeros(2048, np.uint8)
+    ovp = np.zeros(2048, np.uint32)
     cnt = ctypes.c_uint32(0)
     h = timestep_hash(o.seed32, t, axis)
     rc = lib.orc_sweep(ctypes.byref(o.model), oracle._p(o.lattice, ctypes.c_uint32), oracle._p(o.sites, ctypes.c_uint8),
                        oracle._p(dst, ctypes.c_uint32), axis, h, formulation, oracle._p(ovs, ctypes.c_uint32),
-                       oracle._p(ovp, ctypes.c_uint8), ctypes.byref(cnt))
+                       oracle._p(ovp, ctypes.c_uint32), ctypes.byref(cnt))
     assert rc == 0
     X, Y, Z = o.shape
     n = cnt.value
