@@ -377,7 +377,10 @@ def main() -> int:
         "step_roofline": {"alg_bytes_per_step": step_alg, "achieved_GBps": step_alg / (step_ms / 1e3) / 1e9,
                           "frac": step_alg / (step_ms / 1e3) / 1e9 / HBM_PEAK_GBS,
                           "unfused_equivalent_bytes_per_step": unfused_alg,
-                          "unfused_equivalent_frac": unfused_alg / (step_ms / 1e3) / 1e9 / HBM_PEAK_GBS},
+                          "unfused_equivalent_frac": unfused_alg / (step_ms / 1e3) / 1e9 / HBM_PEAK_GBS,
+                          # a step that read and wrote the lattice once (2P+1 B/site)
+                          "fused_floor_bytes_per_step": sweep_bytes_per_site(P) * N,
+                          "fused_floor_frac": sweep_bytes_per_site(P) * N / (step_ms / 1e3) / 1e9 / HBM_PEAK_GBS},
         "kernels": kernels,
         "overflow": {"entries": ovf_entries, "steps_with_overflow": ovf_steps},
         "final_particles": int(counts.sum()),
