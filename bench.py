@@ -287,8 +287,9 @@ def main() -> int:
         return 0
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py runs the MPD-RDME step on one GPU only; multi-GPU sharding is not part of "
-              "this build (see the sharding design in DESIGN.md).", file=sys.stderr)
+        print("bench.py runs the MPD-RDME step on one GPU only: sharding a lattice into z slabs over "
+              "several GPUs (one exchange of boundary planes per step) is designed but not built.",
+              file=sys.stderr)
         return 2
 
     import lattice_microbes_amd as lmx

@@ -61,7 +61,7 @@ uint32_t orc_particle_count_bits(uint32_t P)
 /* Project double exp.                                                      */
 /*                                                                          */
 /* The reference evaluates (float)(1.0-exp(-(double)a)) with the platform   */
-/* double exp (byte_reaction_dev.cu:130-137).  This build uses its own exp, */
+/* double exp (byte_reaction_dev.cu:130-137).  Here the project's own exp,  */
 /* identical on host and device: Cody-Waite argument reduction              */
 /* (x = k*ln2 + r, ln2 split into a 32-bit head with trailing zeros and a   */
 /* tail) followed by the degree-13 Taylor polynomial of exp(r) for          */

@@ -28,5 +28,5 @@ def gpu():
     import lattice_microbes_amd as lmx
 
     if lmx.device_count() < 1:
-        pytest.fail("no HIP device visible: the gpu tests must run on a GPU box")
+        pytest.fail("no HIP device visible: the gpu tests need a machine with a HIP device")
     return 0
