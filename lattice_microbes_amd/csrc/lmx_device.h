@@ -66,6 +66,8 @@ struct Params {
     uint32_t X, Y, Z, XY, N;
     uint32_t P, W, pcb;         // slots, 32-bit words per site, RNG key shift
     uint32_t word_particles;    // 1: one particle per 32-bit word (W == P), else 4 per word
+    uint32_t packed;            // 1: every site of the step's input holds its particles in slots
+                                // 0..n-1, so a word after one whose last slot is empty is zero
     uint32_t nS, nST, R;
     uint32_t boundary;          // 0 periodic, 1 reflecting
     uint32_t seed32;
@@ -223,6 +225,20 @@ __device__ inline void record_overflow(Control *ctl, uint32_t phase, uint32_t si
     }
 }
 
+// Whether word k of a site (k >= 1) can hold a particle, given word k-1: on a
+// packed lattice only if word k-1's last slot is occupied.  Every kernel
+// writes packed sites, so only a lattice written from outside can be
+// unpacked (Params::packed is 0 for the first step after that).
+__device__ inline bool need_next_byte_word(const Params &p, uint32_t prev)
+{
+    return !p.packed || (prev >> 24) != 0u;
+}
+
+__device__ inline bool need_next_particle_word(const Params &p, uint32_t prev)
+{
+    return !p.packed || prev != 0u;
+}
+
 // One particle per 32-bit word (the uint32 lattice of IntMpdRdmeSolver):
 // WordBuf appends particles in order and keeps the first P.
 template <int P>
@@ -254,6 +270,7 @@ template <int W_>
 struct ByteSites {
     static constexpr int W = W_, P = 4 * W_, kPerWord = 4;
     using Buf = SlotBuf<W_>;
+    __device__ static inline bool need_next(const Params &p, uint32_t prev) { return need_next_byte_word(p, prev); }
     __device__ static inline uint32_t part(uint32_t word, int q) { return (word >> (8 * q)) & 0xFFu; }
     __device__ static inline uint32_t slot(const uint32_t *w, int j) { return slot_of(w, j); }
 };
@@ -262,6 +279,7 @@ template <int P_>
 struct WordSites {
     static constexpr int W = P_, P = P_, kPerWord = 1;
     using Buf = WordBuf<P_>;
+    __device__ static inline bool need_next(const Params &p, uint32_t prev) { return need_next_particle_word(p, prev); }
     __device__ static inline uint32_t part(uint32_t word, int) { return word; }
     __device__ static inline uint32_t slot(const uint32_t *w, int j) { return w[j]; }
 };

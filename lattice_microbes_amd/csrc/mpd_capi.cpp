@@ -100,6 +100,10 @@ struct lmx_solver {
     uint32_t nST = 0, X = 0, Y = 0, Z = 0, P = 0, W = 0;
     uint32_t bpp = 1;  // bytes per particle: 1 (byte lattice) or 4 (uint32 lattice, one particle per word)
     uint32_t counts_cap = 256;  // entries of d_counts
+    // Every site of the current lattice holds its particles in its first
+    // slots.  The kernels keep this; a lattice written from outside (native
+    // words, device pointers) may break it until the next step.
+    bool packed = true;
     size_t N = 0;
     double lambda = 0.0;
     std::vector<double> DF;
@@ -217,6 +221,7 @@ lmx::Params make_params(const lmx_solver *s)
     p.N = (uint32_t)s->N;
     p.P = s->P; p.W = s->W;
     p.word_particles = s->bpp == 4 ? 1u : 0u;
+    p.packed = s->packed ? 1u : 0u;
     p.pcb = s->P <= 4 ? 2u : (s->P <= 8 ? 3u : 4u);  // MPD_PARTICLE_COUNT_BITS
     p.nS = s->nS; p.nST = s->nST; p.R = s->R;
     p.boundary = (uint32_t)s->boundary;
@@ -347,8 +352,13 @@ uint32_t fused_zseg(lmx_solver *s)
 // One timestep (runTimestep, MpdRdmeSolver.cu:996-1193).  ev, if given,
 // receives events around each launch (ev[0] before the first launch, then one
 // after every launch); kinds receives the kernel class of each launch.
-int enqueue_step(lmx_solver *s, const lmx::Params &p, hipEvent_t *ev, int *kinds, int *nlaunch)
+int enqueue_step(lmx_solver *s, const lmx::Params &p0, hipEvent_t *ev, int *kinds, int *nlaunch)
 {
+    // The step writes packed sites whatever it reads, so only its first
+    // kernel may see an unpacked lattice; every launch of this step takes the
+    // current flag, and the next step may rely on packing.
+    lmx::Params p = p0;
+    p.packed = s->packed ? 1u : 0u;
     hipError_t e;
     int n = 0;
     auto mark = [&](int kind) -> int {
@@ -375,6 +385,7 @@ int enqueue_step(lmx_solver *s, const lmx::Params &p, hipEvent_t *ev, int *kinds
         if (int rc = mark(LMX_K_RESOLVE)) return rc;
         if (nlaunch) *nlaunch = n;
         s->timestep++;
+        s->packed = true;
         return LMX_OK;
     }
     // The tiled z sweeps list their occupied output sites, and the reaction
@@ -414,6 +425,7 @@ int enqueue_step(lmx_solver *s, const lmx::Params &p, hipEvent_t *ev, int *kinds
     if (int rc = mark(LMX_K_RESOLVE)) return rc;
     if (nlaunch) *nlaunch = n;
     s->timestep++;
+    s->packed = true;
     return LMX_OK;
 }
 
@@ -713,6 +725,7 @@ int lmx_build_diffusion_model(lmx_solver *s, uint32_t nST, const double *DF, con
         if ((rc = upload_tables(s))) return rc;
         LMX_HIP(hipMemset(s->d_ctl, 0, sizeof(lmx::Control)));
         s->timestep = 0;
+        s->packed = true;  // row_major_to_native fills each site's first slots
         s->diffusion_built = true;
         return LMX_OK;
     });
@@ -861,8 +874,24 @@ int lmx_set_lattice_native(lmx_solver *s, const uint32_t *native, size_t words)
             if (((native[i] >> (8 * b)) & 0xFFu) > s->nS)
                 return fail(LMX_ERR_ARG, "latticeData: an invalid species was found");
     }
+    // Whether every site holds its particles in its first slots.
+    bool packed = true;
+    for (size_t i = 0; i < s->N && packed; i++) {
+        bool gap = false;
+        for (uint32_t j = 0; j < s->P; j++) {
+            const uint32_t v = s->bpp == 4 ? native[(size_t)j * s->N + i]
+                                           : (native[(size_t)(j >> 2) * s->N + i] >> (8 * (j & 3))) & 0xFFu;
+            if (v == 0) {
+                gap = true;
+            } else if (gap) {
+                packed = false;
+                break;
+            }
+        }
+    }
     LMX_HIP(hipStreamSynchronize(s->stream));
     LMX_HIP(hipMemcpy(s->d_lat[s->cur], native, words * sizeof(uint32_t), hipMemcpyHostToDevice));
+    s->packed = packed;
     return LMX_OK;
 }
 
@@ -875,6 +904,7 @@ int lmx_device_lattice(lmx_solver *s, void **particles, void **sites)
     if ((rc = check_device_error(s))) return rc;
     *particles = s->d_lat[s->cur];
     *sites = s->d_sites;
+    s->packed = false;  // the caller may write any slot
     return LMX_OK;
 }
 
@@ -899,6 +929,7 @@ int lmx_set_lattice(lmx_solver *s, const uint8_t *rm, size_t size)
         if ((rc = row_major_to_native(s, rm, native))) return rc;
         LMX_HIP(hipStreamSynchronize(s->stream));
         LMX_HIP(hipMemcpy(s->d_lat[s->cur], native.data(), native.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+        s->packed = true;  // each site's particles inserted at its first empty slot
         return LMX_OK;
     });
 }
@@ -1344,6 +1375,7 @@ int lmx_batch_run_steps(lmx_batch *b, uint32_t nsteps)
         for (lmx_solver *s : b->members) {
             if (nsteps & 1u) s->cur = 1 - s->cur;
             s->timestep += nsteps;
+            s->packed = true;
             // Later work on a member's own stream starts after the batch's.
             LMX_HIP(hipStreamWaitEvent(s->stream, b->done, 0));
         }

@@ -58,12 +58,46 @@ __device__ inline void set_lane(uint4 &v, int k, uint32_t x)
     v.w = k == 3 ? x : v.w;
 }
 
+// A group is read in two parts: plane 0 and the site types first, then each
+// plane k >= 1 only if some site's word k-1 has its last slot occupied (or
+// the input may be unpacked).  Callers issue every first part before any
+// rest, so that the rests' wait on plane 0 does not hold back other loads.
+template <int W>
+__device__ inline void load_group_first(const Params &p, const uint32_t *__restrict__ src, uint32_t g,
+                                        Group4<W> &G)
+{
+    G.w[0] = reinterpret_cast<const uint4 *>(src)[g];
+#pragma unroll
+    for (int k = 1; k < W; k++) G.w[k] = make_uint4(0u, 0u, 0u, 0u);
+    G.t = reinterpret_cast<const uint32_t *>(p.sites)[g];
+}
+
+template <int W>
+__device__ inline void load_group_rest(const Params &p, const uint32_t *__restrict__ src, uint32_t g,
+                                       Group4<W> &G)
+{
+#pragma unroll
+    for (int k = 1; k < W; k++) {
+        const uint4 v = G.w[k - 1];
+        if (need_next_byte_word(p, v.x | v.y | v.z | v.w))
+            G.w[k] = reinterpret_cast<const uint4 *>(src + (size_t)k * p.N)[g];
+    }
+}
+
 template <int W>
 __device__ inline void load_group(const Params &p, const uint32_t *__restrict__ src, uint32_t g, Group4<W> &G)
 {
+    load_group_first<W>(p, src, g, G);
+    load_group_rest<W>(p, src, g, G);
+}
+
+// Words of one site, plane k >= 1 read as in load_group.
+template <int W>
+__device__ inline void load_site(const Params &p, const uint32_t *__restrict__ src, uint32_t i, uint32_t *w)
+{
+    w[0] = src[i];
 #pragma unroll
-    for (int k = 0; k < W; k++) G.w[k] = reinterpret_cast<const uint4 *>(src + (size_t)k * p.N)[g];
-    G.t = reinterpret_cast<const uint32_t *>(p.sites)[g];
+    for (int k = 1; k < W; k++) w[k] = need_next_byte_word(p, w[k - 1]) ? src[(size_t)k * p.N + i] : 0u;
 }
 
 template <int W>
@@ -271,7 +305,7 @@ __device__ inline void x4_body(const Params &p, const uint32_t *__restrict__ src
     const int wave_first = (int)g - lane;
 
     Group4<W> G;
-    load_group<W>(p, src, gc, G);
+    load_group_first<W>(p, src, gc, G);
     const uint32_t i0 = gc * 4u;
     const uint32_t x0 = (p.X & (p.X - 1u)) == 0u ? (i0 & (p.X - 1u)) : i0 % p.X;
     const bool has_m = (x0 > 0) || (p.boundary == 0);
@@ -285,7 +319,18 @@ __device__ inline void x4_body(const Params &p, const uint32_t *__restrict__ src
     const int sl = shl ? ll : lane, sr = shr ? lr : lane;
 
     // Words and types of the neighbour sites (left: site 3 of its group,
-    // right: site 0 of its group).
+    // right: site 0 of its group): from the neighbouring lanes, or read at
+    // the wave's edges (word 0 and type with the group's first part).
+    uint32_t lw0 = 0u, rw0 = 0u, ltd = 0u, rtd = 0u;
+    if (active && !shl) {
+        lw0 = src[il];
+        ltd = p.sites[il];
+    }
+    if (active && !shr) {
+        rw0 = src[ir];
+        rtd = p.sites[ir];
+    }
+    load_group_rest<W>(p, src, gc, G);
     uint32_t lw[W], rw[W];
 #pragma unroll
     for (int k = 0; k < W; k++) {
@@ -295,14 +340,16 @@ __device__ inline void x4_body(const Params &p, const uint32_t *__restrict__ src
     uint32_t lt = __shfl(G.t >> 24, sl, 64);
     uint32_t rt = __shfl(G.t & 0xFFu, sr, 64);
     if (active && !shl) {
+        lw[0] = lw0;
 #pragma unroll
-        for (int k = 0; k < W; k++) lw[k] = src[(size_t)k * p.N + il];
-        lt = p.sites[il];
+        for (int k = 1; k < W; k++) lw[k] = need_next_byte_word(p, lw[k - 1]) ? src[(size_t)k * p.N + il] : 0u;
+        lt = ltd;
     }
     if (active && !shr) {
+        rw[0] = rw0;
 #pragma unroll
-        for (int k = 0; k < W; k++) rw[k] = src[(size_t)k * p.N + ir];
-        rt = p.sites[ir];
+        for (int k = 1; k < W; k++) rw[k] = need_next_byte_word(p, rw[k - 1]) ? src[(size_t)k * p.N + ir] : 0u;
+        rt = rtd;
     }
     if (!has_m) {
 #pragma unroll
@@ -425,6 +472,26 @@ __device__ inline bool load_row(const Params &p, const uint32_t *__restrict__ sr
     rr %= (int)len;
     load_group<W>(p, src, (base + (uint32_t)rr * stride) >> 2, G);
     return true;
+}
+
+// First part of load_row: returns the group index for load_group_rest, or
+// ~0u when the row lies beyond a reflecting edge (G then stays zero).
+template <int W>
+__device__ inline uint32_t load_row_first(const Params &p, const uint32_t *__restrict__ src, uint32_t base,
+                                          uint32_t stride, uint32_t len, int r, Group4<W> &G)
+{
+    int rr = r;
+    if (rr < 0) {
+        if (p.boundary) { zero_group<W>(G); return ~0u; }
+        rr += (int)len;
+    } else if (rr >= (int)len) {
+        if (p.boundary) { zero_group<W>(G); return ~0u; }
+        rr -= (int)len;
+    }
+    rr %= (int)len;
+    const uint32_t g = (base + (uint32_t)rr * stride) >> 2;
+    load_group_first<W>(p, src, g, G);
+    return g;
 }
 
 template <int W>
@@ -730,14 +797,16 @@ __device__ inline void tile_body(const Params &p, const uint32_t *__restrict__ s
 
     Group4<W> G;
     zero_group<W>(G);
-    if (valid) load_group<W>(p, src, (base + r * stride) >> 2, G);
+    const uint32_t gG = (base + r * stride) >> 2;
+    if (valid) load_group_first<W>(p, src, gG, G);
     const bool top = valid && ly == 0;
     const bool bottom = valid && ly == nrows - 1;
     Group4<W> Ap, Bp;  // apron rows above and below the tile
     zero_group<W>(Ap);
     zero_group<W>(Bp);
-    if (top) load_row<W>(p, src, base, stride, len, (int)r0 - 1, Ap);
-    if (bottom) load_row<W>(p, src, base, stride, len, (int)(r0 + nrows), Bp);
+    uint32_t gA = ~0u, gB = ~0u;
+    if (top) gA = load_row_first<W>(p, src, base, stride, len, (int)r0 - 1, Ap);
+    if (bottom) gB = load_row_first<W>(p, src, base, stride, len, (int)(r0 + nrows), Bp);
     // Site types of the neighbouring rows along the axis, read directly (the
     // block loads those rows itself, so these hit in cache): the decisions
     // then need no barrier, and their table loads overlap other waves' work.
@@ -746,6 +815,9 @@ __device__ inline void tile_body(const Params &p, const uint32_t *__restrict__ s
     uint32_t tmy = 0, tpy = 0;
     if (hm_ok) tmy = top ? Ap.t : sites32[(base + (r > 0 ? r - 1 : len - 1) * stride) >> 2];
     if (hp_ok) tpy = bottom ? Bp.t : sites32[(base + (r + 1 < len ? r + 1 : 0u) * stride) >> 2];
+    if (valid) load_group_rest<W>(p, src, gG, G);
+    if (gA != ~0u) load_group_rest<W>(p, src, gA, Ap);
+    if (gB != ~0u) load_group_rest<W>(p, src, gB, Bp);
 
     const unsigned long long occ = valid ? group_occ<W>(G) : 0ull;
     unsigned long long mi = 0, pl = 0;
@@ -899,8 +971,7 @@ __device__ inline uint32_t group_nz(const Group4<W> &G)
 template <int W>
 __device__ inline void global_site_words(const Params &p, const uint32_t *__restrict__ lat, uint32_t i, uint32_t *w)
 {
-#pragma unroll
-    for (int k = 0; k < W; k++) w[k] = lat[(size_t)k * p.N + i];
+    load_site<W>(p, lat, i, w);
 }
 
 template <int W, bool REACT>
@@ -1141,13 +1212,27 @@ __device__ inline void reaction4_body(const Params &p, uint32_t *__restrict__ la
     for (int q = 0; q < kRG; q++) {
         const uint32_t g = g0 + 256u * q;
         if (g < ngroups) {
+            w4[q][0] = reinterpret_cast<const uint4 *>(lat)[g];
 #pragma unroll
-            for (int k = 0; k < W; k++) w4[q][k] = reinterpret_cast<const uint4 *>(lat + (size_t)k * p.N)[g];
+            for (int k = 1; k < W; k++) w4[q][k] = make_uint4(0u, 0u, 0u, 0u);
             t4[q] = reinterpret_cast<const uint32_t *>(p.sites)[g];
         } else {
 #pragma unroll
             for (int k = 0; k < W; k++) w4[q][k] = make_uint4(0u, 0u, 0u, 0u);
             t4[q] = 0u;
+        }
+    }
+    // Planes k >= 1 after every group's plane 0 (read as in load_group_rest).
+#pragma unroll
+    for (int q = 0; q < kRG; q++) {
+        const uint32_t g = g0 + 256u * q;
+        if (g < ngroups) {
+#pragma unroll
+            for (int k = 1; k < W; k++) {
+                const uint4 v = w4[q][k - 1];
+                if (need_next_byte_word(p, v.x | v.y | v.z | v.w))
+                    w4[q][k] = reinterpret_cast<const uint4 *>(lat + (size_t)k * p.N)[g];
+            }
         }
     }
 #pragma unroll

@@ -40,13 +40,15 @@ __global__ __launch_bounds__(256) void sweep_kernel(Params p, const uint32_t *__
     const uint32_t im = (c > 0) ? i - stride : i + (len - 1) * stride;
     const uint32_t ip = (c + 1 < len) ? i + stride : i - (len - 1) * stride;
 
+    // Word k >= 1 of a site is read only if word k-1 leaves room for it
+    // (L::need_next: its last slot occupied, or the input may be unpacked).
     uint32_t wi[W], wm[W], wp[W], out[W];
     uint32_t any = 0;
 #pragma unroll
     for (int k = 0; k < W; k++) {
-        wi[k] = src[(size_t)k * p.N + i];
-        wm[k] = has_m ? src[(size_t)k * p.N + im] : 0u;
-        wp[k] = has_p ? src[(size_t)k * p.N + ip] : 0u;
+        wi[k] = (k == 0 || L::need_next(p, wi[k > 0 ? k - 1 : 0])) ? src[(size_t)k * p.N + i] : 0u;
+        wm[k] = has_m && (k == 0 || L::need_next(p, wm[k > 0 ? k - 1 : 0])) ? src[(size_t)k * p.N + im] : 0u;
+        wp[k] = has_p && (k == 0 || L::need_next(p, wp[k > 0 ? k - 1 : 0])) ? src[(size_t)k * p.N + ip] : 0u;
         any |= wi[k] | wm[k] | wp[k];
     }
     if (any == 0) {
@@ -72,7 +74,7 @@ __global__ __launch_bounds__(256) void reaction_kernel(Params p, uint32_t *__res
     const uint32_t st = p.sites[i];
     uint32_t w[W];
 #pragma unroll
-    for (int k = 0; k < W; k++) w[k] = lat[(size_t)k * p.N + i];
+    for (int k = 0; k < W; k++) w[k] = (k == 0 || L::need_next(p, w[k > 0 ? k - 1 : 0])) ? lat[(size_t)k * p.N + i] : 0u;
     const unsigned long long h = timestep_hash(p.seed32, p.ctl->timestep, 3);
     if (react_site_l<L>(p, i, st, w, h)) {
 #pragma unroll

@@ -4,6 +4,8 @@ Every case builds the same model arrays for the product (through the C ABI)
 and for the oracle, runs the same seeded steps and compares the native
 lattices word for word.  Integer state, so the bar is exact equality.
 """
+import dataclasses
+
 import numpy as np
 import pytest
 
@@ -327,3 +329,78 @@ def test_served_jobs_equal_their_solo_runs(gpu):
     assert max(j.admitted - j.arrival for j in jobs) >= 0.0
     for j in jobs:
         assert j.digest == serving.run_alone(j), f"job {j.id} differs from its solo run"
+
+
+def _reverse_slots(native, P, words):
+    """Every site's particles moved to its last slots in reverse order: an
+    unpacked lattice (empty slots before occupied ones), as a caller may write
+    it through the native setter or device pointers."""
+    W = native.shape[0]
+    flat = native.reshape(W, -1)
+    if words:
+        slots = flat.T.copy()                                   # (N, P)
+    else:
+        slots = flat.T.copy().view(np.uint8).reshape(-1, P)     # (N, P): slot 4w+b
+    rev = slots[:, ::-1].copy()
+    if words:
+        out = rev.T
+    else:
+        out = rev.reshape(-1, W, 4).view(np.uint32).reshape(-1, W).T
+    return np.ascontiguousarray(out).reshape(native.shape)
+
+
+@pytest.mark.parametrize("words", [False, True], ids=["byte", "uint32"])
+def test_unpacked_lattice_matches_oracle(gpu, words):
+    """A lattice with empty slots before occupied ones (set through the
+    native setter) is read slot by slot in the first step, as the oracle reads
+    it; the steps after it see packed sites again."""
+    spec = models.random_model(shape=(16, 12, 10), P=8, nS=4, nST=2, R=5, density=0.3, seed=8)
+    if words:
+        spec = dataclasses.replace(spec, lattice=spec.lattice.astype(np.uint32))
+        solver = lmx.IntMpdRdmeSolver(0)
+    else:
+        solver = lmx.MpdRdmeSolver(0)
+    spec.load_into(solver, seed=4)
+    orc = oracle.Oracle(spec, seed_top=4)
+    unpacked = _reverse_slots(solver.getLatticeNative(), spec.P, words)
+    assert not np.array_equal(unpacked, solver.getLatticeNative())
+    solver.setLatticeNative(unpacked)
+    orc.lattice[:] = unpacked.reshape(-1)
+    for steps in (1, 1, 3):
+        solver.runTimesteps(steps)
+        assert orc.run(steps) == 0
+        np.testing.assert_array_equal(solver.getLatticeNative(), orc.native())
+    solver.close()
+
+
+def test_unpacked_lattice_through_device_pointers(gpu):
+    """The same through the device-hook path: a particle written on the device
+    into the last slot of an empty site is read in the next step."""
+    import torch
+
+    if not torch.cuda.is_initialized():
+        pytest.skip("torch must start before the library in this process (run with -m gpu)")
+    from lattice_microbes_amd.solver import DeviceHookLattice
+
+    spec = models.random_model(shape=(16, 12, 10), P=8, nS=4, nST=2, R=5, density=0.3, seed=9)
+    solver = lmx.MpdRdmeSolver(0)
+    spec.load_into(solver, seed=6)
+    orc = oracle.Oracle(spec, seed_top=6)
+    solver.runTimesteps(2)
+    assert orc.run(2) == 0
+    nat = solver.getLatticeNative()
+    empty = np.argwhere((nat == 0).all(axis=0))[:20]            # (z, y, x) of empty sites
+    dl = DeviceHookLattice(solver)
+    dl.refresh()
+    view = dl.getParticleLatticeView()
+    for z, y, x in empty:
+        view[1, z, y, x, 3] = 2                                # slot 7 of 8
+    dl.synchronize()
+    ov = orc.lattice.reshape(2, *nat.shape[1:])
+    for z, y, x in empty:
+        ov[1, z, y, x] = np.uint32(2) << 24
+    for steps in (1, 2):
+        solver.runTimesteps(steps)
+        assert orc.run(steps) == 0
+        np.testing.assert_array_equal(solver.getLatticeNative(), orc.native())
+    solver.close()
