@@ -170,7 +170,10 @@ int lmx_set_sites(lmx_solver *s, const uint8_t *rowMajor, size_t size);
  * device particle planes (W planes of N uint32, the native layout) and site
  * types (N bytes, index x + X*y + X*Y*z), after the solver's stream has
  * drained.  They stay valid until the next lmx_run_steps or lmx_set_*; the
- * caller finishes its own device work on them before the next step.
+ * caller finishes its own device work on them before the next step.  The
+ * caller may write any slot, so the next step reads every word plane of the
+ * lattice (the kernels otherwise skip a plane after a word whose last slot is
+ * empty, which the packed sites they write allow).
  * lmx_validate_lattice then checks them on the device (particle types at most
  * numberSpecies, site types below numberSiteTypes; LMX_ERR_INVALID_LATTICE). */
 int lmx_device_lattice(lmx_solver *s, void **particles, void **sites);
