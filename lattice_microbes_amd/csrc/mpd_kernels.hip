@@ -40,8 +40,12 @@ __global__ __launch_bounds__(256) void sweep_kernel(Params p, const uint32_t *__
     const uint32_t im = (c > 0) ? i - stride : i + (len - 1) * stride;
     const uint32_t ip = (c + 1 < len) ? i + stride : i - (len - 1) * stride;
 
-    // Word k >= 1 of a site is read only if word k-1 leaves room for it
-    // (L::need_next: its last slot occupied, or the input may be unpacked).
+    // The site types and word 0 of the three sites first; word k >= 1 of a
+    // site only if word k-1 leaves room for it (L::need_next: its last slot
+    // occupied, or the input may be unpacked).
+    const uint32_t sti = p.sites[i];
+    const uint32_t stm = has_m ? p.sites[im] : 0u;
+    const uint32_t stp = has_p ? p.sites[ip] : 0u;
     uint32_t wi[W], wm[W], wp[W], out[W];
     uint32_t any = 0;
 #pragma unroll
@@ -56,9 +60,6 @@ __global__ __launch_bounds__(256) void sweep_kernel(Params p, const uint32_t *__
         for (int k = 0; k < W; k++) dst[(size_t)k * p.N + i] = 0u;
         return;
     }
-    const uint32_t sti = p.sites[i];
-    const uint32_t stm = has_m ? p.sites[im] : 0u;
-    const uint32_t stp = has_p ? p.sites[ip] : 0u;
     pull_site_l<L>(p, i, im, ip, wi, wm, wp, sti, stm, stp, has_m, has_p, h, substep, out);
 #pragma unroll
     for (int k = 0; k < W; k++) dst[(size_t)k * p.N + i] = out[k];
