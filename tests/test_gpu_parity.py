@@ -404,3 +404,32 @@ def test_unpacked_lattice_through_device_pointers(gpu):
         assert orc.run(steps) == 0
         np.testing.assert_array_equal(solver.getLatticeNative(), orc.native())
     solver.close()
+
+
+@pytest.mark.parametrize("words", [False, True], ids=["byte", "uint32"])
+def test_upper_planes_filling(gpu, words):
+    """Upper word planes start empty (their reads skipped on packed sites)
+    and fill as sites crowd: each step equals the oracle's."""
+    spec = models.random_model(shape=(12, 10, 8), P=8, nS=3, nST=1, R=0, density=0.45, seed=12, qmax=0.5)
+    lat = spec.lattice.copy()
+    if words:
+        lat[..., 1:] = 0        # only slot 0: planes 1..7 start empty
+        spec = dataclasses.replace(spec, lattice=lat.astype(np.uint32))
+        solver = lmx.IntMpdRdmeSolver(0)
+    else:
+        lat[..., 4:] = 0        # slots 0..3: plane 1 starts empty
+        spec = dataclasses.replace(spec, lattice=lat)
+        solver = lmx.MpdRdmeSolver(0)
+    spec.load_into(solver, seed=21)
+    orc = oracle.Oracle(spec, seed_top=21)
+    W = spec.P if words else spec.P // 4
+    assert not solver.getLatticeNative()[1:].any()
+    filled = False
+    for _ in range(12):
+        solver.runTimesteps(1)
+        assert orc.run(1) == 0
+        got = solver.getLatticeNative()
+        np.testing.assert_array_equal(got, orc.native())
+        filled |= bool(got[1:].any())
+    assert filled and W > 1
+    solver.close()
