@@ -68,6 +68,7 @@ struct Params {
     uint32_t word_particles;    // 1: one particle per 32-bit word (W == P), else 4 per word
     uint32_t packed;            // 1: every site of the step's input holds its particles in slots
                                 // 0..n-1, so a word after one whose last slot is empty is zero
+    uint32_t plane_flags;       // 1: the kernels keep the chunk flags of planes k >= 1 (below)
     uint32_t nS, nST, R;
     uint32_t boundary;          // 0 periodic, 1 reflecting
     uint32_t seed32;
@@ -237,6 +238,33 @@ __device__ inline bool need_next_byte_word(const Params &p, uint32_t prev)
 __device__ inline bool need_next_particle_word(const Params &p, uint32_t prev)
 {
     return !p.packed || prev != 0u;
+}
+
+// Chunk flags of the byte lattice (Params::plane_flags): after a buffer's W
+// planes, for each plane k >= 1, one word per chunk of kChunkGroups groups
+// (256 sites, aligned), 1 while that chunk of plane k is known to be all
+// zero.  Every kernel that writes a nonzero word into a chunk of plane k
+// clears its flag; a wave that writes a whole chunk and writes only zeros sets
+// it; a zero word may be left unwritten if its chunk's flag was 1 when the
+// launch began (each word has one writer per launch).  The host zeroes the
+// flags (nothing known) whenever it writes a buffer or runs kernels that do
+// not keep them.
+constexpr uint32_t kChunkGroups = 64;
+
+__host__ __device__ inline uint32_t chunk_count(uint32_t N) { return ((N >> 2) + kChunkGroups - 1) / kChunkGroups; }
+
+__device__ inline uint32_t *chunk_flag(const Params &p, uint32_t *lat, uint32_t k, uint32_t g)
+{
+    return lat + (size_t)p.W * p.N + (size_t)(k - 1) * chunk_count(p.N) + g / kChunkGroups;
+}
+
+// Clears the flag of group g's chunk in plane k >= 1 after a nonzero write.
+__device__ inline void clear_chunk_flag(const Params &p, uint32_t *lat, uint32_t k, uint32_t g)
+{
+    if (!p.plane_flags) return;
+    uint32_t *f = chunk_flag(p, lat, k, g);
+    if (__hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u)
+        __hip_atomic_store(f, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // One particle per 32-bit word (the uint32 lattice of IntMpdRdmeSolver):

@@ -104,6 +104,11 @@ struct lmx_solver {
     // slots.  The kernels keep this; a lattice written from outside (native
     // words, device pointers) may break it until the next step.
     bool packed = true;
+    // The chunk flags of both buffers are valid (kept by every kernel since
+    // they were last zeroed); cleared by host writes and by steps whose
+    // kernels do not keep them.
+    bool flags_fresh = false;
+    size_t flag_words = 0;  // per buffer, after the planes
     size_t N = 0;
     double lambda = 0.0;
     std::vector<double> DF;
@@ -222,6 +227,7 @@ lmx::Params make_params(const lmx_solver *s)
     p.P = s->P; p.W = s->W;
     p.word_particles = s->bpp == 4 ? 1u : 0u;
     p.packed = s->packed ? 1u : 0u;
+    p.plane_flags = 0u;
     p.pcb = s->P <= 4 ? 2u : (s->P <= 8 ? 3u : 4u);  // MPD_PARTICLE_COUNT_BITS
     p.nS = s->nS; p.nST = s->nST; p.R = s->R;
     p.boundary = (uint32_t)s->boundary;
@@ -370,11 +376,29 @@ int enqueue_step(lmx_solver *s, const lmx::Params &p0, hipEvent_t *ev, int *kind
         n++;
         return LMX_OK;
     };
+    const bool fast = !s->force_generic && lmx::fast_path_ok(p);
+    // Chunk flags: kept by the default sweeps-path kernels of the byte
+    // lattice (x4, LDS tiles, the reaction phases, the resolver); any other
+    // step leaves them stale, so they are zeroed before the next step that
+    // uses them.
+    const bool flags_on = fast && s->path == LMX_PATH_SWEEPS && s->yz_kernel == 0 && s->flag_words > 0;
+    if (flags_on) {
+        if (!s->flags_fresh) {
+            for (int b = 0; b < 2; b++) {
+                e = hipMemsetAsync(s->d_lat[b] + s->N * s->W, 0, s->flag_words * sizeof(uint32_t), s->stream);
+                if (e != hipSuccess) return hip_fail(e, "hipMemsetAsync");
+            }
+            s->flags_fresh = true;
+        }
+        p.plane_flags = 1u;
+    } else {
+        p.plane_flags = 0u;
+        s->flags_fresh = false;
+    }
     if (ev) {
         e = hipEventRecord(ev[0], s->stream);
         if (e != hipSuccess) return hip_fail(e, "hipEventRecord");
     }
-    const bool fast = !s->force_generic && lmx::fast_path_ok(p);
     if (fast && s->path == LMX_PATH_FUSED && lmx::fused_ok(p)) {
         e = lmx::launch_fused(p, s->d_lat[s->cur], s->d_lat[1 - s->cur], fused_zseg(s), s->stream);
         if (e != hipSuccess) return hip_fail(e, "fused step launch");
@@ -657,8 +681,12 @@ int lmx_build_diffusion_model(lmx_solver *s, uint32_t nST, const double *DF, con
 
         free_device(s);
         const size_t lat_bytes = s->N * s->W * sizeof(uint32_t);
-        hipError_t e = hipMalloc((void **)&s->d_lat[0], lat_bytes);
-        if (e == hipSuccess) e = hipMalloc((void **)&s->d_lat[1], lat_bytes);
+        // the byte lattice's chunk flags follow the planes of each buffer
+        s->flag_words = (s->bpp == 1 && s->W > 1) ? (size_t)(s->W - 1) * lmx::chunk_count((uint32_t)s->N) : 0;
+        s->flags_fresh = false;
+        const size_t buf_bytes = lat_bytes + s->flag_words * sizeof(uint32_t);
+        hipError_t e = hipMalloc((void **)&s->d_lat[0], buf_bytes);
+        if (e == hipSuccess) e = hipMalloc((void **)&s->d_lat[1], buf_bytes);
         if (e == hipSuccess) e = hipMalloc((void **)&s->d_sites, s->N);
         if (e == hipSuccess && s->nS > s->counts_cap) {
             hipFree(s->d_counts);
@@ -892,6 +920,7 @@ int lmx_set_lattice_native(lmx_solver *s, const uint32_t *native, size_t words)
     LMX_HIP(hipStreamSynchronize(s->stream));
     LMX_HIP(hipMemcpy(s->d_lat[s->cur], native, words * sizeof(uint32_t), hipMemcpyHostToDevice));
     s->packed = packed;
+    s->flags_fresh = false;
     return LMX_OK;
 }
 
@@ -905,6 +934,7 @@ int lmx_device_lattice(lmx_solver *s, void **particles, void **sites)
     *particles = s->d_lat[s->cur];
     *sites = s->d_sites;
     s->packed = false;  // the caller may write any slot
+    s->flags_fresh = false;
     return LMX_OK;
 }
 
@@ -930,6 +960,7 @@ int lmx_set_lattice(lmx_solver *s, const uint8_t *rm, size_t size)
         LMX_HIP(hipStreamSynchronize(s->stream));
         LMX_HIP(hipMemcpy(s->d_lat[s->cur], native.data(), native.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
         s->packed = true;  // each site's particles inserted at its first empty slot
+        s->flags_fresh = false;
         return LMX_OK;
     });
 }
@@ -1202,6 +1233,14 @@ int batch_prepare(lmx_batch *b)
         LMX_HIP(hipStreamSynchronize(s->stream));
         lmx::Member &m = b->h_members[r];
         m.p = make_params(s);
+        if (s->flag_words > 0) {
+            if (!s->flags_fresh) {
+                for (int bb = 0; bb < 2; bb++)
+                    LMX_HIP(hipMemset(s->d_lat[bb] + s->N * s->W, 0, s->flag_words * sizeof(uint32_t)));
+                s->flags_fresh = true;
+            }
+            m.p.plane_flags = 1u;  // the batched kernels are the default ones
+        }
         m.lat[0] = s->d_lat[0];
         m.lat[1] = s->d_lat[1];
         m.cur = (uint32_t)s->cur;

@@ -92,6 +92,40 @@ __device__ inline void load_group(const Params &p, const uint32_t *__restrict__ 
 }
 
 // Words of one site, plane k >= 1 read as in load_group.
+// Stores a group's W planes from a sweep wave.  With chunk flags, a zero word
+// of plane k >= 1 is not written if its chunk was known zero; whole_chunk
+// (the wave's 64 lanes hold the 64 groups of one aligned chunk, inactive lanes
+// only beyond the lattice) lets the wave also set the flag after writing only
+// zeros.  Every lane of the wave that holds a group must call it.
+template <int W>
+__device__ inline void store_group_sweep(const Params &p, uint32_t *__restrict__ dst, uint32_t g, const uint4 *v,
+                                         bool whole_chunk)
+{
+    reinterpret_cast<uint4 *>(dst)[g] = v[0];
+#pragma unroll
+    for (int k = 1; k < W; k++) {
+        const bool nz = (v[k].x | v[k].y | v[k].z | v[k].w) != 0u;
+        if (!p.plane_flags) {
+            reinterpret_cast<uint4 *>(dst + (size_t)k * p.N)[g] = v[k];
+            continue;
+        }
+        uint32_t *f = chunk_flag(p, dst, k, g);
+        const uint32_t known = __hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (nz || known == 0u) reinterpret_cast<uint4 *>(dst + (size_t)k * p.N)[g] = v[k];
+        if (whole_chunk) {
+            const unsigned long long act = __ballot(1);
+            const bool any_nz = __ballot(nz) != 0ull;
+            const uint32_t lane = __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+            if (lane == (uint32_t)__builtin_ctzll(act)) {
+                if (any_nz && known != 0u) __hip_atomic_store(f, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if (!any_nz && known == 0u) __hip_atomic_store(f, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            }
+        } else if (nz && known != 0u) {
+            __hip_atomic_store(f, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+
 template <int W>
 __device__ inline void load_site(const Params &p, const uint32_t *__restrict__ src, uint32_t i, uint32_t *w)
 {
@@ -400,16 +434,11 @@ __device__ inline void x4_body(const Params &p, const uint32_t *__restrict__ src
     if (!has_m) lplus = 0;
     if (!has_p) rminus = 0;
 
-    uint4 *out4[W];
-#pragma unroll
-    for (int k = 0; k < W; k++) out4[k] = reinterpret_cast<uint4 *>(dst + (size_t)k * p.N) + g;
-    uint32_t any = (uint32_t)(occ != 0) | lplus | rminus;
-    if (any == 0) {
-#pragma unroll
-        for (int k = 0; k < W; k++) *out4[k] = make_uint4(0u, 0u, 0u, 0u);
-        return;
-    }
     uint4 res[W];
+#pragma unroll
+    for (int k = 0; k < W; k++) res[k] = make_uint4(0u, 0u, 0u, 0u);
+    const uint32_t any = (uint32_t)(occ != 0) | lplus | rminus;
+    if (any != 0)
 #pragma unroll
     for (int s = 0; s < 4; s++) {
         uint32_t wo[W], wm[W], wp[W], o[W];
@@ -437,8 +466,8 @@ __device__ inline void x4_body(const Params &p, const uint32_t *__restrict__ src
 #pragma unroll
         for (int k = 0; k < W; k++) set_lane(res[k], s, o[k]);
     }
-#pragma unroll
-    for (int k = 0; k < W; k++) *out4[k] = res[k];
+    // A wave holds 64 consecutive groups starting at a multiple of 64.
+    store_group_sweep<W>(p, dst, g, res, true);
     (void)P;
 }
 
@@ -885,8 +914,8 @@ __device__ inline void tile_body(const Params &p, const uint32_t *__restrict__ s
                 for (int k = 0; k < W; k++) set_lane(res[k], s, ov[k]);
             }
         }
-#pragma unroll
-        for (int k = 0; k < W; k++) reinterpret_cast<uint4 *>(dst + (size_t)k * p.N + i0)[0] = res[k];
+        // With X % 256 == 0 a wave is one aligned chunk of a row.
+        store_group_sweep<W>(p, dst, i0 >> 2, res, Gx == kChunkGroups && (p.X & 255u) == 0u);
     }
     if (REACT) {
         // The output sites that hold particles go to the reaction list with
@@ -1281,7 +1310,11 @@ __device__ inline void reaction4_body(const Params &p, uint32_t *__restrict__ la
         if (changed & (1u << q)) {
             const uint32_t g = g0 + 256u * q;
 #pragma unroll
-            for (int k = 0; k < W; k++) reinterpret_cast<uint4 *>(lat + (size_t)k * p.N)[g] = w4[q][k];
+            for (int k = 0; k < W; k++) {
+                reinterpret_cast<uint4 *>(lat + (size_t)k * p.N)[g] = w4[q][k];
+                if (k > 0 && (w4[q][k].x | w4[q][k].y | w4[q][k].z | w4[q][k].w) != 0u)
+                    clear_chunk_flag(p, lat, k, g);
+            }
         }
     }
 }
@@ -1326,7 +1359,10 @@ __global__ __launch_bounds__(256) void reaction_list_kernel(Params p, uint32_t *
             for (int q = 0; q < W; q++) w[q] = rl.words[(size_t)q * total + e];
             if (react_site_table<W>(p, i, rl.type[e], w, h3, tab)) {
 #pragma unroll
-                for (int q = 0; q < W; q++) lat[(size_t)q * p.N + i] = w[q];
+                for (int q = 0; q < W; q++) {
+                    lat[(size_t)q * p.N + i] = w[q];
+                    if (q > 0 && w[q] != 0u) clear_chunk_flag(p, lat, q, i >> 2);
+                }
             }
         }
     }

@@ -238,6 +238,7 @@ __device__ void resolve_body(const Params &p, uint32_t *lat)
                         uint32_t *wp = lat + (size_t)(occ >> 2) * p.N + s2;
                         const uint32_t old = load_agent(wp);
                         store_agent(wp, old | (particle << (8 * (occ & 3))));
+                        if ((occ >> 2) > 0) clear_chunk_flag(p, lat, occ >> 2, s2 >> 2);
                     }
                 }
             }
