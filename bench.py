@@ -350,7 +350,10 @@ def main() -> int:
     roofline = {"bound": "hbm", "kernel": dom, "achieved": dk["GBps"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": dk["GBps"] / HBM_PEAK_GBS, "traffic": traffic,
                 "alg_bytes_per_launch": dk["alg_bytes"], "avg_launch_us": dk["avg_us"],
-                "copy_peak_measured": copy_gbps.value, "frac_of_copy_peak": dk["GBps"] / copy_gbps.value}
+                "copy_peak_measured": copy_gbps.value, "frac_of_copy_peak": dk["GBps"] / copy_gbps.value,
+                # HBM bytes actually moved (rocprofv3 FETCH/WRITE) per launch time: below the
+                # algorithmic rate where the kernels skip the zero upper planes of packed sites
+                "traffic_GBps": (traffic / (dk["avg_us"] / 1e6) / 1e9) if traffic else None}
     # Bytes of the step as run: three sweeps plus the reaction list (written
     # and read once), or plus the full reaction read.
     unfused_alg = (3 * sweep_bytes_per_site(P) + (reaction_bytes_per_site(P) if spec.R else 0)) * N
