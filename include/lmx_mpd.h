@@ -167,7 +167,8 @@ int lmx_set_lattice_native(lmx_solver *s, const uint32_t *native, size_t words);
 int lmx_set_sites(lmx_solver *s, const uint8_t *rowMajor, size_t size);
 /* Device-side hooks (the host copies of copyFromGPU/copyToGPU around
  * hookSimulation, MpdRdmeSolver.cu:958-985, without the copies): the current
- * device particle planes (W planes of N uint32, the native layout) and site
+ * device particle planes (W planes of N uint32, the native layout; the
+ * library's own bookkeeping words follow them and are not to be written) and site
  * types (N bytes, index x + X*y + X*Y*z), after the solver's stream has
  * drained.  They stay valid until the next lmx_run_steps or lmx_set_*; the
  * caller finishes its own device work on them before the next step.  The
