@@ -39,6 +39,7 @@ hipError_t launch_batch_sweeps(const Member *m, const uint32_t *start_x, const u
                                uint32_t W, hipStream_t st);
 hipError_t launch_batch_resolve(Member *m, uint32_t n, hipStream_t st);
 hipError_t launch_copy(const void *src, void *dst, size_t bytes, int num_cu, hipStream_t st);
+hipError_t launch_fill(uint32_t *ptr, size_t n, uint32_t value, hipStream_t st);
 hipError_t launch_validate(const Params &p, const uint32_t *lat, hipStream_t st);
 hipError_t launch_counts(const uint32_t *lat, size_t nwords, uint32_t nS, bool words, uint32_t *counts,
                          hipStream_t st);

@@ -385,8 +385,8 @@ int enqueue_step(lmx_solver *s, const lmx::Params &p0, hipEvent_t *ev, int *kind
     if (flags_on) {
         if (!s->flags_fresh) {
             for (int b = 0; b < 2; b++) {
-                e = hipMemsetAsync(s->d_lat[b] + s->N * s->W, 0, s->flag_words * sizeof(uint32_t), s->stream);
-                if (e != hipSuccess) return hip_fail(e, "hipMemsetAsync");
+                e = lmx::launch_fill(s->d_lat[b] + s->N * s->W, s->flag_words, 0u, s->stream);
+                if (e != hipSuccess) return hip_fail(e, "flag fill launch");
             }
             s->flags_fresh = true;
         }
@@ -985,7 +985,7 @@ int lmx_get_species_counts(lmx_solver *s, uint32_t *counts, uint32_t n)
         int rc = require_ready(s);
         if (rc) return rc;
         if (!counts || n < s->nS) return fail(LMX_ERR_ARG, "counts buffer smaller than the number of species");
-        LMX_HIP(hipMemsetAsync(s->d_counts, 0, s->counts_cap * sizeof(uint32_t), s->stream));
+        LMX_HIP(lmx::launch_fill(s->d_counts, s->counts_cap, 0u, s->stream));
         hipError_t e = lmx::launch_counts(s->d_lat[s->cur], s->N * s->W, s->nS, s->bpp == 4, s->d_counts, s->stream);
         if (e != hipSuccess) return hip_fail(e, "count launch");
         LMX_HIP(hipStreamSynchronize(s->stream));

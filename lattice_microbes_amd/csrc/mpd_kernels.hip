@@ -357,6 +357,22 @@ hipError_t launch_copy(const void *src, void *dst, size_t bytes, int num_cu, hip
     return hipGetLastError();
 }
 
+// Fills n words with a value (used instead of a memset node, so that work
+// enqueued on a stream someone captures holds kernels only).
+__global__ __launch_bounds__(256) void fill_kernel(uint32_t *__restrict__ ptr, size_t n, uint32_t value)
+{
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) ptr[i] = value;
+}
+
+hipError_t launch_fill(uint32_t *ptr, size_t n, uint32_t value, hipStream_t st)
+{
+    if (n == 0) return hipSuccess;
+    const size_t nb = std::min<size_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(fill_kernel, dim3((unsigned)nb), dim3(256), 0, st, ptr, n, value);
+    return hipGetLastError();
+}
+
 __global__ void exp_kernel(const double *x, double *y, size_t n)
 {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
