@@ -433,3 +433,35 @@ def test_upper_planes_filling(gpu, words):
         filled |= bool(got[1:].any())
     assert filled and W > 1
     solver.close()
+
+
+@pytest.mark.parametrize("other", ["march", "fused", "generic"])
+def test_option_switches_mid_run(gpu, other):
+    """Steps by the default kernels (which keep the upper planes' chunk
+    flags), then by kernels that do not, then by the default ones again: the
+    flags left stale in between are reset, and every phase equals the
+    oracle."""
+    spec = models.random_model(shape=(64, 8, 8), P=8, nS=3, nST=2, R=4, density=0.4, seed=14, qmax=0.5)
+    solver = lmx.MpdRdmeSolver(0)
+    spec.load_into(solver, seed=17)
+    orc = oracle.Oracle(spec, seed_top=17)
+
+    def phase(n):
+        solver.runTimesteps(n)
+        assert orc.run(n) == 0
+        np.testing.assert_array_equal(solver.getLatticeNative(), orc.native())
+
+    phase(3)
+    if other == "march":
+        solver.setLaunchOption(lmx._lib.OPT_YZ_KERNEL, 1)
+    elif other == "fused":
+        solver.setPath(lmx._lib.PATH_FUSED)
+    else:
+        solver.setLaunchOption(lmx._lib.OPT_FORCE_GENERIC, 1)
+    phase(3)
+    solver.setLaunchOption(lmx._lib.OPT_YZ_KERNEL, 0)
+    solver.setPath(lmx._lib.PATH_SWEEPS)
+    solver.setLaunchOption(lmx._lib.OPT_FORCE_GENERIC, 0)
+    phase(4)
+    assert solver.getLatticeNative()[1].any()    # the second plane was in use
+    solver.close()
