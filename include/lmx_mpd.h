@@ -73,11 +73,22 @@ int lmx_create(int device, lmx_solver **out);
 /* MpdRdmeSolver::~MpdRdmeSolver (MpdRdmeSolver.cu:164-190). */
 int lmx_destroy(lmx_solver *s);
 
+/* "rdme.mpd.overflowhandler" (MpdRdmeSolver.cu:126-146). */
+enum lmx_overflow_mode {
+    /* "classic" or "" (the default): each entry into the first site with room
+     * among the nearby sites of the same type (:1103-1189, Lattice.cpp:106-128) */
+    LMX_OVERFLOW_CLASSIC = 0,
+    /* "relaxed": each entry into its own site's first empty slot after the
+     * step; an entry whose site is full stays listed and is retried after the
+     * next step (correct_overflows, :1677-1761, launched at :1086-1101) */
+    LMX_OVERFLOW_RELAXED = 1
+};
+
 /* MpdRdmeSolver::initialize (MpdRdmeSolver.cu:107-162): parameters "seed",
- * "timestep" (tau, seconds) and "rdme.mpd.overflowhandler" (0 = classic; the
- * relaxed handler is not offered, see its launch at :1087).  seedTop == 0
- * draws the seed from the clock as the reference does (:113-122).
- * seed32 = (seedTop<<16) | (replicate & 0xFFFF) (:123). */
+ * "timestep" (tau, seconds) and "rdme.mpd.overflowhandler" (overflow_mode,
+ * enum lmx_overflow_mode).  seedTop == 0 draws the seed from the clock as the
+ * reference does (:113-122).  seed32 = (seedTop<<16) | (replicate & 0xFFFF)
+ * (:123). */
 int lmx_initialize(lmx_solver *s, uint32_t replicate, uint32_t seedTop, double timestep,
                    int overflow_mode, int boundary);
 /* The seed actually used (after a clock seed). */
@@ -188,6 +199,12 @@ int lmx_get_species_counts(lmx_solver *s, uint32_t *counts, uint32_t n);
 /* Overflow statistics (overflowTimesteps/overflowListUses, :1104-1187):
  * total entries resolved and number of steps that had any. */
 int lmx_get_overflow_stats(lmx_solver *s, uint64_t *entries, uint64_t *steps);
+/* Relaxed handler: entries left unresolved after the last step, whose own
+ * site was full (the reference's "%d unresolved overflows" warning,
+ * MpdRdmeSolver.cu:1094-1101).  They stay listed and count against the list's
+ * 2048 entries: more than that at the end of a step is LMX_ERR_OVERFLOW_CAP.
+ * Always 0 with the classic handler. */
+int lmx_get_unresolved_overflows(lmx_solver *s, uint32_t *count);
 
 /* Measurement (no reference counterpart; replaces the lptf PROF_CUDA_* events,
  * MpdRdmeSolver.cu:1014-1083).  Runs nsteps steps bracketed by HIP events on

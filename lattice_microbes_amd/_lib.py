@@ -26,6 +26,8 @@ ERR_INVALID_LATTICE = 9
 
 BOUNDARY_PERIODIC = 0
 BOUNDARY_REFLECTING = 1
+OVERFLOW_CLASSIC = 0
+OVERFLOW_RELAXED = 1
 
 OPT_MARCH_SEGMENT = 1
 OPT_FORCE_GENERIC = 2
@@ -77,6 +79,7 @@ SIGNATURES = {
     "lmx_copy_bandwidth": (c_int, [c_int, c_size_t, c_int, ctypes.POINTER(c_double)]),
     "lmx_get_species_counts": (c_int, [c_void_p, _u32p, c_uint32]),
     "lmx_get_overflow_stats": (c_int, [c_void_p, _u64p, _u64p]),
+    "lmx_get_unresolved_overflows": (c_int, [c_void_p, _u32p]),
     "lmx_time_steps": (c_int, [c_void_p, c_uint32, _dp, _dp, _u64p, c_uint32]),
     "lmx_batch_create": (c_int, [c_int, POINTER(c_void_p)]),
     "lmx_batch_destroy": (c_int, [c_void_p]),

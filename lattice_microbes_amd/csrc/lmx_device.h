@@ -29,8 +29,14 @@ struct Control {
     uint32_t error_step;        // timestep of the first error
     unsigned long long ovf_total;
     unsigned long long ovf_steps;
+    // Relaxed handler: entries whose own site was full at the end of a step,
+    // in list order, retried after the next step before its own entries.
+    uint32_t carry_count;
+    uint32_t carry_pad;
     unsigned long long ovf_keys[kMaxOverflows];
     uint32_t ovf_part[kMaxOverflows];   // the particle of list entry k
+    uint32_t carry_site[kMaxOverflows];
+    uint32_t carry_part[kMaxOverflows];
     // The reaction list a fused z sweep fills: entries per shard (may exceed
     // the capacity) and whether any shard overflowed.  The resolver clears
     // both at the end of each step.
@@ -69,6 +75,7 @@ struct Params {
     uint32_t packed;            // 1: every site of the step's input holds its particles in slots
                                 // 0..n-1, so a word after one whose last slot is empty is zero
     uint32_t plane_flags;       // 1: the kernels keep the chunk flags of planes k >= 1 (below)
+    uint32_t relaxed;           // overflow handler: 0 classic (nearby sites), 1 relaxed (own site)
     uint32_t nS, nST, R;
     uint32_t boundary;          // 0 periodic, 1 reflecting
     uint32_t seed32;

@@ -228,6 +228,7 @@ lmx::Params make_params(const lmx_solver *s)
     p.word_particles = s->bpp == 4 ? 1u : 0u;
     p.packed = s->packed ? 1u : 0u;
     p.plane_flags = 0u;
+    p.relaxed = s->overflow_mode == LMX_OVERFLOW_RELAXED ? 1u : 0u;
     p.pcb = s->P <= 4 ? 2u : (s->P <= 8 ? 3u : 4u);  // MPD_PARTICLE_COUNT_BITS
     p.nS = s->nS; p.nST = s->nST; p.R = s->R;
     p.boundary = (uint32_t)s->boundary;
@@ -543,7 +544,8 @@ int lmx_initialize(lmx_solver *s, uint32_t replicate, uint32_t seedTop, double t
 {
     if (!s) return fail(LMX_ERR_ARG, "null solver");
     if (!(timestep > 0.0)) return fail(LMX_ERR_ARG, "timestep: A positive timestep must be specified for the solver.");
-    if (overflow_mode != 0) return fail(LMX_ERR_ARG, "rdme.mpd.overflowhandler: only the classic handler is supported");
+    if (overflow_mode != LMX_OVERFLOW_CLASSIC && overflow_mode != LMX_OVERFLOW_RELAXED)
+        return fail(LMX_ERR_ARG, "rdme.mpd.overflowhandler: unknown handler");
     if (boundary != LMX_BOUNDARY_PERIODIC && boundary != LMX_BOUNDARY_REFLECTING)
         return fail(LMX_ERR_ARG, "unknown boundary mode");
     if (seedTop == 0) {
@@ -1006,6 +1008,18 @@ int lmx_get_overflow_stats(lmx_solver *s, uint64_t *entries, uint64_t *steps)
     LMX_HIP(hipMemcpy(&c, s->d_ctl, offsetof(lmx::Control, ovf_keys), hipMemcpyDeviceToHost));
     if (entries) *entries = c.ovf_total;
     if (steps) *steps = c.ovf_steps;
+    return LMX_OK;
+}
+
+int lmx_get_unresolved_overflows(lmx_solver *s, uint32_t *count)
+{
+    if (!s || !count) return fail(LMX_ERR_ARG, "null argument");
+    int rc = select_device(s);
+    if (rc) return rc;
+    LMX_HIP(hipStreamSynchronize(s->stream));
+    lmx::Control c;
+    LMX_HIP(hipMemcpy(&c, s->d_ctl, offsetof(lmx::Control, ovf_keys), hipMemcpyDeviceToHost));
+    *count = c.carry_count;
     return LMX_OK;
 }
 

@@ -5,7 +5,8 @@
 //  reaction_kernel  per-site reaction, one site per thread, any shape
 //                   (precomp_reaction_kernel, MpdRdmeSolver.cu:1592-1675)
 //  resolve_kernel   canonical-order overflow placement on the device (the
-//                   classic host handler, MpdRdmeSolver.cu:1103-1189)
+//                   classic host handler, MpdRdmeSolver.cu:1103-1189, or the
+//                   relaxed correct_overflows, :1677-1761)
 //  count_kernel     species counts (ByteLattice::getParticleCounts; the
 //                   uint32 lattice's in count_words_kernel)
 //
@@ -120,6 +121,19 @@ __device__ inline uint32_t site_occupancy(const Params &p, const uint32_t *lat, 
     return occ;
 }
 
+// A particle into slot occ (the site's first empty slot) of site s.
+__device__ inline void place_particle(const Params &p, uint32_t *lat, uint32_t s, uint32_t occ, uint32_t particle)
+{
+    if (p.word_particles) {
+        store_agent(lat + (size_t)occ * p.N + s, particle);
+    } else {
+        uint32_t *wp = lat + (size_t)(occ >> 2) * p.N + s;
+        const uint32_t old = load_agent(wp);
+        store_agent(wp, old | (particle << (8 * (occ & 3))));
+        if ((occ >> 2) > 0) clear_chunk_flag(p, lat, occ >> 2, s >> 2);
+    }
+}
+
 __device__ inline void set_error(Control *ctl, uint32_t code, uint32_t step)
 {
     if (atomicCAS(&ctl->error, 0u, code) == 0u) ctl->error_step = step;
@@ -141,11 +155,12 @@ __device__ void resolve_body(const Params &p, uint32_t *lat)
     for (int k = tid; k < kListShards; k += (int)blockDim.x) ctl->list_count[k] = 0u;
     if (tid == 0) ctl->list_overflow = 0u;
     const uint32_t t = s_t;
-    if (n == 0) {
+    const uint32_t carried = p.relaxed ? ctl->carry_count : 0u;
+    if (n == 0 && carried == 0) {
         if (tid == 0) ctl->timestep = t + 1;
         return;
     }
-    if (n > kMaxOverflows) {
+    if (n > kMaxOverflows || carried + n > kMaxOverflows) {
         if (tid == 0) {
             set_error(ctl, kErrOverflowCap, t);
             ctl->ovf_count = 0;
@@ -172,6 +187,42 @@ __device__ void resolve_body(const Params &p, uint32_t *lat)
             }
             __syncthreads();
         }
+    }
+
+    if (p.relaxed) {
+        // Relaxed handler (correct_overflows, MpdRdmeSolver.cu:1677-1761): the
+        // carried entries, then this step's in canonical order, each into its
+        // own site's first empty slot; entries of one site go in that order
+        // (the kernel's rounds) and different sites do not interact, so one
+        // thread walks the list.  A full site keeps the entry for the next step.
+        if (tid == 0) {
+            uint32_t kept = 0;
+            for (uint32_t e = 0; e < carried + n; e++) {
+                uint32_t site, particle;
+                if (e < carried) {
+                    site = ctl->carry_site[e];
+                    particle = ctl->carry_part[e];
+                } else {
+                    const unsigned long long key = keys[e - carried];
+                    site = (uint32_t)((key >> 24) & 0xFFFFFFFFull);
+                    particle = ctl->ovf_part[key & 0x7FFull];
+                }
+                const uint32_t occ = site_occupancy(p, lat, site);
+                if (occ < p.P) {
+                    place_particle(p, lat, site, occ, particle);
+                } else {
+                    ctl->carry_site[kept] = site;  // kept <= e: entry e has been read
+                    ctl->carry_part[kept] = particle;
+                    kept++;
+                }
+            }
+            ctl->carry_count = kept;
+            ctl->ovf_total += n;
+            if (n > 0) ctl->ovf_steps += 1;
+            ctl->ovf_count = 0;
+            ctl->timestep = t + 1;
+        }
+        return;
     }
 
     const uint32_t X = p.X, Y = p.Y, Z = p.Z;
@@ -231,15 +282,7 @@ __device__ void resolve_body(const Params &p, uint32_t *lat)
                     const uint32_t y = y0 + rem / nz;
                     const uint32_t z = z0 + rem % nz;
                     const uint32_t s2 = x + X * y + p.XY * z;
-                    const uint32_t occ = site_occupancy(p, lat, s2);
-                    if (p.word_particles) {
-                        store_agent(lat + (size_t)occ * p.N + s2, particle);
-                    } else {
-                        uint32_t *wp = lat + (size_t)(occ >> 2) * p.N + s2;
-                        const uint32_t old = load_agent(wp);
-                        store_agent(wp, old | (particle << (8 * (occ & 3))));
-                        if ((occ >> 2) > 0) clear_chunk_flag(p, lat, occ >> 2, s2 >> 2);
-                    }
+                    place_particle(p, lat, s2, site_occupancy(p, lat, s2), particle);
                 }
             }
         }

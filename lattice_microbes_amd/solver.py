@@ -10,6 +10,7 @@ the step.
 from __future__ import annotations
 
 import ctypes
+import warnings
 from ctypes import byref, c_double, c_size_t, c_uint32, c_uint64, c_void_p
 from typing import Callable, Dict, List, Optional
 
@@ -43,6 +44,10 @@ class MpdRdmeSolver:
     # The byte lattice: one byte per particle, at most 255 species
     # (MpdRdmeSolver::allocateLattice, MpdRdmeSolver.cu:189-199).
     bytesPerParticle = 1
+    # MpdRdmeSolver reads rdme.mpd.overflowhandler ("classic", "relaxed");
+    # IntMpdRdmeSolver has only the classic handler and ignores it.
+    _parses_overflow_handler = True
+    overflowHandler = "classic"
 
     def __init__(self, device: int = 0):
         self._lib = _lib.load()
@@ -99,11 +104,16 @@ class MpdRdmeSolver:
         seed_top = int(float(self.parameters.get("seed", "0") or 0)) & 0xFFFFFFFF
         tau = float(self.parameters.get("timestep", "0") or 0)
         handler = self.parameters.get("rdme.mpd.overflowhandler", "")
-        if handler not in ("", "classic"):
-            raise _lib.InvalidArgException(_lib.ERR_ARG,
-                                           f"rdme.mpd.overflowhandler '{handler}' is not supported")
+        overflow_mode = _lib.OVERFLOW_CLASSIC
+        if self._parses_overflow_handler:
+            if handler == "relaxed":
+                overflow_mode = _lib.OVERFLOW_RELAXED
+            elif handler not in ("", "classic"):
+                # the reference warns and keeps the classic handler (MpdRdmeSolver.cu:142-146)
+                warnings.warn(f"Unknown overflow handler requested: '{handler}'")
+        self.overflowHandler = "relaxed" if overflow_mode == _lib.OVERFLOW_RELAXED else "classic"
         bmode = {"periodic": _lib.BOUNDARY_PERIODIC, "reflecting": _lib.BOUNDARY_REFLECTING}[boundary]
-        _lib.check(self._lib.lmx_initialize(self._h, self.replicate, seed_top, tau, 0, bmode))
+        _lib.check(self._lib.lmx_initialize(self._h, self.replicate, seed_top, tau, overflow_mode, bmode))
         self.tau = tau
 
     @property
@@ -270,6 +280,13 @@ class MpdRdmeSolver:
         _lib.check(self._lib.lmx_get_overflow_stats(self._h, byref(e), byref(s)))
         return e.value, s.value
 
+    def getUnresolvedOverflows(self) -> int:
+        """Relaxed handler: overflow entries whose own site was full after the
+        last step; they are retried after the next one (MpdRdmeSolver.cu:1094-1101)."""
+        n = c_uint32()
+        _lib.check(self._lib.lmx_get_unresolved_overflows(self._h, byref(n)))
+        return int(n.value)
+
     def getTable(self, which: str) -> np.ndarray:
         idx = {"T": 0, "qp0": 1, "qp1": 2, "qp2": 3, "rate": 4}[which]
         n = c_size_t()
@@ -373,6 +390,10 @@ class MpdRdmeSolver:
             self.runTimesteps(stop - t, synchronize=True)
             t = stop
             time = t * self.tau
+            if self.overflowHandler == "relaxed":
+                unresolved = self.getUnresolvedOverflows()
+                if unresolved > 0:
+                    warnings.warn(f"{unresolved} unresolved overflows")
             # The lattice comes to the host for a host hook or a lattice frame;
             # counts alone are taken on the GPU (the same numbers as counting
             # the host copy, which then equals the device lattice).
@@ -649,6 +670,7 @@ class IntMpdRdmeSolver(MpdRdmeSolver):
     frames are written with 4 bytes per particle."""
 
     bytesPerParticle = 4
+    _parses_overflow_handler = False
 
 
 class MESolverFactory:

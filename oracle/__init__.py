@@ -99,6 +99,11 @@ def load() -> ctypes.CDLL:
     lib.orc_react.argtypes = [POINTER(OrcModel), u32p, u8p, c_uint64, u32p, u32p, u32p, POINTER(c_uint64)]
     lib.orc_resolve_overflows.restype = c_int
     lib.orc_resolve_overflows.argtypes = [POINTER(OrcModel), u32p, u8p, u32p, u32p, c_uint32]
+    lib.orc_resolve_relaxed.restype = c_int
+    lib.orc_resolve_relaxed.argtypes = [POINTER(OrcModel), u32p, u32p, u32p, c_uint32, u32p, u32p, u32p]
+    lib.orc_run_handler.restype = c_int
+    lib.orc_run_handler.argtypes = [POINTER(OrcModel), u32p, u8p, c_uint32, c_uint32, c_int, c_int, u32p, u32p,
+                                    u32p, POINTER(OrcStats)]
     lib.orc_run.restype = c_int
     lib.orc_run.argtypes = [POINTER(OrcModel), u32p, u8p, c_uint32, c_uint32, c_int, POINTER(OrcStats)]
     lib.orc_nsm_create.restype = c_void_p
@@ -140,8 +145,14 @@ class Oracle:
     uint32 spec lattice selects the uint32 lattice (one particle per word,
     IntMpdRdmeSolver); otherwise the byte lattice."""
 
-    def __init__(self, spec, seed_top: int = 1, replicate: int = 0):
+    def __init__(self, spec, seed_top: int = 1, replicate: int = 0, overflow: str = "classic"):
+        if overflow not in ("classic", "relaxed"):
+            raise ValueError(f"overflow handler {overflow!r}")
         lib = load()
+        self.relaxed = overflow == "relaxed"
+        self.carry_site = np.zeros(2048, np.uint32)
+        self.carry_particle = np.zeros(2048, np.uint32)
+        self.carry_count = np.zeros(1, np.uint32)
         self.lib = lib
         self.spec = spec
         X, Y, Z = spec.shape
@@ -227,11 +238,19 @@ class Oracle:
         return int(self.model.seed32)
 
     def run(self, nsteps: int, formulation: int = 0) -> int:
-        rc = self.lib.orc_run(ctypes.byref(self.model), _p(self.lattice, c_uint32), _p(self.sites, c_uint8),
-                              self.timestep, nsteps, formulation, ctypes.byref(self.stats))
+        rc = self.lib.orc_run_handler(ctypes.byref(self.model), _p(self.lattice, c_uint32),
+                                      _p(self.sites, c_uint8), self.timestep, nsteps, formulation,
+                                      int(self.relaxed), _p(self.carry_site, c_uint32),
+                                      _p(self.carry_particle, c_uint32), _p(self.carry_count, c_uint32),
+                                      ctypes.byref(self.stats))
         if rc == 0:
             self.timestep += nsteps
         return rc
+
+    @property
+    def unresolved(self) -> int:
+        """Overflow entries the relaxed handler is carrying to the next step."""
+        return int(self.carry_count[0])
 
     def native(self) -> np.ndarray:
         X, Y, Z = self.shape

@@ -710,6 +710,36 @@ int orc_resolve_overflows(const orc_model *m, uint32_t *lat, const uint8_t *site
     return ORC_OK;
 }
 
+/* Relaxed handler (correct_overflows, MpdRdmeSolver.cu:1677-1761): the list
+ * as the kernel sees it holds the entries left unresolved by earlier steps
+ * first (it rewrites them from index 0, :1745-1749, and the next step's
+ * sweeps append after them), then this step's.  Entries of one site go in
+ * list order (its rounds, :1717-1729), each into the site's first empty slot
+ * (:1733-1742); a full site keeps the entry in the list.  Entries of
+ * different sites do not interact, so list order is the whole schedule.  The
+ * kernel asserts total < TUNE_MPD_MAX_PARTICLE_OVERFLOWS (:1697); here more
+ * than ORC_MAX_OVERFLOWS entries is the list-capacity error. */
+int orc_resolve_relaxed(const orc_model *m, uint32_t *lat, const uint32_t *ovf_site,
+                        const uint32_t *ovf_particle, uint32_t count, uint32_t *carry_site,
+                        uint32_t *carry_particle, uint32_t *carry_count)
+{
+    uint32_t carried = *carry_count;
+    if (count > ORC_MAX_OVERFLOWS || carried + count > ORC_MAX_OVERFLOWS) return ORC_ERR_OVERFLOW_CAP;
+    size_t N = (size_t)m->X * m->Y * m->Z;
+    uint32_t kept = 0;
+    for (uint32_t e = 0; e < carried + count; e++) {
+        uint32_t site = e < carried ? carry_site[e] : ovf_site[e - carried];
+        uint32_t particle = e < carried ? carry_particle[e] : ovf_particle[e - carried];
+        if (!add_particle(m, lat, N, site, particle)) {
+            carry_site[kept] = site;   /* kept <= e: entry e has been read */
+            carry_particle[kept] = particle;
+            kept++;
+        }
+    }
+    *carry_count = kept;
+    return ORC_OK;
+}
+
 /* ------------------------------------------------------------------------ */
 /* Timestep                                                                 */
 /* ------------------------------------------------------------------------ */
@@ -719,6 +749,14 @@ int orc_resolve_overflows(const orc_model *m, uint32_t *lat, const uint8_t *site
 int orc_run(const orc_model *m, uint32_t *lat, const uint8_t *sites, uint32_t t0,
             uint32_t nsteps, int formulation, orc_stats *stats)
 {
+    return orc_run_handler(m, lat, sites, t0, nsteps, formulation, 0, NULL, NULL, NULL, stats);
+}
+
+int orc_run_handler(const orc_model *m, uint32_t *lat, const uint8_t *sites, uint32_t t0,
+                    uint32_t nsteps, int formulation, int relaxed, uint32_t *carry_site,
+                    uint32_t *carry_particle, uint32_t *carry_count, orc_stats *stats)
+{
+    if (relaxed && (!carry_site || !carry_particle || !carry_count)) return ORC_ERR_ARG;
     size_t N = (size_t)m->X * m->Y * m->Z;
     size_t words = N * (word_mode(m) ? m->P : m->P / 4);
     uint32_t *tmp = (uint32_t *)malloc(words * sizeof(uint32_t));
@@ -748,8 +786,12 @@ int orc_run(const orc_model *m, uint32_t *lat, const uint8_t *sites, uint32_t t0
                             ovf_particle, &count, stats ? &stats->reactions : NULL);
             if (err) break;
         }
-        if (count > 0) {
+        if (relaxed && (count > 0 || *carry_count > 0))
+            err = orc_resolve_relaxed(m, lat, ovf_site, ovf_particle, count, carry_site, carry_particle,
+                                      carry_count);
+        else if (count > 0)
             err = orc_resolve_overflows(m, lat, sites, ovf_site, ovf_particle, count);
+        if (count > 0) {
             if (stats) {
                 stats->overflow_entries += count;
                 stats->overflow_steps++;

@@ -135,11 +135,24 @@ int orc_react(const orc_model *m, uint32_t *lat, const uint8_t *sites, uint64_t 
 /* Classic overflow resolution (MpdRdmeSolver.cu:1103-1189, Lattice.cpp:106-128). */
 int orc_resolve_overflows(const orc_model *m, uint32_t *lat, const uint8_t *sites,
                           const uint32_t *ovf_site, const uint32_t *ovf_particle, uint32_t count);
+/* Relaxed overflow resolution (correct_overflows, MpdRdmeSolver.cu:1677-1761):
+ * the entries carried from earlier steps, then this step's, each into the
+ * first empty slot of its own site; an entry whose site is full is carried
+ * to the next step (carry_* hold up to ORC_MAX_OVERFLOWS entries). */
+int orc_resolve_relaxed(const orc_model *m, uint32_t *lat, const uint32_t *ovf_site,
+                        const uint32_t *ovf_particle, uint32_t count, uint32_t *carry_site,
+                        uint32_t *carry_particle, uint32_t *carry_count);
 
 /* Full timesteps t0 .. t0+nsteps-1 (MpdRdmeSolver.cu:996-1193).  lat is the
  * native lattice (W planes of N words), updated in place. */
 int orc_run(const orc_model *m, uint32_t *lat, const uint8_t *sites, uint32_t t0,
             uint32_t nsteps, int formulation, orc_stats *stats);
+/* The same with the overflow handler chosen (rdme.mpd.overflowhandler,
+ * MpdRdmeSolver.cu:126-146): relaxed != 0 resolves with orc_resolve_relaxed
+ * and keeps the unresolved entries in carry_* across steps and calls. */
+int orc_run_handler(const orc_model *m, uint32_t *lat, const uint8_t *sites, uint32_t t0,
+                    uint32_t nsteps, int formulation, int relaxed, uint32_t *carry_site,
+                    uint32_t *carry_particle, uint32_t *carry_count, orc_stats *stats);
 
 #ifdef __cplusplus
 }

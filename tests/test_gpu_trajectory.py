@@ -55,7 +55,7 @@ def test_second_replicate_and_bad_file(tmp_path, capsys):
         assert not np.array_equal(f.getLattice(0, 1), f.getLattice(1, 1))
     # an unsupported parameter is reported the way the reference does, not raised
     bad = str(tmp_path / "bad.lm")
-    spec.to_file(bad, {"seed": SEED, "maxTime": repr(2 * spec.tau), "rdme.mpd.overflowhandler": "other"})
+    spec.to_file(bad, {"seed": SEED, "maxTime": repr(2 * spec.tau), "timestep": "0"})
     assert not runSimulation(bad, 0)
     assert "Exception during execution of replicate" in capsys.readouterr().err
 
