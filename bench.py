@@ -267,6 +267,8 @@ def main() -> int:
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--no-uint32", action="store_true", help="skip the uint32-lattice measurement")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default kernels instead of timing every y/z kernel and window shape first")
     ap.add_argument("--ensemble-replicates", type=int, default=256)
     ap.add_argument("--ensemble-steps", type=int, default=20)
     ap.add_argument("--serving-jobs", type=int, default=24)
@@ -306,11 +308,21 @@ def main() -> int:
     if args.path == "fused":
         solver.setPath(lmx._lib.PATH_FUSED)
     log("model uploaded")
+    tuned = None
+    if args.path == "sweeps" and not args.no_autotune:
+        tuned = solver.autotune(steps=8)
+        log(f"autotune: {tuned}")
     if args.warmup:
         solver.runTimesteps(args.warmup)
     log(f"warmup {args.warmup} steps done")
-    total_ms, per = solver.timeSteps(args.steps, per_kernel=True)
+    # The timed region: K steps back to back, HIP events around the loop only.
+    total_ms, _ = solver.timeSteps(args.steps)
     log(f"timed {args.steps} steps: {total_ms:.2f} ms")
+    # Per-kernel durations from a second run of K steps with HIP events around
+    # every launch (the event records add gaps between kernels, so this run's
+    # total is not the headline).
+    evented_ms, per = solver.timeSteps(args.steps, per_kernel=True)
+    log(f"per-kernel run of {args.steps} steps: {evented_ms:.2f} ms")
     counts = solver.getSpeciesCounts()
     ovf_entries, ovf_steps = solver.getOverflowStats()
     native = solver.getLatticeNative()
@@ -318,7 +330,9 @@ def main() -> int:
 
     N, P, K = spec.N, spec.P, args.steps
     value = N * K / (total_ms / 1e3)
-    listed = args.path == "sweeps" and uses_reaction_list(spec)
+    tile_list = tuned is None or (tuned["path"] == 0 and tuned["yz_kernel"] == 0 and tuned["fuse_reaction"])
+    listed = args.path == "sweeps" and uses_reaction_list(spec) and tile_list
+    path = "fused" if args.path == "fused" or (tuned and tuned["path"] == 1) else "sweeps"
     list_bytes = occupied * list_entry_bytes(P) if listed else 0
     # per-kernel averages and algorithmic bytes per launch (with the reaction
     # list, the z sweep also writes one entry per occupied site and the
@@ -375,7 +389,7 @@ def main() -> int:
         "config": {"workload": f"{spec.name} {spec.shape[0]}x{spec.shape[1]}x{spec.shape[2]}",
                    "sites": N, "slots_per_site": P, "species": spec.nS, "reactions": spec.R,
                    "site_types": spec.nST, "particles": int(spec.initial_counts().sum()),
-                   "lambda_m": spec.spacing, "tau_s": spec.tau, "path": args.path,
+                   "lambda_m": spec.spacing, "tau_s": spec.tau, "path": path,
                    "reaction_phase": "list" if listed else "full pass", "occupied_sites_end": occupied},
         "roofline": roofline,
         "step_roofline": {"alg_bytes_per_step": step_alg, "achieved_GBps": step_alg / (step_ms / 1e3) / 1e9,
@@ -388,6 +402,9 @@ def main() -> int:
         "kernels": kernels,
         "overflow": {"entries": ovf_entries, "steps_with_overflow": ovf_steps},
         "final_particles": int(counts.sum()),
+        "ms_per_step_with_kernel_events": evented_ms / K,
+        # launch shape chosen by lmx_autotune (8 timed steps per candidate) before the warmup
+        "autotune": tuned,
     }
     solver.close()
     if not args.no_uint32 and args.path == "sweeps":

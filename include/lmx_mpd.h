@@ -154,6 +154,23 @@ enum lmx_option {
 };
 int lmx_set_launch_option(lmx_solver *s, int option, int value);
 
+/* Launch-shape autotuner (no reference counterpart: the reference fixes its
+ * block shapes at build time, options.cmake TUNE_MPD_*).  From the current
+ * state it times nsteps steps (after one untimed step) of each y/z sweep
+ * kernel and window shape: LDS row tiles with and without the reaction list,
+ * register march and register columns of 4 and 8 rows, compacted tiles, and
+ * the fused step.  It restores the state after each, times two interleaved
+ * passes and keeps the fastest by the minimum of each, but keeps the current
+ * configuration unless another is at least 2% faster (LMX_OPT_FORCE_GENERIC
+ * is cleared).  Every option gives bit-identical
+ * results, so afterwards the lattice, step counter, overflow statistics and
+ * carried entries are as before the call.  best_ms (optional) receives the
+ * best time per step; chosen (optional, 4 ints) the path, LMX_OPT_YZ_KERNEL,
+ * LMX_OPT_MARCH_SEGMENT and LMX_OPT_FUSE_REACTION kept.  Allocates a copy of
+ * both lattice buffers for the duration of the call; not for ensemble
+ * members (LMX_ERR_STATE). */
+int lmx_autotune(lmx_solver *s, uint32_t nsteps, double *best_ms, int *chosen);
+
 /* runTimestep (MpdRdmeSolver.cu:996-1193) nsteps times, enqueued on the
  * solver's stream without a host synchronization per step (the reference
  * synchronizes every step, :1091).  Device-side errors are sticky and are

@@ -80,6 +80,7 @@ SIGNATURES = {
     "lmx_get_species_counts": (c_int, [c_void_p, _u32p, c_uint32]),
     "lmx_get_overflow_stats": (c_int, [c_void_p, _u64p, _u64p]),
     "lmx_get_unresolved_overflows": (c_int, [c_void_p, _u32p]),
+    "lmx_autotune": (c_int, [c_void_p, c_uint32, _dp, POINTER(c_int)]),
     "lmx_time_steps": (c_int, [c_void_p, c_uint32, _dp, _dp, _u64p, c_uint32]),
     "lmx_batch_create": (c_int, [c_int, POINTER(c_void_p)]),
     "lmx_batch_destroy": (c_int, [c_void_p]),

@@ -1023,6 +1023,127 @@ int lmx_get_unresolved_overflows(lmx_solver *s, uint32_t *count)
     return LMX_OK;
 }
 
+int lmx_autotune(lmx_solver *s, uint32_t nsteps, double *best_ms, int *chosen)
+{
+    return guarded([&]() -> int {
+        int rc = require_ready(s);
+        if (rc) return rc;
+        if (s->batch) return fail(LMX_ERR_STATE, "the solver is an ensemble member; tune it before adding it");
+        if (nsteps == 0 || nsteps > 10000) return fail(LMX_ERR_ARG, "nsteps must be in 1..10000");
+        LMX_HIP(hipStreamSynchronize(s->stream));
+        if (s->tables_dirty && (rc = upload_tables(s))) return rc;
+
+        // Everything a step reads or advances: both buffers (planes and chunk
+        // flags), the control block (step counter, overflow statistics and
+        // carried entries) and the host-side bookkeeping.
+        const size_t buf_bytes = (s->N * s->W + s->flag_words) * sizeof(uint32_t);
+        struct Scratch {
+            void *p = nullptr;
+            ~Scratch() { if (p) (void)hipFree(p); }
+        } snap;
+        LMX_HIP(hipMalloc(&snap.p, 2 * buf_bytes + sizeof(lmx::Control)));
+        char *base = (char *)snap.p;
+        const uint32_t t0 = s->timestep;
+        const int cur0 = s->cur;
+        const bool packed0 = s->packed, fresh0 = s->flags_fresh;
+        for (int i = 0; i < 2; i++)
+            LMX_HIP(hipMemcpyAsync(base + i * buf_bytes, s->d_lat[i], buf_bytes, hipMemcpyDeviceToDevice, s->stream));
+        LMX_HIP(hipMemcpyAsync(base + 2 * buf_bytes, s->d_ctl, sizeof(lmx::Control), hipMemcpyDeviceToDevice,
+                               s->stream));
+        auto restore = [&]() -> int {
+            for (int i = 0; i < 2; i++)
+                LMX_HIP(hipMemcpyAsync(s->d_lat[i], base + i * buf_bytes, buf_bytes, hipMemcpyDeviceToDevice,
+                                       s->stream));
+            LMX_HIP(hipMemcpyAsync(s->d_ctl, base + 2 * buf_bytes, sizeof(lmx::Control), hipMemcpyDeviceToDevice,
+                                   s->stream));
+            s->timestep = t0;
+            s->cur = cur0;
+            s->packed = packed0;
+            s->flags_fresh = fresh0;
+            return LMX_OK;
+        };
+
+        struct Config {
+            int path, yz;
+            uint32_t seg;
+            bool fuse_reaction;
+        };
+        // The y/z kernels and their window shapes; options a shape cannot use
+        // fall back to the sweeps path and are timed as such.
+        const Config cands[] = {
+            {LMX_PATH_SWEEPS, 0, 8, true},  {LMX_PATH_SWEEPS, 0, 8, false}, {LMX_PATH_SWEEPS, 1, 4, true},
+            {LMX_PATH_SWEEPS, 1, 8, true},  {LMX_PATH_SWEEPS, 2, 8, true},  {LMX_PATH_SWEEPS, 3, 4, true},
+            {LMX_PATH_SWEEPS, 3, 8, true},  {LMX_PATH_FUSED, 0, 8, true},
+        };
+        const Config before{s->path, s->yz_kernel, s->march_seg, s->fuse_reaction};
+        const bool generic0 = s->force_generic;
+        s->force_generic = false;
+        hipEvent_t e0, e1;
+        LMX_HIP(hipEventCreate(&e0));
+        LMX_HIP(hipEventCreate(&e1));
+        struct Events {
+            hipEvent_t a, b;
+            ~Events() { (void)hipEventDestroy(a); (void)hipEventDestroy(b); }
+        } events{e0, e1};
+        // The current configuration first; two interleaved passes, the
+        // minimum of each candidate; another configuration replaces the
+        // current one only if it is at least 2% faster, so that timing noise
+        // between near-equal shapes does not flip the choice.
+        std::vector<Config> list{before};
+        for (const Config &c : cands)
+            if (c.path != before.path || c.yz != before.yz || c.seg != before.seg ||
+                c.fuse_reaction != before.fuse_reaction)
+                list.push_back(c);
+        std::vector<double> times(list.size(), -1.0);
+        for (int pass = 0; pass < 2 && !rc; pass++) {
+            for (size_t k = 0; k < list.size() && !rc; k++) {
+                const Config &c = list[k];
+                s->path = c.path;
+                s->yz_kernel = c.yz;
+                s->march_seg = c.seg;
+                s->fuse_reaction = c.fuse_reaction;
+                if ((rc = restore())) break;
+                lmx::Params p = make_params(s);
+                if ((rc = enqueue_step(s, p, nullptr, nullptr, nullptr))) break;  // untimed first step
+                p = make_params(s);
+                LMX_HIP(hipEventRecord(e0, s->stream));
+                for (uint32_t i = 0; i < nsteps && !rc; i++) rc = enqueue_step(s, p, nullptr, nullptr, nullptr);
+                if (rc) break;
+                LMX_HIP(hipEventRecord(e1, s->stream));
+                LMX_HIP(hipEventSynchronize(e1));
+                float ms = 0.0f;
+                LMX_HIP(hipEventElapsedTime(&ms, e0, e1));
+                const double per = (double)ms / nsteps;
+                if (times[k] < 0.0 || per < times[k]) times[k] = per;
+            }
+        }
+        size_t win = 0;
+        for (size_t k = 1; k < times.size(); k++)
+            if (times[k] >= 0.0 && times[k] < times[win]) win = k;
+        if (times[win] > 0.98 * times[0]) win = 0;
+        const double best = times[win];
+        const Config pick = list[win];
+        const Config keep = rc ? before : pick;
+        s->path = keep.path;
+        s->yz_kernel = keep.yz;
+        s->march_seg = keep.seg;
+        s->fuse_reaction = keep.fuse_reaction;
+        s->force_generic = rc ? generic0 : false;
+        const int rc2 = restore();
+        LMX_HIP(hipStreamSynchronize(s->stream));
+        if (rc) return rc;
+        if (rc2) return rc2;
+        if (best_ms) *best_ms = best;
+        if (chosen) {
+            chosen[0] = keep.path;
+            chosen[1] = keep.yz;
+            chosen[2] = (int)keep.seg;
+            chosen[3] = keep.fuse_reaction ? 1 : 0;
+        }
+        return LMX_OK;
+    });
+}
+
 int lmx_time_steps(lmx_solver *s, uint32_t nsteps, double *total_ms, double *kernel_ms, uint64_t *kernel_launches,
                    uint32_t nkernels)
 {

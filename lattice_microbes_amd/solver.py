@@ -186,6 +186,16 @@ class MpdRdmeSolver:
         """Launch-shape options (lmx_option); results do not depend on them."""
         _lib.check(self._lib.lmx_set_launch_option(self._h, int(option), int(value)))
 
+    def autotune(self, steps: int = 8) -> Dict[str, object]:
+        """Times every y/z sweep kernel and window shape for ``steps`` steps
+        from the current state and keeps the fastest (lmx_autotune); the state
+        is unchanged afterwards and results do not depend on the choice."""
+        best = c_double()
+        chosen = (ctypes.c_int * 4)()
+        _lib.check(self._lib.lmx_autotune(self._h, int(steps), byref(best), chosen))
+        return {"path": int(chosen[0]), "yz_kernel": int(chosen[1]), "march_segment": int(chosen[2]),
+                "fuse_reaction": bool(chosen[3]), "ms_per_step": best.value}
+
     # ------------------------------------------------------------------ stepping
     def runTimesteps(self, nsteps: int, synchronize: bool = True) -> None:
         """runTimestep (MpdRdmeSolver.cu:996-1193) nsteps times."""
