@@ -212,7 +212,8 @@ def uint32_bench(spec, steps: int, seed: int, copy_gbps: float):
     try:
         wspec.load_into(w, replicate=0, seed=seed)
         w.runTimesteps(3)
-        total_ms, per = w.timeSteps(steps, per_kernel=True)
+        total_ms, _ = w.timeSteps(steps)                  # events around the loop only
+        _, per = w.timeSteps(steps, per_kernel=True)      # per-launch events for the kernel averages
     finally:
         w.close()
     N, P = spec.N, spec.P

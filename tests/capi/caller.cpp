@@ -82,6 +82,6 @@ int main(int argc, char **argv)
     std::fwrite(native.data(), sizeof(uint32_t), native.size(), o);
     std::fwrite(counts.data(), sizeof(uint32_t), counts.size(), o);
     std::fclose(o);
-    std::printf("capi driver: %u steps on %ux%ux%u\n", steps, X, Y, Z);
+    std::printf("capi caller: %u steps on %ux%ux%u\n", steps, X, Y, Z);
     return 0;
 }

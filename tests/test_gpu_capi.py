@@ -1,5 +1,5 @@
 """The C ABI from a C++ caller with no Python in its process
-(tests/capi/driver.cpp, built by build()): a model written to a file, run
+(tests/capi/caller.cpp, built by build()): a model written to a file, run
 through lmx_create / lmx_initialize / lmx_build_model /
 lmx_build_diffusion_model / lmx_run_steps, and its native lattice and species
 counts compared bit for bit with the oracle."""
@@ -14,7 +14,7 @@ import oracle
 
 pytestmark = pytest.mark.gpu
 
-DRIVER = os.path.join(os.path.dirname(os.path.abspath(__file__)), "capi", "_build", "capi_driver")
+CALLER = os.path.join(os.path.dirname(os.path.abspath(__file__)), "capi", "_build", "capi_caller")
 
 
 def write_model(path, spec, replicate, seed_top, steps):
@@ -41,13 +41,13 @@ def write_model(path, spec, replicate, seed_top, steps):
     dict(shape=(32, 8, 6), P=16, nS=5, nST=2, R=5, density=0.2, boundary="reflecting"),
 ], ids=["P8", "P16-reflecting"])
 def test_cpp_caller_matches_oracle(gpu, tmp_path, case):
-    if not os.path.exists(DRIVER):
-        pytest.fail("the C++ driver is not built (build() compiles tests/capi/driver.cpp)")
+    if not os.path.exists(CALLER):
+        pytest.fail("the C++ caller is not built (build() compiles tests/capi/caller.cpp)")
     spec = models.random_model(seed=19, **case)
     steps, replicate, seed_top = 7, 2, 31
     model, out = str(tmp_path / "model.bin"), str(tmp_path / "out.bin")
     write_model(model, spec, replicate, seed_top, steps)
-    r = subprocess.run([DRIVER, model, out], capture_output=True, text=True, timeout=120)
+    r = subprocess.run([CALLER, model, out], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     X, Y, Z = spec.shape
     words = (spec.P // 4) * X * Y * Z
