@@ -465,3 +465,41 @@ def test_option_switches_mid_run(gpu, other):
     phase(4)
     assert solver.getLatticeNative()[1].any()    # the second plane was in use
     solver.close()
+
+
+def test_whole_cell_full_size_matches_oracle(gpu):
+    """The default kernels at the whole-cell shape (256x256x512, P=8, 100
+    species, 300 reactions: whole-chunk waves keeping the plane flags, the
+    XCD block order and the reaction-list shards at their real scale) equal
+    the oracle word for word after 2 steps (the oracle takes seconds per
+    step at this size)."""
+    spec = models.whole_cell(seed=1)
+    solver = lmx.MpdRdmeSolver(0)
+    spec.load_into(solver, seed=3)
+    solver.runTimesteps(2)
+    orc = oracle.Oracle(spec, seed_top=3)
+    assert orc.run(2) == 0
+    np.testing.assert_array_equal(solver.getLatticeNative(), orc.native())
+    np.testing.assert_array_equal(solver.getSpeciesCounts(), orc.species_counts())
+    solver.close()
+
+
+def test_whole_cell_full_size_paths_agree(gpu):
+    """At the whole-cell shape the fused step, the marching y/z sweep and the
+    one-site-per-thread kernels end 6 steps on the default kernels' state."""
+    spec = models.whole_cell(seed=2)
+    states = []
+    for variant in ("default", "fused", "march", "generic"):
+        s = lmx.MpdRdmeSolver(0)
+        spec.load_into(s, seed=5)
+        if variant == "fused":
+            s.setPath(lmx._lib.PATH_FUSED)
+        elif variant == "march":
+            s.setLaunchOption(lmx._lib.OPT_YZ_KERNEL, 1)
+        elif variant == "generic":
+            s.setLaunchOption(lmx._lib.OPT_FORCE_GENERIC, 1)
+        s.runTimesteps(6)
+        states.append(s.getLatticeNative())
+        s.close()
+    for k in range(1, len(states)):
+        np.testing.assert_array_equal(states[k], states[0])
