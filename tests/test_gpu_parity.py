@@ -184,6 +184,11 @@ ALIGNED = [
     ((64, 16, 16), 8, "periodic", 0.01),
     ((32, 24, 20), 16, "reflecting", 0.004),
     ((256, 20, 6), 8, "periodic", 0.01),
+    # rows longer than one x chunk of the tile sweeps (64 groups): several
+    # chunks per row, a partial last chunk, whole-chunk flag waves at X = 512
+    ((300, 6, 5), 8, "periodic", 0.3),
+    ((512, 4, 6), 16, "reflecting", 0.2),
+    ((1028, 3, 4), 8, "periodic", 0.05),
 ]
 
 
