@@ -288,7 +288,7 @@ def test_ensemble_members_bit_exact(gpu):
     """A batch of uneven lattices, models and seeds: after every run each
     member equals its own oracle run (and so its plain run), with and
     without the captured graph, also after members leave and join."""
-    shapes = [(8, 6, 5), (12, 7, 9), (32, 12, 20), (16, 1, 3), (4, 33, 3), (64, 16, 16)]
+    shapes = [(8, 6, 5), (12, 7, 9), (32, 12, 20), (16, 1, 3), (4, 33, 3), (64, 16, 16), (300, 4, 6), (512, 3, 4)]
     for graph in (True, False):
         ens = lmx.Ensemble(0, graph=graph)
         pairs = []
