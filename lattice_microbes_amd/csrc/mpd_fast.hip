@@ -1339,14 +1339,7 @@ __global__ __launch_bounds__(256) void reaction_list_kernel(Params p, uint32_t *
         for (uint32_t blk = blockIdx.x; blk < nblk; blk += gridDim.x) reaction4_body<W>(p, lat, blk);
         return;
     }
-    __shared__ float2 s_tab[kTabLds];
-    const uint32_t ntab = p.nST * p.nS;
-    const bool in_lds = ntab <= kTabLds;
-    if (in_lds) {
-        for (uint32_t k = threadIdx.x; k < ntab; k += blockDim.x) s_tab[k] = reinterpret_cast<const float2 *>(p.prob1)[k];
-    }
-    __syncthreads();
-    const float2 *tab = in_lds ? s_tab : reinterpret_cast<const float2 *>(p.prob1);
+    const float2 *tab = reinterpret_cast<const float2 *>(p.prob1);
     const unsigned long long h3 = timestep_hash(p.seed32, p.ctl->timestep, 3);
     const size_t total = (size_t)kListShards * rl.cap;
     for (uint32_t shard = blockIdx.x; shard < (uint32_t)kListShards; shard += gridDim.x) {
