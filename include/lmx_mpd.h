@@ -161,8 +161,9 @@ int lmx_set_launch_option(lmx_solver *s, int option, int value);
  * register march and register columns of 4 and 8 rows, compacted tiles, and
  * the fused step.  It restores the state after each, times two interleaved
  * passes and keeps the fastest by the minimum of each, but keeps the current
- * configuration unless another is at least 2% faster (LMX_OPT_FORCE_GENERIC
- * is cleared).  Every option gives bit-identical
+ * configuration unless another is at least 2% faster.  With
+ * LMX_OPT_FORCE_GENERIC set only the current configuration is timed and every
+ * option stays as it is.  Every option gives bit-identical
  * results, so afterwards the lattice, step counter, overflow statistics and
  * carried entries are as before the call.  best_ms (optional) receives the
  * best time per step; chosen (optional, 4 ints) the path, LMX_OPT_YZ_KERNEL,
@@ -273,6 +274,7 @@ int lmx_batch_run_steps(lmx_batch *b, uint32_t nsteps);
 int lmx_batch_synchronize(lmx_batch *b);
 /* lmx_batch_run_steps + synchronize, timed with HIP events (milliseconds). */
 int lmx_batch_time_steps(lmx_batch *b, uint32_t nsteps, double *total_ms);
+
 
 #ifdef __cplusplus
 }

@@ -578,6 +578,13 @@ int lmx_build_model(lmx_solver *s, uint32_t nS, uint32_t R, const uint32_t *init
         if (R > 0 && kCols == 0) return fail(LMX_ERR_ARG, "K: must have at least 1 column");
         if (R > 16384) return fail(LMX_ERR_ARG, "The number of reaction table entries exceeds the maximum supported by the solver (16384).");
         if ((R > 0 || nS > 0) && (!types || !K || !S || !D) && R > 0) return fail(LMX_ERR_ARG, "null model array");
+        if (s->batch) return fail(LMX_ERR_STATE, "the solver is an ensemble member; remove it before rebuilding its model");
+        // The fields below are overwritten before every check has passed, so
+        // until this call succeeds the solver has no model and no lattice:
+        // a failed rebuild leaves nothing that a step or a readback could use
+        // with the old device tables.
+        s->model_built = false;
+        s->diffusion_built = false;
         s->nS = nS;
         s->R = R;
         s->initialCounts.assign(initialCounts ? initialCounts : nullptr, initialCounts ? initialCounts + nS : nullptr);
@@ -655,9 +662,15 @@ int lmx_build_diffusion_model(lmx_solver *s, uint32_t nST, const double *DF, con
         const unsigned long long N = (unsigned long long)X * Y * Z;
         if (N >= (1ull << 32)) return fail(LMX_ERR_ARG, "lattice has 2^32 or more sites");
         if (!DF || !lattice || !sites || (s->R > 0 && !RL)) return fail(LMX_ERR_ARG, "null model array");
+        if (s->batch) return fail(LMX_ERR_STATE, "the solver is an ensemble member; remove it before rebuilding its lattice");
         int rc = select_device(s);
         if (rc) return rc;
         LMX_HIP(hipStreamSynchronize(s->stream));
+        // Shape, tables and buffers are replaced below before every check and
+        // allocation has passed; the solver has no lattice until this call
+        // succeeds, so a failed rebuild cannot launch a step over buffers of
+        // another size.
+        s->diffusion_built = false;
 
         s->nST = nST;
         s->X = X; s->Y = Y; s->Z = Z; s->P = P;
@@ -1076,8 +1089,10 @@ int lmx_autotune(lmx_solver *s, uint32_t nsteps, double *best_ms, int *chosen)
             {LMX_PATH_SWEEPS, 3, 8, true},  {LMX_PATH_FUSED, 0, 8, true},
         };
         const Config before{s->path, s->yz_kernel, s->march_seg, s->fuse_reaction};
+        // With LMX_OPT_FORCE_GENERIC the caller chose the one-site-per-thread
+        // kernels, which have no y/z shapes to choose among: only the current
+        // configuration is timed, and every option stays as it is.
         const bool generic0 = s->force_generic;
-        s->force_generic = false;
         hipEvent_t e0, e1;
         LMX_HIP(hipEventCreate(&e0));
         LMX_HIP(hipEventCreate(&e1));
@@ -1091,8 +1106,8 @@ int lmx_autotune(lmx_solver *s, uint32_t nsteps, double *best_ms, int *chosen)
         // between near-equal shapes does not flip the choice.
         std::vector<Config> list{before};
         for (const Config &c : cands)
-            if (c.path != before.path || c.yz != before.yz || c.seg != before.seg ||
-                c.fuse_reaction != before.fuse_reaction)
+            if (!generic0 && (c.path != before.path || c.yz != before.yz || c.seg != before.seg ||
+                c.fuse_reaction != before.fuse_reaction))
                 list.push_back(c);
         std::vector<double> times(list.size(), -1.0);
         for (int pass = 0; pass < 2 && !rc; pass++) {
@@ -1128,7 +1143,7 @@ int lmx_autotune(lmx_solver *s, uint32_t nsteps, double *best_ms, int *chosen)
         s->yz_kernel = keep.yz;
         s->march_seg = keep.seg;
         s->fuse_reaction = keep.fuse_reaction;
-        s->force_generic = rc ? generic0 : false;
+        s->force_generic = generic0;
         const int rc2 = restore();
         LMX_HIP(hipStreamSynchronize(s->stream));
         if (rc) return rc;
@@ -1325,6 +1340,10 @@ struct lmx_batch {
     hipGraphExec_t exec = nullptr;
     std::vector<lmx_solver *> graph_members;
     uint32_t graph_total[4] = {0, 0, 0, 0};
+    // recorded after the last upload from h_members/h_start: the pinned
+    // buffers are rewritten only once that upload has left them
+    hipEvent_t staged = nullptr;
+    bool staged_pending = false;
 };
 
 namespace {
@@ -1336,10 +1355,68 @@ void batch_drop_graph(lmx_batch *b)
     b->graph_members.clear();
 }
 
+// A member's own work is done, its tables are current and its chunk flags
+// zeroed.  Uploading changed tables allocates; without may_alloc that is an
+// error instead.
+int member_ready(lmx_solver *s, bool may_alloc)
+{
+    if (s->tables_dirty) {
+        if (!may_alloc)
+            return fail(LMX_ERR_STATE, "a member's reaction rates changed since its tables were uploaded; an allocating "
+                                       "call (lmx_batch_run_steps or lmx_run_steps) uploads them");
+        int rc = upload_tables(s);
+        if (rc) return rc;
+    }
+    LMX_HIP(hipStreamSynchronize(s->stream));
+    if (s->flag_words > 0 && !s->flags_fresh) {
+        for (int bb = 0; bb < 2; bb++)
+            LMX_HIP(hipMemset(s->d_lat[bb] + s->N * s->W, 0, s->flag_words * sizeof(uint32_t)));
+        s->flags_fresh = true;
+    }
+    return LMX_OK;
+}
+
+// Descriptor of one member from its host-side state (buffer parity, tables)
+// and the grid sizes of its share of the four batched launches.
+void fill_member(const lmx_solver *s, lmx::Member &m, uint32_t blocks[4])
+{
+    m.p = make_params(s);
+    if (s->flag_words > 0) m.p.plane_flags = 1u;  // the batched kernels are the default ones
+    m.lat[0] = s->d_lat[0];
+    m.lat[1] = s->d_lat[1];
+    m.cur = (uint32_t)s->cur;
+    m.pad = 0;
+    const uint32_t threads = lmx::tile_threads_for(s->W);
+    m.geo[0] = lmx::tile_geom(m.p, 1, threads);
+    m.geo[1] = lmx::tile_geom(m.p, 2, threads);
+    blocks[0] = lmx::x4_blocks(m.p);
+    blocks[1] = lmx::tile_blocks(m.geo[0]);
+    blocks[2] = lmx::tile_blocks(m.geo[1]);
+    blocks[3] = lmx::reaction_blocks(m.p);
+}
+
+// After nsteps batched steps were enqueued: the members' host-side state
+// advances, and later work on a member's own stream starts after the batch's.
+int batch_finish(lmx_batch *b, uint32_t nsteps)
+{
+    LMX_HIP(hipEventRecord(b->done, b->stream));
+    for (lmx_solver *s : b->members) {
+        if (nsteps & 1u) s->cur = 1 - s->cur;
+        s->timestep += nsteps;
+        s->packed = true;
+        LMX_HIP(hipStreamWaitEvent(s->stream, b->done, 0));
+    }
+    return LMX_OK;
+}
+
 // Descriptors of every member from its host-side state (buffer parity,
 // tables); the device copy is refreshed before each run.
 int batch_prepare(lmx_batch *b)
 {
+    if (b->staged_pending) {
+        LMX_HIP(hipEventSynchronize(b->staged));
+        b->staged_pending = false;
+    }
     const uint32_t n = (uint32_t)b->members.size();
     if (n > b->cap) {
         uint32_t cap = b->cap ? b->cap : 16;
@@ -1361,30 +1438,10 @@ int batch_prepare(lmx_batch *b)
     uint32_t acc[4] = {0, 0, 0, 0};
     for (uint32_t r = 0; r < n; r++) {
         lmx_solver *s = b->members[r];
-        if (s->tables_dirty) {
-            int rc = upload_tables(s);
-            if (rc) return rc;
-        }
-        LMX_HIP(hipStreamSynchronize(s->stream));
-        lmx::Member &m = b->h_members[r];
-        m.p = make_params(s);
-        if (s->flag_words > 0) {
-            if (!s->flags_fresh) {
-                for (int bb = 0; bb < 2; bb++)
-                    LMX_HIP(hipMemset(s->d_lat[bb] + s->N * s->W, 0, s->flag_words * sizeof(uint32_t)));
-                s->flags_fresh = true;
-            }
-            m.p.plane_flags = 1u;  // the batched kernels are the default ones
-        }
-        m.lat[0] = s->d_lat[0];
-        m.lat[1] = s->d_lat[1];
-        m.cur = (uint32_t)s->cur;
-        m.pad = 0;
-        const uint32_t threads = lmx::tile_threads_for(s->W);
-        m.geo[0] = lmx::tile_geom(m.p, 1, threads);
-        m.geo[1] = lmx::tile_geom(m.p, 2, threads);
-        const uint32_t blocks[4] = {lmx::x4_blocks(m.p), lmx::tile_blocks(m.geo[0]), lmx::tile_blocks(m.geo[1]),
-                                    lmx::reaction_blocks(m.p)};
+        int rc = member_ready(s, true);
+        if (rc) return rc;
+        uint32_t blocks[4];
+        fill_member(s, b->h_members[r], blocks);
         for (int k = 0; k < 4; k++) {
             b->h_start[k * (b->cap + 1) + r] = acc[k];
             acc[k] += blocks[k];
@@ -1397,6 +1454,8 @@ int batch_prepare(lmx_batch *b)
     LMX_HIP(hipMemcpyAsync(b->d_members, b->h_members, sizeof(lmx::Member) * n, hipMemcpyHostToDevice, b->stream));
     LMX_HIP(hipMemcpyAsync(b->d_start, b->h_start, sizeof(uint32_t) * 4 * (b->cap + 1), hipMemcpyHostToDevice,
                            b->stream));
+    LMX_HIP(hipEventRecord(b->staged, b->stream));
+    b->staged_pending = true;
     return LMX_OK;
 }
 
@@ -1452,6 +1511,7 @@ int lmx_batch_create(int device, lmx_batch **out)
         hipError_t e = hipSetDevice(device);
         if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
         if (e == hipSuccess) e = hipEventCreateWithFlags(&b->done, hipEventDisableTiming);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&b->staged, hipEventDisableTiming);
         if (e != hipSuccess) {
             int rc = hip_fail(e, "lmx_batch_create");
             lmx_batch_destroy(b);
@@ -1474,6 +1534,7 @@ int lmx_batch_destroy(lmx_batch *b)
     if (b->h_members) hipHostFree(b->h_members);
     if (b->h_start) hipHostFree(b->h_start);
     if (b->done) hipEventDestroy(b->done);
+    if (b->staged) hipEventDestroy(b->staged);
     if (b->stream) hipStreamDestroy(b->stream);
     delete b;
     return LMX_OK;
@@ -1545,17 +1606,10 @@ int lmx_batch_run_steps(lmx_batch *b, uint32_t nsteps)
             for (uint32_t i = 0; i < nsteps; i++)
                 if ((rc = batch_enqueue_step(b))) return rc;
         }
-        LMX_HIP(hipEventRecord(b->done, b->stream));
-        for (lmx_solver *s : b->members) {
-            if (nsteps & 1u) s->cur = 1 - s->cur;
-            s->timestep += nsteps;
-            s->packed = true;
-            // Later work on a member's own stream starts after the batch's.
-            LMX_HIP(hipStreamWaitEvent(s->stream, b->done, 0));
-        }
-        return LMX_OK;
+        return batch_finish(b, nsteps);
     });
 }
+
 
 int lmx_batch_synchronize(lmx_batch *b)
 {
