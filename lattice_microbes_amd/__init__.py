@@ -8,9 +8,10 @@ solver class, and generators for the benchmark models.
 """
 from ._lib import (LmxError, InvalidArgException, OverflowException, device_count, library_path,
                    load as load_library, torch_interop)
-from .solver import INT_SOLVER_NAME, IntMpdRdmeSolver, MESolverFactory, MpdRdmeSolver, SOLVER_NAME, timestep_seed
+from .solver import (INT_SOLVER_NAME, IntMpdRdmeSolver, MESolverFactory, MpdRdmeSolver, SOLVER_NAME, clear_abort,
+                     request_abort, timestep_seed)
 from .ensemble import Ensemble
 from . import models
 
 __all__ = ["LmxError", "InvalidArgException", "OverflowException", "device_count", "library_path",
-           "load_library", "torch_interop", "MESolverFactory", "MpdRdmeSolver", "SOLVER_NAME", "IntMpdRdmeSolver", "INT_SOLVER_NAME", "timestep_seed", "Ensemble", "models"]
+           "load_library", "torch_interop", "MESolverFactory", "MpdRdmeSolver", "SOLVER_NAME", "IntMpdRdmeSolver", "INT_SOLVER_NAME", "timestep_seed", "request_abort", "clear_abort", "Ensemble", "models"]

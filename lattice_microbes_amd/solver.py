@@ -10,6 +10,10 @@ the step.
 from __future__ import annotations
 
 import ctypes
+import signal
+import sys
+import threading
+import time as _time
 import warnings
 from ctypes import byref, c_double, c_size_t, c_uint32, c_uint64, c_void_p
 from typing import Callable, Dict, List, Optional
@@ -25,6 +29,26 @@ INT_SOLVER_NAME = "lm::rdme::IntMpdRdmeSolver"
 
 def _ptr(a: np.ndarray, ctype):
     return a.ctypes.data_as(ctypes.POINTER(ctype))
+
+
+# globalAbort (src/core/SignalHandler.cpp:109): a stop request that every
+# running trajectory sees before its next batch of steps.
+_abort = threading.Event()
+# Steps per lmx_run_steps batch at most: a stop request and the overflow-use
+# window of 1000 steps are seen within this many steps.
+_MAX_BATCH = 1000
+
+
+def request_abort() -> None:
+    """Ask running trajectories to stop (the reference's globalAbort): each
+    ends after its current batch of steps, and its end of trajectory still
+    runs (final lattice to onEndTrajectory, buffered counts and max counts
+    written), MpdRdmeSolver.cu:561-565, 662-667."""
+    _abort.set()
+
+
+def clear_abort() -> None:
+    _abort.clear()
 
 
 def timestep_seed(seed32: int, timestep: int, substep: int) -> int:
@@ -392,42 +416,89 @@ class MpdRdmeSolver:
 
         next_c, next_l, next_h = wi, lwi, hi
         end = _last_step(self.tau, maxTime)
-        while time < maxTime:
-            # Run to the next step at which anything happens (an interval of 0
-            # fires after every step, as in the reference) or the last step.
-            due = min([next_c, next_l] + ([next_h] if hook_enabled else []))
-            stop = min(max(due, t + 1), max(end, t + 1))
-            self.runTimesteps(stop - t, synchronize=True)
-            t = stop
-            time = t * self.tau
-            if self.overflowHandler == "relaxed":
-                unresolved = self.getUnresolvedOverflows()
-                if unresolved > 0:
-                    warnings.warn(f"{unresolved} unresolved overflows")
-            # The lattice comes to the host for a host hook or a lattice frame;
-            # counts alone are taken on the GPU (the same numbers as counting
-            # the host copy, which then equals the device lattice).
-            hook_due = hook_enabled and t >= next_h
-            host = t >= next_l or (hook_due and not device)
-            if host and not device:
-                lattice.copyFromGPU()
-            if hook_due:
-                host |= self._hook_check(time, lattice, dlat, rec)
-                next_h += hi
-            if t >= next_l:
-                if device:
-                    lattice.copyFromGPU()
-                rec.lattice(time, lattice)
-                next_l += lwi
-            if t >= next_c:
-                rec.species_counts(time, lattice if host else None)
-                next_c += wi
+        perf_iv = float(prm.get("perfPrintInterval", "") or 60.0)
+        window_end, ovf_at_window = 1000, self.getOverflowStats()[1]
+        perf_t0, perf_steps = _time.perf_counter(), 0
+        # SIGINT and SIGTERM become a stop request for the duration of the
+        # trajectory (the reference's signal thread sets globalAbort).
+        restore = {}
+        if threading.current_thread() is threading.main_thread():
+            for sig in (signal.SIGINT, signal.SIGTERM):
+                restore[sig] = signal.signal(sig, lambda *_: _abort.set())
+        try:
+            while time < maxTime:
+                if _abort.is_set():
+                    print("Global abort: terminating solver", file=sys.stderr, flush=True)
+                    break
+                # Run to the next step at which anything happens (an interval of
+                # 0 fires after every step, as in the reference) or the last step.
+                due = min([next_c, next_l] + ([next_h] if hook_enabled else []))
+                stop = min(max(due, t + 1), max(end, t + 1), window_end, t + _MAX_BATCH)
+                self.runTimesteps(stop - t, synchronize=True)
+                perf_steps += stop - t
+                t = stop
+                time = t * self.tau
+                if self.overflowHandler == "relaxed":
+                    unresolved = self.getUnresolvedOverflows()
+                    if unresolved > 0:
+                        warnings.warn(f"{unresolved} unresolved overflows")
+                if t >= window_end:
+                    # overflowTimesteps / overflowListUses (MpdRdmeSolver.cu:1180-1186)
+                    used = self.getOverflowStats()[1]
+                    if self.overflowHandler != "relaxed" and used - ovf_at_window > 10:
+                        warnings.warn(f"{used - ovf_at_window} uses of the particle overflow list in the last "
+                                      "1000 timesteps, performance may be degraded.")
+                    ovf_at_window, window_end = used, window_end + 1000
+                elapsed = _time.perf_counter() - perf_t0
+                if elapsed >= perf_iv:
+                    step_s = elapsed / perf_steps
+                    print(f"Average walltime per timestep: {1e3 * step_s:.2f} ms. Progress: {time:.4f}s/"
+                          f"{maxTime:.4f}s ({100.0 * time / maxTime:.3g}% done / "
+                          f"{step_s * (maxTime - time) / self.tau:.2g} seconds walltime remaining)",
+                          file=sys.stderr, flush=True)
+                    perf_t0, perf_steps = _time.perf_counter(), 0
+                self._events(t, time, next_c, next_l, next_h, hook_enabled, device, lattice, dlat, rec)
+                if hook_enabled and t >= next_h:
+                    next_h += hi
+                if t >= next_l:
+                    next_l += lwi
+                if t >= next_c:
+                    next_c += wi
+        except BaseException:
+            # what was recorded so far still reaches the file
+            try:
+                rec.finish()
+            except Exception:
+                pass
+            raise
+        finally:
+            for sig, h in restore.items():
+                signal.signal(sig, h)
         if device:
             self._device_call(self.onEndTrajectory, None, dlat, validate=False)
         lattice.copyFromGPU()
         if not device:
             self.onEndTrajectory(lattice)
         return rec.finish()
+
+    def _events(self, t, time, next_c, next_l, next_h, hook_enabled, device, lattice, dlat, rec) -> None:
+        """The records of an event step: hook, then lattice frame, then species
+        counts (MpdRdmeSolver.cu:612-659)."""
+        # The lattice comes to the host for a host hook or a lattice frame;
+        # counts alone are taken on the GPU (the same numbers as counting
+        # the host copy, which then equals the device lattice).
+        hook_due = hook_enabled and t >= next_h
+        host = t >= next_l or (hook_due and not device)
+        if host and not device:
+            lattice.copyFromGPU()
+        if hook_due:
+            host |= self._hook_check(time, lattice, dlat, rec)
+        if t >= next_l:
+            if device:
+                lattice.copyFromGPU()
+            rec.lattice(time, lattice)
+        if t >= next_c:
+            rec.species_counts(time, lattice if host else None)
 
     def _device_call(self, fn, time, dlat: "DeviceHookLattice", validate: bool) -> int:
         """Calls a device-side hook with fresh device views, waits for the

@@ -3,7 +3,10 @@ CPU: the solver's device calls are replaced by the oracle (a test double, so
 that the event schedule, the hook return codes, the file records and the
 max-count bookkeeping are checked without a GPU).  The same cases run on the
 GPU, through the HIP library, in test_gpu_trajectory.py."""
+import warnings
+
 import numpy as np
+import pytest
 
 import lattice_microbes_amd as lmx
 from lattice_microbes_amd.models import ModelSpec
@@ -73,6 +76,9 @@ class OracleSolver(lmx.MpdRdmeSolver):
 
     def getSpeciesCounts(self):
         return self.orc.species_counts()
+
+    def getOverflowStats(self):
+        return self.orc.stats.overflow_entries, self.orc.stats.overflow_steps
 
 
 class EditingOracleSolver(EditingHooks, OracleSolver):
@@ -172,3 +178,85 @@ def test_hook_code_3_does_not_upload(tmp_path):
         # counts at step 4 come from the (emptied) host copy, as the
         # reference counts the host lattice after the hook
         assert f.getSpeciesCounts(1)[-1].sum() == 0
+
+
+class StoppingSolver(EditingOracleSolver):
+    """The hook of step 8 asks every trajectory to stop (globalAbort)."""
+
+    def hookSimulation(self, time, lattice):
+        code = super().hookSimulation(time, lattice)
+        if int(round(time / self.tau)) == 8:
+            lmx.request_abort()
+        return code
+
+
+def test_stop_request_ends_with_the_end_of_trajectory(tmp_path, capsys):
+    """A stop request (the reference's globalAbort, read before every step,
+    MpdRdmeSolver.cu:561-565) ends the loop after the events of the step it
+    was made at; the end of the trajectory still runs (:662-667): the final
+    lattice goes to onEndTrajectory and the counts and max counts recorded so
+    far are written."""
+    spec = small_spec()
+    path = str(tmp_path / "stop.lm")
+    spec.to_file(path, {"seed": SEED, "maxTime": repr(HOOK_STEPS * spec.tau), "writeInterval": HOOK_WI,
+                        "latticeWriteInterval": HOOK_LWI, "hookInterval": HOOK_HI})
+    solver = StoppingSolver()
+    try:
+        assert runSolver(path, 5, solver)
+    finally:
+        lmx.clear_abort()
+    assert "Global abort: terminating solver" in capsys.readouterr().err
+    assert solver.calls == [0, 4, 8]
+    frames, counts, site_steps, final = expected_hooked(spec, 5, stop=8)
+    with SimulationFile(path, writable=False) as f:
+        assert len(f.getLatticeTimes(5)) == len(frames) == 3
+        for k, fr in enumerate(frames):
+            np.testing.assert_array_equal(f.getLattice(5, k), fr, err_msg=f"frame {k}")
+        np.testing.assert_array_equal(f.getSpeciesCounts(5), counts)
+        np.testing.assert_array_equal(f.getSpeciesCountTimes(5), [t * spec.tau for t in range(0, 9, HOOK_WI)])
+        assert f.getReplicateDataset(5, "MaxParticleCounts").sum() > 0
+    np.testing.assert_array_equal(solver.ended.reshape(-1), final)
+    assert site_steps == [8]
+
+
+class CountingOverflows(OracleSolver):
+    """Reports `per_window` steps with overflows in every window of 1000
+    steps (the oracle runs the lattice; the statistics are scripted)."""
+
+    per_window = 0
+
+    def getOverflowStats(self):
+        done = sum(self.steps_run)
+        return done, (done // 1000) * self.per_window + min(done % 1000, self.per_window)
+
+
+@pytest.mark.parametrize("per_window, warned", [(11, 2), (10, 0)])
+def test_overflow_use_warning_per_1000_steps(tmp_path, per_window, warned):
+    """More than 10 steps with overflows in a window of 1000 steps warns
+    (overflowTimesteps / overflowListUses, MpdRdmeSolver.cu:1180-1186); the
+    step batches end at the window boundaries."""
+    spec = small_spec()
+    path = str(tmp_path / "ovf.lm")
+    spec.to_file(path, {"seed": SEED, "maxTime": repr(2500 * spec.tau), "writeInterval": 2500,
+                        "latticeWriteInterval": 2500})
+    solver = CountingOverflows()
+    solver.per_window = per_window
+    with warnings.catch_warnings(record=True) as w:
+        warnings.simplefilter("always")
+        assert runSolver(path, 0, solver)
+    msgs = [str(x.message) for x in w if "uses of the particle overflow list" in str(x.message)]
+    assert len(msgs) == warned
+    assert all(m.startswith(f"{per_window} uses") for m in msgs)
+    assert solver.steps_run == [1000, 1000, 500]
+
+
+def test_progress_line(tmp_path, capsys):
+    """perfPrintInterval (seconds; MpdRdmeSolver.cu:526-603): the progress
+    line with the average wall time per step."""
+    spec = small_spec()
+    path = str(tmp_path / "perf.lm")
+    spec.to_file(path, {"seed": SEED, "maxTime": repr(6 * spec.tau), "writeInterval": 3,
+                        "latticeWriteInterval": 6, "perfPrintInterval": 0})
+    assert runSolver(path, 0, OracleSolver())
+    err = capsys.readouterr().err
+    assert err.count("Average walltime per timestep") == 2 and "100% done" in err

@@ -108,11 +108,12 @@ def oracle_with_rate(spec, orc, rxid, rate, replicate):
     return spec, nxt
 
 
-def expected_hooked(spec, replicate):
+def expected_hooked(spec, replicate, stop=HOOK_STEPS):
     """Frames, counts, site-frame steps and the final lattice bytes of the
     hooked trajectory.  At time 0 the counts and the frame are recorded
     before the first hook call (MpdRdmeSolver.cu:539-557), at later events
-    after it (:620-660)."""
+    after it (:620-660).  stop < HOOK_STEPS: the trajectory ends after the
+    events of step stop (a stop request made by its hook)."""
     orc = oracle.Oracle(spec, seed_top=SEED, replicate=replicate)
     W, (X, Y, Z) = spec.P // 4, spec.shape
 
@@ -121,7 +122,7 @@ def expected_hooked(spec, replicate):
 
     BEGIN_EDIT.apply(*views(orc))
     frames, counts, site_steps = [orc.row_major()], [orc.species_counts()], []
-    for t in range(0, HOOK_STEPS + 1):
+    for t in range(0, stop + 1):
         if t > 0:
             assert orc.run(1) == 0
         if t % HOOK_HI == 0:
