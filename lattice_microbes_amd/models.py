@@ -454,3 +454,32 @@ def random_model(shape=(8, 6, 5), P=8, nS=4, nST=3, R=6, density=0.5, seed=0, bo
     slots = _place_particles(rng, np.arange(N), sp, N, P)
     return ModelSpec("random", tuple(shape), spacing, tau, P, nS, nST, types, K, S, D, DF, RL,
                      _to_row_major(slots, shape, P), _sites_row_major(st), boundary)
+
+
+# --------------------------------------------------------------------------- ensemble and serving workloads
+ENSEMBLE_SIZES = (32, 48, 64, 96, 128)
+
+
+def ensemble_mix(nrep: int = 256, seed: int = 1):
+    """The batched-ensemble workload: nrep replicates of the bimolecular
+    model, each on a lattice drawn uniformly from 32^3, 48^3, 64^3, 96^3 and
+    128^3, with its own seed.  Returns (model, replicate, seed) per replicate:
+    replicate r runs its lattice size's model with replicate number r and
+    seed 1000 + r."""
+    rng = np.random.default_rng(seed)
+    picks = rng.integers(0, len(ENSEMBLE_SIZES), nrep)
+    templates = {L: bimolecular(shape=(L, L, L), seed=seed + L) for L in ENSEMBLE_SIZES}
+    return [(templates[ENSEMBLE_SIZES[int(picks[r])]], r, 1000 + r) for r in range(nrep)]
+
+
+def whole_cell_reduced(seed: int = 1) -> ModelSpec:
+    """The whole-cell model reduced for serving: 256x256x128, 30 species, 90
+    reactions, 2.5e5 particles (a quarter of the full lattice's)."""
+    return whole_cell(shape=(256, 256, 128), nS=30, R=90, n_particles=250_000, seed=seed)
+
+
+def serving_templates(seed: int = 1) -> List[ModelSpec]:
+    """The request mix of the serving workload: the bimolecular 64^3 model,
+    the capsule 128x128x256 and the reduced whole-cell 256x256x128."""
+    return [bimolecular(shape=(64, 64, 64), seed=seed), capsule_minde(seed=seed + 1),
+            whole_cell_reduced(seed=seed + 2)]

@@ -182,3 +182,106 @@ def test_tracked_particles_diffuse_as_mpd_random_walk(gpu):
     c = np.corrcoef(d.T)
     for a, b in ((0, 1), (0, 2), (1, 2)):
         assert abs(c[a, b]) < z999 / math.sqrt(n), (a, b, c[a, b])
+
+
+def _nsm_counts(s, steps, replicates, seed_top):
+    """Species counts at steps * tau of next-subvolume replicates of model s
+    (the exact RDME), run in threads over the host cores: the oracle releases
+    the interpreter while it runs."""
+    import os
+    from concurrent.futures import ThreadPoolExecutor
+
+    lib = oracle.load()
+    S = np.ascontiguousarray(s.S, dtype=np.int32).reshape(-1)
+    DF = np.ascontiguousarray(s.DF, dtype=np.float64).reshape(-1)
+    o = oracle.Oracle(s, seed_top=seed_top, replicate=0)
+    X, Y, Z = s.shape
+
+    def one(r):
+        h = lib.orc_nsm_create(X, Y, Z, s.P, s.nS, s.nST, s.R, oracle._p(o.types, ctypes.c_uint32),
+                               oracle._p(o.k, ctypes.c_double), oracle._p(o.si, ctypes.c_uint32),
+                               oracle._p(o.s2i, ctypes.c_uint32), oracle._p(S, ctypes.c_int32),
+                               oracle._p(DF, ctypes.c_double), oracle._p(o.RL32, ctypes.c_uint32), s.spacing,
+                               oracle._p(o.lattice, ctypes.c_uint32), oracle._p(o.sites, ctypes.c_uint8),
+                               seed_top, r + 1)
+        assert h
+        rc = lib.orc_nsm_run(h, steps * s.tau, 1 << 62, 0.0, 0, None, None)
+        counts = np.zeros(s.nS, np.uint32)
+        lib.orc_nsm_counts(h, oracle._p(counts, ctypes.c_uint32))
+        lib.orc_nsm_destroy(h)
+        assert rc == 0
+        return counts
+
+    workers = max(1, min(16, os.cpu_count() or 1))
+    with ThreadPoolExecutor(workers) as ex:
+        return np.array(list(ex.map(one, range(replicates))), dtype=np.float64)
+
+
+def _cme_bimolecular(K, kr, n0, t):
+    """Distribution of C at time t for the well-mixed A + B <-> C chemical
+    master equation from A = B = n0, C = 0: states c = 0..n0, association
+    propensity K (n0-c)^2, dissociation kr c (a tridiagonal generator),
+    p(t) = expm(Q t) p(0)."""
+    from scipy.sparse import diags
+    from scipy.sparse.linalg import expm_multiply
+
+    c = np.arange(n0 + 1, dtype=np.float64)
+    up = K * (n0 - c) ** 2          # c -> c + 1
+    down = kr * c                    # c -> c - 1
+    Q = diags([-(up + down), up[:-1], down[1:]], [0, -1, 1], format="csc")  # dp/dt = Q p
+    p0 = np.zeros(n0 + 1)
+    p0[0] = 1.0
+    p = expm_multiply(Q * t, p0)
+    p = np.clip(p, 0.0, None)
+    p /= p.sum()
+    mu = float((p * c).sum())
+    return mu, float((p * c * c).sum() - mu * mu)
+
+
+def test_bimolecular_64_matches_nsm_and_cme(gpu):
+    """The plain benchmark model itself (64^3 reflecting, lambda 32 nm, tau
+    50 us, 1000 A + 1000 B, kf 1e6 /M/s, kr 0.1 /s), 2000 steps = 0.1 s:
+    mean and variance of C over 512 MPD replicates against 256
+    next-subvolume replicates of the same lattice (exact RDME; 99% Welch and
+    F intervals), and against the well-mixed chemical master equation solved
+    by matrix exponential (99% normal interval for the mean, 99% bootstrap
+    interval for the variance).  The CME is the well-mixed limit: on the
+    cubic lattice the diffusion-limited correlation lowers the effective
+    association rate by k_sv lambda^2 W / (D_A + D_B) ~ 0.7% (W = 0.2527, the
+    lattice constant of the return probability), a quarter of the mean's
+    interval here.  A rate-unit slip (a factor N, N_A or 1000 in the
+    conversion) or a lost factor 2 moves the mean by far more."""
+    from lattice_microbes_amd import models
+    from scipy import stats
+
+    s = models.bimolecular(seed=1)
+    assert s.shape == (64, 64, 64) and s.boundary == "reflecting"
+    steps, R_mpd, R_nsm = 2000, 512, 256
+    ens, solvers = ensemble(s, R_mpd, 500)
+    ens.run(steps)
+    counts = np.array([sv.getSpeciesCounts() for sv in solvers], dtype=np.float64)
+    ens.close()
+    for sv in solvers:
+        sv.close()
+    assert np.all(counts[:, 0] + counts[:, 2] == 1000) and np.all(counts[:, 1] + counts[:, 2] == 1000)
+    c_mpd = counts[:, 2]
+    c_nsm = _nsm_counts(s, steps, R_nsm, 13)[:, 2]
+    assert c_mpd.mean() > 10.0  # the association has progressed
+
+    diff = c_mpd.mean() - c_nsm.mean()
+    se = math.sqrt(c_mpd.var(ddof=1) / R_mpd + c_nsm.var(ddof=1) / R_nsm)
+    assert abs(diff) < Z99 * se, f"MPD {c_mpd.mean():.3f} vs NSM {c_nsm.mean():.3f} (99% half-width {Z99 * se:.3f})"
+    f = c_mpd.var(ddof=1) / c_nsm.var(ddof=1)
+    lo, hi = stats.f.ppf([0.005, 0.995], R_mpd - 1, R_nsm - 1)
+    assert lo <= f <= hi, f"MPD/NSM variance ratio {f:.3f} outside [{lo:.3f}, {hi:.3f}]"
+
+    K = float(s.K[0])      # association, per pair in the whole volume (LM units)
+    kr = float(s.K[1])
+    mu, var = _cme_bimolecular(K, kr, 1000, steps * s.tau)
+    se_m = c_mpd.std(ddof=1) / math.sqrt(R_mpd)
+    assert abs(c_mpd.mean() - mu) < Z99 * se_m, \
+        f"MPD mean {c_mpd.mean():.3f} vs CME {mu:.3f} (99% half-width {Z99 * se_m:.3f})"
+    rng = np.random.default_rng(1)
+    boot = [c_mpd[rng.integers(0, R_mpd, R_mpd)].var(ddof=1) for _ in range(2000)]
+    vlo, vhi = np.percentile(boot, [0.5, 99.5])
+    assert vlo <= var <= vhi, f"MPD variance interval [{vlo:.3f}, {vhi:.3f}] misses the CME's {var:.3f}"
