@@ -72,43 +72,165 @@ def uses_reaction_list(spec) -> bool:
     return spec.shape[0] % 4 == 0 and spec.R > 0 and not bool((spec.types == ZEROTH).any())
 
 
-def cpu_baseline(spec, budget_s: float, seed: int):
-    """NSM port on the same model, single-threaded, bounded by wall time."""
+def kernel_source_digest() -> str:
+    """sha256 over the device sources (csrc/*.hip and csrc/lmx_*.h, sorted by
+    name): the kernels the HBM counters in profiles/traffic.json were
+    collected from."""
+    import glob
+    import hashlib
+
+    h = hashlib.sha256()
+    csrc = os.path.join(ROOT, "lattice_microbes_amd", "csrc")
+    for path in sorted(glob.glob(os.path.join(csrc, "*.hip")) + glob.glob(os.path.join(csrc, "lmx_*.h"))):
+        h.update(os.path.basename(path).encode())
+        h.update(open(path, "rb").read())
+    return h.hexdigest()
+
+
+def measured_traffic(key: str):
+    """HBM bytes per launch of one kernel (rocprofv3 FETCH_SIZE + WRITE_SIZE,
+    scaled by a plain copy's counters) from profiles/traffic.json, used only
+    if they were collected from the current kernel sources."""
+    tfile = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(tfile):
+        return None, "no counters committed"
+    try:
+        data = json.load(open(tfile))
+    except (OSError, ValueError):
+        return None, "profiles/traffic.json unreadable"
+    if data.get("kernel_sources_sha256") != kernel_source_digest():
+        return None, "counters were collected from other kernel sources: dropped"
+    value = data.get("bytes_per_launch", {}).get(key)
+    if value is None:
+        return None, f"no counters for {key}"
+    return value, f"rocprofv3 FETCH_SIZE/WRITE_SIZE at kernel sources {data['kernel_sources_sha256'][:16]}"
+
+
+def host_cpu() -> dict:
+    """The host's CPU model and core counts (os.cpu_count and this process's
+    affinity)."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        avail = os.cpu_count()
+    return {"model": model, "nproc": os.cpu_count(), "affinity": avail}
+
+
+def nsm_run(spec, seed: int, replicate: int, budget_s: float):
+    """The next-subvolume port (NextSubvolumeSolver, one thread) on one
+    replicate of spec for about budget_s of wall time: (simulated seconds,
+    events, wall seconds, setup seconds).  The C port runs without the GIL, so
+    several calls proceed in parallel from Python threads."""
     import ctypes
 
     import oracle
 
     lib = oracle.load()
-    orc = oracle.Oracle(spec, seed_top=seed)
+    orc = oracle.Oracle(spec, seed_top=seed, replicate=replicate)
     X, Y, Z = spec.shape
     S = np.ascontiguousarray(spec.S, dtype=np.int32).reshape(-1)
     DF = np.ascontiguousarray(spec.DF, dtype=np.float64).reshape(-1)
-    P = ctypes.POINTER
     t0 = time.perf_counter()
     h = lib.orc_nsm_create(X, Y, Z, spec.P, spec.nS, spec.nST, spec.R, oracle._p(orc.types, ctypes.c_uint32),
                            oracle._p(orc.k, ctypes.c_double), oracle._p(orc.si, ctypes.c_uint32),
                            oracle._p(orc.s2i, ctypes.c_uint32), oracle._p(S, ctypes.c_int32),
                            oracle._p(DF, ctypes.c_double), oracle._p(orc.RL32, ctypes.c_uint32), spec.spacing,
                            oracle._p(orc.lattice, ctypes.c_uint32), oracle._p(orc.sites, ctypes.c_uint8),
-                           seed, 1)
+                           seed, replicate + 1)
     if not h:
         raise RuntimeError("NSM setup failed")
     setup_s = time.perf_counter() - t0
     wall = 0.0
     chunk = 200_000
-    while wall < budget_s:
-        t1 = time.perf_counter()
-        rc = lib.orc_nsm_run(h, 1e30, chunk, 0.0, 0, None, None)
-        wall += time.perf_counter() - t1
-        if rc:
-            raise RuntimeError(f"NSM error {rc}")
-    sim_t = lib.orc_nsm_time(h)
-    events = lib.orc_nsm_events(h)
-    lib.orc_nsm_destroy(h)
+    try:
+        while wall < budget_s:
+            t1 = time.perf_counter()
+            rc = lib.orc_nsm_run(h, 1e30, chunk, 0.0, 0, None, None)
+            wall += time.perf_counter() - t1
+            if rc:
+                raise RuntimeError(f"NSM error {rc}")
+        return lib.orc_nsm_time(h), lib.orc_nsm_events(h), wall, setup_s
+    finally:
+        lib.orc_nsm_destroy(h)
+
+
+# Kernels of the default whole-cell step, by rocprofv3 kernel name prefix.
+TRAFFIC_KERNELS = {"sweep_x": "void lmx::sweep_x4_kernel<2>", "sweep_y": "void lmx::sweep_tile_kernel<2, false>",
+                   "sweep_z": "void lmx::sweep_tile_kernel<2, true>", "reaction": "void lmx::reaction_list_kernel<2>"}
+
+
+def counter_means(path: str, counter: str) -> dict:
+    """Mean of one counter per dispatch, by kernel name, from a rocprofv3
+    counter_collection.csv."""
+    import csv
+
+    acc = {}
+    with open(path) as f:
+        for row in csv.DictReader(f):
+            if row["Counter_Name"] != counter:
+                continue
+            acc.setdefault(row["Kernel_Name"], []).append(float(row["Counter_Value"]))
+    return {k: sum(v) / len(v) for k, v in acc.items()}
+
+
+def write_traffic(fetch_csv: str, write_csv: str, copy_fetch_csv: str, copy_write_csv: str, key: str) -> dict:
+    """profiles/traffic.json from two rocprofv3 --pmc runs of the benchmark
+    (FETCH_SIZE, WRITE_SIZE, KiB per dispatch) and the same two counters for
+    a plain 1 GiB device copy (bench.py --calibrate-copy), which scale the
+    counters to bytes; stamped with the kernel sources they were measured
+    at."""
+    gib = float(1 << 30)
+
+    def copy_factor(path, counter):
+        vals = [v for k, v in counter_means(path, counter).items() if "copyBuffer" in k]
+        if not vals:
+            raise SystemExit(f"no copy kernel in {path}")
+        return gib / (max(vals) * 1024.0)
+
+    fr, fw = copy_factor(copy_fetch_csv, "FETCH_SIZE"), copy_factor(copy_write_csv, "WRITE_SIZE")
+    fetch, write = counter_means(fetch_csv, "FETCH_SIZE"), counter_means(write_csv, "WRITE_SIZE")
+    out = {}
+    for name, prefix in TRAFFIC_KERNELS.items():
+        kf = [k for k in fetch if k.startswith(prefix)]
+        kw = [k for k in write if k.startswith(prefix)]
+        if kf and kw:
+            out[f"{key}:{name}"] = int(round(fetch[kf[0]] * 1024.0 * fr + write[kw[0]] * 1024.0 * fw))
+    data = {"kernel_sources_sha256": kernel_source_digest(), "read_scale": fr, "write_scale": fw,
+            "bytes_per_launch": out}
+    with open(os.path.join(ROOT, "profiles", "traffic.json"), "w") as f:
+        json.dump(data, f, indent=1)
+        f.write("\n")
+    return data
+
+
+def cpu_baseline(spec, budget_s: float, seed: int, ens_workers: int, sim_steps: int):
+    """CPU figures beside the GPU ones, all from this project's C restatements
+    of the reference's CPU solvers:
+    * value: the NSM port on the benchmark model, one core, bounded by wall
+      time; equivalent site-steps/s = N x (simulated time / tau) / wall;
+    * all_cores_ensemble: the ensemble workload's replicates run by the NSM
+      port one replicate per host core (the reference's ReplicateRunner runs
+      one replicate per thread), as simulations per hour;
+    * serial_mpd_oracle: one step of the serial MPD restatement of the
+      benchmark model, for context."""
+    import threading
+
+    import oracle
+    from lattice_microbes_amd import models
+
+    sim_t, events, wall, setup_s = nsm_run(spec, seed, 0, budget_s)
     eq_steps = sim_t / spec.tau
-    value = spec.N * eq_steps / wall
-    return {
-        "value": value,
+    out = {
+        "value": spec.N * eq_steps / wall,
         "unit": "site-steps/s",
         "cores": 1,
         "kind": "port",
@@ -117,72 +239,119 @@ def cpu_baseline(spec, budget_s: float, seed: int):
                    f"{sim_t * 1e6:.3f} us simulated = {eq_steps:.3f} tau-steps equivalent "
                    f"(setup {setup_s:.1f} s untimed)"),
         "events_per_s": events / wall,
+        "host": host_cpu(),
     }
+    # all cores: replicate r of the ensemble mix on thread r
+    mix = models.ensemble_mix(max(ens_workers, 1), seed)
+    res = [None] * len(mix)
+
+    def work(k):
+        spec_k, r, sd = mix[k]
+        res[k] = nsm_run(spec_k, sd, r, budget_s)
+
+    threads = [threading.Thread(target=work, args=(k,)) for k in range(len(mix))]
+    t0 = time.perf_counter()
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join()
+    span = time.perf_counter() - t0
+    if any(r is None for r in res):
+        raise RuntimeError("an NSM ensemble worker failed")
+    tau = mix[0][0].tau
+    sims_per_s = sum(r[0] / (sim_steps * tau) / r[2] for r in res)
+    out["all_cores_ensemble"] = {
+        "workload": f"bimolecular replicates of the ensemble mix ({', '.join(str(m[0].shape[0]) for m in mix)} cubed)",
+        "threads": len(mix), "sim_steps": sim_steps,
+        "sims_per_hr": sims_per_s * 3600.0,
+        "events_per_s": sum(r[1] / r[2] for r in res),
+        "wall_s": span,
+    }
+    # the serial MPD restatement (the oracle), one step of the benchmark model
+    orc = oracle.Oracle(spec, seed_top=seed)
+    t0 = time.perf_counter()
+    if orc.run(1) != 0:
+        raise RuntimeError("oracle step failed")
+    step_s = time.perf_counter() - t0
+    out["serial_mpd_oracle"] = {"s_per_step": step_s, "site_steps_per_s": spec.N / step_s, "cores": 1}
+    return out
 
 
-def ensemble_bench(nrep: int, steps: int, seed: int):
+def ensemble_bench(nrep: int, steps: int, seed: int, sync_steps: int):
     """Batched ensemble: nrep replicates of the bimolecular model on uneven
     lattices (32^3, 48^3, 64^3, 96^3, 128^3 drawn uniformly), each with its own
-    seed, stepped by one Ensemble against the same replicates run one at a
-    time; the first replicates are checked bit for bit between the two."""
+    seed, stepped by one Ensemble, against plain MPD one simulation at a time:
+    each replicate runs all its steps (launches queued back to back) before
+    the next one starts; and, as the reference steps (a host synchronisation
+    after every step, MpdRdmeSolver.cu:1091), sync_steps steps of every
+    replicate.  Every replicate's final lattice is compared between the
+    batched and the one-at-a-time runs."""
+    import hashlib
+
     import lattice_microbes_amd as lmx
     from lattice_microbes_amd import models
 
-    sizes = [32, 48, 64, 96, 128]
-    rng = np.random.default_rng(seed)
-    picks = rng.integers(0, len(sizes), nrep)
-    templates = {L: models.bimolecular(shape=(L, L, L), seed=seed + L) for L in sizes}
+    mix = models.ensemble_mix(nrep, seed)
+    sites = sum(spec.N for spec, _, _ in mix)
 
-    def make(r):
-        L = sizes[picks[r]]
+    def make(spec, r, sd):
         s = lmx.MpdRdmeSolver(0)
-        templates[L].load_into(s, replicate=r, seed=1000 + r)
+        spec.load_into(s, replicate=r, seed=sd)
         return s
 
-    plain = [make(r) for r in range(nrep)]
-    sites = sum(int(np.prod(s.shape)) for s in plain)
-    for s in plain:
-        s.runTimesteps(1)
-    t0 = time.perf_counter()
-    for s in plain:
-        s.runTimesteps(steps, synchronize=False)
-    for s in plain:
-        s.synchronize()
-    t_plain = time.perf_counter() - t0
-    # the reference's cadence: the host waits for every step (MpdRdmeSolver.cu:1091)
-    sub = plain[nrep - max(1, nrep // 8):]
-    sub_sites = sum(int(np.prod(s.shape)) for s in sub)
-    t0 = time.perf_counter()
-    for s in sub:
-        for _ in range(steps):
-            s.runTimesteps(1)
-    t_sync = time.perf_counter() - t0
-    batch = [make(r) for r in range(nrep)]
+    def digest(s):
+        return hashlib.sha256(s.getLatticeNative().tobytes()).hexdigest()
+
+    # batched: one warm step (graph capture), then the timed steps
+    batch = [make(*m) for m in mix]
     ens = lmx.Ensemble(0)
     for s in batch:
         ens.add(s)
     ens.run(1)
     ms = ens.time_steps(steps)
-    # every plain replicate outside sub ran 1 + steps steps, like the batch
-    check = [r for r in range(min(nrep, 16)) if all(plain[r] is not q for q in sub)]
-    same = all(np.array_equal(plain[r].getLatticeNative(), batch[r].getLatticeNative()) for r in check)
+    batched = [digest(s) for s in batch]
     ens.close()
-    for s in plain + batch:
+    for s in batch:
+        s.close()
+    del batch
+    # plain, one simulation at a time: 1 + steps steps each, the timed steps
+    # measured with HIP events on the solver's stream
+    plain_ms = 0.0
+    same = 0
+    for (spec, r, sd), d in zip(mix, batched):
+        s = make(spec, r, sd)
+        s.runTimesteps(1)
+        t, _ = s.timeSteps(steps)
+        plain_ms += t
+        same += int(digest(s) == d)
+        s.close()
+    # the reference's cadence: the host waits for every step
+    sync_wall = 0.0
+    for spec, r, sd in mix:
+        s = make(spec, r, sd)
+        s.runTimesteps(1)
+        t0 = time.perf_counter()
+        for _ in range(sync_steps):
+            s.runTimesteps(1)
+        sync_wall += time.perf_counter() - t0
         s.close()
     b_rate = sites * steps / (ms / 1e3)
-    p_rate = sites * steps / t_plain
-    sync_rate = sub_sites * steps / t_sync
+    p_rate = sites * steps / (plain_ms / 1e3)
+    sync_rate = sites * sync_steps / sync_wall
     sim_steps = 20000
     return {
-        "workload": f"{nrep} bimolecular replicates, lattices drawn from {sizes} cubed, own seeds",
+        "workload": f"{nrep} bimolecular replicates, lattices drawn from {list(models.ENSEMBLE_SIZES)} cubed, "
+                    "own seeds",
         "replicates": nrep, "sites": sites, "steps_timed": steps,
-        "batched_site_steps_per_s": b_rate, "plain_site_steps_per_s": p_rate,
-        "plain_stepsync_site_steps_per_s": sync_rate,
+        "batched_site_steps_per_s": b_rate,
+        "plain_one_at_a_time_site_steps_per_s": p_rate,
+        "plain_stepsync_site_steps_per_s": sync_rate, "stepsync_steps_timed": sync_steps,
         "speedup_vs_plain": b_rate / p_rate, "speedup_vs_plain_stepsync": b_rate / sync_rate,
         "sims_per_hr_batched": nrep / (ms / 1e3 / steps * sim_steps) * 3600.0,
-        "sims_per_hr_plain": nrep / (t_plain / steps * sim_steps) * 3600.0,
+        "sims_per_hr_plain": nrep / (plain_ms / 1e3 / steps * sim_steps) * 3600.0,
+        "sims_per_hr_plain_stepsync": nrep / (sync_wall / sync_steps * sim_steps) * 3600.0,
         "sim_steps": sim_steps,
-        "bit_exact_vs_plain": {"checked": len(check), "identical": bool(same)},
+        "bit_exact_vs_plain": {"checked": nrep, "identical": same},
     }
 
 
@@ -232,25 +401,37 @@ def uint32_bench(spec, steps: int, seed: int, copy_gbps: float):
 def serving_bench(njobs: int, seed: int):
     """Continuous batching of a Poisson request stream (20 req/s) mixing the
     bimolecular 64^3, capsule 128x128x256 and whole-cell 256x256x128 (30
-    species) models, 1e3-1e4 steps each, against the same requests served one
-    at a time; every job's final lattice is compared between the two."""
+    species) models, 1e3-1e4 steps each, admitted smallest work first (a job
+    waiting over 5 s goes first) while the batch holds at most 32M sites,
+    against the same requests served one at a time in arrival order; every
+    job's final lattice is compared between the two.  The reference's cadence
+    (a host synchronisation after every step) is timed on every fourth
+    request."""
     from lattice_microbes_amd import models, serving
 
-    templates = [models.bimolecular(shape=(64, 64, 64), seed=seed),
-                 models.capsule_minde(seed=seed + 1),
-                 models.whole_cell(shape=(256, 256, 128), nS=30, R=90, n_particles=250_000, seed=seed + 2)]
-    jobs = serving.poisson_jobs(templates, njobs, 20.0, (1000, 10000), seed=seed + 10)
-    served = serving.Server(0, hbm_budget=200 << 30, max_active=8).run(jobs)
+    templates = models.serving_templates(seed)
+    mk = lambda: serving.poisson_jobs(templates, njobs, 20.0, (1000, 10000), seed=seed + 10)
+    jobs = mk()
+    served = serving.Server(0, hbm_budget=200 << 30, max_sites=32 << 20, policy="sjf").run(jobs)
     digest = {j.id: j.digest for j in jobs}
-    jobs2 = serving.poisson_jobs(templates, njobs, 20.0, (1000, 10000), seed=seed + 10)
+    jobs2 = mk()
     plain = serving.serve_plain(jobs2)
-    same = all(digest[j.id] == j.digest for j in jobs2)
+    same = sum(int(digest[j.id] == j.digest) for j in jobs2)
+    sample = mk()[::4]
+    for j in sample:
+        j.arrival = 0.0
+    sync = serving.serve_plain(sample, step_sync=True, keep_digest=False)
     return {
         "workload": f"{njobs} requests, Poisson 20/s, bimolecular 64^3 / capsule 128x128x256 / "
                     "whole-cell 256x256x128 (30 species), 1e3-1e4 steps",
+        "admission": "smallest work first (sites x steps), a request waiting over 5 s first; <= 32M sites per step",
         "served": served, "plain": plain,
         "throughput_vs_plain": served["site_steps_per_s"] / plain["site_steps_per_s"],
-        "bit_exact_vs_plain": {"jobs": njobs, "identical": bool(same)},
+        "plain_stepsync_sample": {"jobs": sync["jobs"], "site_steps_per_s": sync["site_steps_per_s"]},
+        "throughput_vs_plain_stepsync": served["site_steps_per_s"] / sync["site_steps_per_s"],
+        "p50_vs_plain": served["p50_latency_s"] / plain["p50_latency_s"],
+        "p99_vs_plain": served["p99_latency_s"] / plain["p99_latency_s"],
+        "bit_exact_vs_plain": {"jobs": njobs, "identical": same},
     }
 
 
@@ -271,11 +452,21 @@ def main() -> int:
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default kernels instead of timing every y/z kernel and window shape first")
     ap.add_argument("--ensemble-replicates", type=int, default=256)
-    ap.add_argument("--ensemble-steps", type=int, default=20)
-    ap.add_argument("--serving-jobs", type=int, default=24)
+    ap.add_argument("--ensemble-steps", type=int, default=2000)
+    ap.add_argument("--ensemble-sync-steps", type=int, default=100,
+                    help="steps per replicate of the per-step-synchronised plain run")
+    ap.add_argument("--serving-jobs", type=int, default=100)
+    ap.add_argument("--write-traffic", nargs=4, metavar=("FETCH_CSV", "WRITE_CSV", "COPY_FETCH_CSV", "COPY_WRITE_CSV"),
+                    help="write profiles/traffic.json from rocprofv3 FETCH_SIZE/WRITE_SIZE runs of the benchmark "
+                         "and of --calibrate-copy, then exit")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain 1 GiB device copy (HBM counter calibration)")
     args = ap.parse_args()
+
+    if args.write_traffic:
+        spec_key = {"whole_cell": "whole_cell_256x256x512"}.get(args.workload, args.workload)
+        print(json.dumps(write_traffic(*args.write_traffic, spec_key)), flush=True)
+        return 0
 
     if args.calibrate_copy:
         import torch
@@ -351,13 +542,7 @@ def main() -> int:
                          "alg_bytes": b, "GBps": (b / (avg_ms / 1e3) / 1e9) if b and avg_ms > 0 else None}
     dom = max((k for k in kernels if kernels[k]["alg_bytes"]), key=lambda k: kernels[k]["avg_us"] * kernels[k]["launches"])
     dk = kernels[dom]
-    traffic = None
-    tfile = os.path.join(ROOT, "profiles", "traffic.json")
-    if os.path.exists(tfile):
-        try:
-            traffic = json.load(open(tfile)).get(f"{spec.name}_{spec.shape[0]}x{spec.shape[1]}x{spec.shape[2]}:{dom}")
-        except Exception:
-            traffic = None
+    traffic, traffic_note = measured_traffic(f"{spec.name}_{spec.shape[0]}x{spec.shape[1]}x{spec.shape[2]}:{dom}")
     # Measured copy peak (1 GiB copy, one 16-byte load and store per thread;
     # read + write bytes).
     copy_gbps = ctypes.c_double(0.0)
@@ -368,7 +553,8 @@ def main() -> int:
                 "copy_peak_measured": copy_gbps.value, "frac_of_copy_peak": dk["GBps"] / copy_gbps.value,
                 # HBM bytes actually moved (rocprofv3 FETCH/WRITE) per launch time: below the
                 # algorithmic rate where the kernels skip the zero upper planes of packed sites
-                "traffic_GBps": (traffic / (dk["avg_us"] / 1e6) / 1e9) if traffic else None}
+                "traffic_GBps": (traffic / (dk["avg_us"] / 1e6) / 1e9) if traffic else None,
+                "traffic_source": traffic_note}
     # Bytes of the step as run: three sweeps plus the reaction list (written
     # and read once), or plus the full reaction read.
     unfused_alg = (3 * sweep_bytes_per_site(P) + (reaction_bytes_per_site(P) if spec.R else 0)) * N
@@ -413,13 +599,15 @@ def main() -> int:
         out["uint32_lattice"] = uint32_bench(spec, 30, args.seed, copy_gbps.value)
     if not args.no_ensemble:
         log(f"ensemble: {args.ensemble_replicates} replicates")
-        out["ensemble"] = ensemble_bench(args.ensemble_replicates, args.ensemble_steps, args.seed)
+        out["ensemble"] = ensemble_bench(args.ensemble_replicates, args.ensemble_steps, args.seed,
+                                         args.ensemble_sync_steps)
     if not args.no_serving:
         log(f"serving: {args.serving_jobs} requests")
         out["serving"] = serving_bench(args.serving_jobs, args.seed)
     if not args.no_cpu:
         log(f"cpu baseline ({args.cpu_budget:.0f}s budget)")
-        out["cpu_baseline"] = cpu_baseline(spec, args.cpu_budget, args.seed)
+        out["cpu_baseline"] = cpu_baseline(spec, args.cpu_budget, args.seed, min(16, host_cpu()["affinity"] or 1),
+                                           20000)
         out["cpu_baseline"]["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]
     else:
         out["cpu_baseline"] = None

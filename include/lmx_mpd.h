@@ -265,8 +265,12 @@ int lmx_batch_destroy(lmx_batch *b);
 int lmx_batch_add(lmx_batch *b, lmx_solver *s);
 int lmx_batch_remove(lmx_batch *b, lmx_solver *s);
 int lmx_batch_size(const lmx_batch *b, uint32_t *n);
-/* LMX_BATCH_GRAPH: 1 (default) = replay a captured graph of one step. */
-enum lmx_batch_option { LMX_BATCH_GRAPH = 1 };
+/* LMX_BATCH_GRAPH: 1 (default) = replay a captured graph of one step.
+ * LMX_BATCH_REACTION_LIST: 1 (default) = a member without zero-order
+ * reactions runs its reaction phase over the occupied sites its z sweep lists
+ * (as a single solver does); 0 = the full in-place pass for every member.
+ * Results are identical either way. */
+enum lmx_batch_option { LMX_BATCH_GRAPH = 1, LMX_BATCH_REACTION_LIST = 2 };
 int lmx_batch_set_option(lmx_batch *b, int option, int value);
 /* Advances every member by nsteps (asynchronous; errors are reported by
  * lmx_batch_synchronize, or by a member's own calls). */

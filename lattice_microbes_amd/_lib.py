@@ -35,6 +35,7 @@ OPT_YZ_KERNEL = 3
 OPT_FUSED_ZSEG = 4
 OPT_FUSE_REACTION = 5
 BATCH_GRAPH = 1
+BATCH_REACTION_LIST = 2
 
 PATH_SWEEPS = 0
 PATH_FUSED = 1

@@ -11,11 +11,44 @@
 
 namespace lmx {
 
+// Division by a divisor fixed per launch, without a division instruction
+// sequence: q = floor(n / d) = umulhi(n, mul) >> shr for n < 2^31 (the
+// round-up method of Granlund and Montgomery, "Division by invariant
+// integers using multiplication", 1994): with l = ceil(log2 d),
+// mul = ceil(2^(31+l) / d) < 2^32 and shr = l - 1; d = 1 is the identity.
+struct FastDiv {
+    uint32_t d, mul, shr;
+};
+
+__host__ __device__ inline FastDiv make_fastdiv(uint32_t d)
+{
+    FastDiv f{d, 0u, 0u};
+    if (d > 1) {
+        uint32_t l = 0;
+        while ((1ull << l) < d) l++;
+        f.mul = (uint32_t)(((1ull << (31 + l)) + d - 1) / d);
+        f.shr = l - 1;
+    }
+    return f;
+}
+
+__host__ __device__ inline uint32_t fast_div(uint32_t n, const FastDiv &f)
+{
+#ifdef __HIP_DEVICE_COMPILE__
+    return f.d == 1 ? n : __umulhi(n, f.mul) >> f.shr;
+#else
+    return f.d == 1 ? n : (uint32_t)(((unsigned long long)n * f.mul) >> 32) >> f.shr;
+#endif
+}
+
 // Geometry of a tiled y (axis 1) or z (axis 2) sweep: tiles of Gx groups of
 // 4 x-sites by Ty rows along the axis, nxchunk x chunks and nouter rows of
-// the remaining axis.
+// the remaining axis.  A block holds No tiles of consecutive outer rows (more
+// than one when a row is shorter than the block), nob blocks cover the outer
+// rows; the divisors of the block and thread decompositions as FastDiv.
 struct TileGeom {
-    uint32_t stride, len, ostride, nouter, Gx, Ty, nxchunk, ntiles, tile_first;
+    uint32_t stride, len, ostride, nouter, Gx, Ty, nxchunk, ntiles, tile_first, No, nob;
+    FastDiv div_nob, div_Gx, div_nxchunk, div_ntiles, div_tile;
 };
 
 __host__ __device__ inline uint32_t tile_threads_for(uint32_t W) { return W == 4 ? 512u : 1024u; }
@@ -29,27 +62,47 @@ __host__ __device__ inline TileGeom tile_geom(const Params &p, int axis, uint32_
     g.nouter = axis == 1 ? p.Z : p.Y;
     const uint32_t gxn = p.X >> 2;
     g.Gx = gxn < 64u ? gxn : 64u;
-    g.Ty = threads / g.Gx;
+    g.Ty = threads / g.Gx < g.len ? threads / g.Gx : g.len;
     g.nxchunk = (gxn + g.Gx - 1) / g.Gx;
     g.ntiles = (g.len + g.Ty - 1) / g.Ty;
+    // Tiles per block: as many as the threads hold, with their two apron
+    // rows each within the block's LDS rows (threads + 128 entries: No * Gx
+    // <= 64).
+    uint32_t no = threads / (g.Gx * g.Ty);
+    if (no * g.Gx > 64u) no = 64u / g.Gx;
+    if (no > g.nouter) no = g.nouter;
+    g.No = no > 0u ? no : 1u;
+    g.nob = (g.nouter + g.No - 1) / g.No;
     // Adjacent tiles share apron rows: number them consecutively when they
     // are closer in memory than the outer rows (y sweep).
     g.tile_first = (unsigned long long)g.Ty * g.stride < (unsigned long long)g.ostride ? 1u : 0u;
+    g.div_nob = make_fastdiv(g.nob);
+    g.div_Gx = make_fastdiv(g.Gx);
+    g.div_nxchunk = make_fastdiv(g.nxchunk);
+    g.div_ntiles = make_fastdiv(g.ntiles);
+    g.div_tile = make_fastdiv(g.Gx * g.Ty);
     return g;
 }
 
 // Blocks of one tiled sweep: a multiple of 8 so that block b runs on XCD b % 8.
 __host__ __device__ inline uint32_t tile_blocks(const TileGeom &g)
 {
-    const uint32_t nblocks = g.nxchunk * g.nouter * g.ntiles;
+    const uint32_t nblocks = g.nxchunk * g.nob * g.ntiles;
     return (nblocks + 7) / 8 * 8;
 }
+
+// Reaction phase of a member: none (no reactions), the full in-place pass, or
+// the list its z sweep fills (no zero-order reactions).
+constexpr uint32_t kReactNone = 0, kReactFull = 1, kReactList = 2;
 
 struct Member {
     Params p;
     uint32_t *lat[2];
     TileGeom geo[2];     // y and z tiled sweeps
+    ReactionList rl;     // the member's own reaction list (kReactList)
     uint32_t cur;        // lat[cur] holds the state at the start of a step; the batched resolver flips it
+    uint32_t react;      // kReactNone, kReactFull or kReactList
+    uint32_t nshards;    // list shards its z sweep fills (kReactList)
     uint32_t pad;
 };
 

@@ -5,6 +5,8 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include "lmx_device.h"
+
 namespace lmx {
 
 struct Params;
@@ -31,6 +33,9 @@ hipError_t launch_reaction_fast(const Params &p, uint32_t *lat, hipStream_t st);
 hipError_t launch_prob_tables(const Params &p, float *prob1, float *prob0, hipStream_t st);
 uint32_t x4_blocks(const Params &p);
 uint32_t reaction_blocks(const Params &p);
+struct TileGeom;
+uint32_t batch_list_shards(const TileGeom &zgeo, uint32_t W);
+uint32_t batch_list_blocks(uint32_t nshards);
 // One batched step without the resolver: x, y, z sweeps and (if total[3] > 0)
 // the reaction phase; start_* are device prefix arrays of n+1 block offsets,
 // total[] the four grid sizes.

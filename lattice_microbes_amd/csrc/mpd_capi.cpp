@@ -359,8 +359,11 @@ uint32_t fused_zseg(lmx_solver *s)
 // One timestep (runTimestep, MpdRdmeSolver.cu:996-1193).  ev, if given,
 // receives events around each launch (ev[0] before the first launch, then one
 // after every launch); kinds receives the kernel class of each launch.
-int enqueue_step(lmx_solver *s, const lmx::Params &p0, hipEvent_t *ev, int *kinds, int *nlaunch)
+// rl_in: the reaction list the z sweep fills (default: the solver's own).
+int enqueue_step(lmx_solver *s, const lmx::Params &p0, hipEvent_t *ev, int *kinds, int *nlaunch,
+                 const lmx::ReactionList *rl_in = nullptr)
 {
+    const lmx::ReactionList &rlist = rl_in ? *rl_in : s->rlist;
     // The step writes packed sites whatever it reads, so only its first
     // kernel may see an unpacked lattice; every launch of this step takes the
     // current flag, and the next step may rely on packing.
@@ -417,7 +420,7 @@ int enqueue_step(lmx_solver *s, const lmx::Params &p0, hipEvent_t *ev, int *kind
     // phase evaluates only those (not with zero-order reactions, which make
     // every site a candidate).
     const bool z_reacts = fast && (s->yz_kernel == 0 || s->yz_kernel == 2) && s->fuse_reaction && s->R > 0 &&
-                          !s->any_qp0 && s->rlist.idx;
+                          !s->any_qp0 && rlist.idx;
     for (int axis = 0; axis < 3; axis++) {
         const bool react = axis == 2 && z_reacts;
         if (!fast)
@@ -425,9 +428,9 @@ int enqueue_step(lmx_solver *s, const lmx::Params &p0, hipEvent_t *ev, int *kind
         else if (axis > 0 && s->yz_kernel == 3 && !react && lmx::col_ok(p, axis, s->march_seg))
             e = lmx::launch_sweep_col(p, s->d_lat[s->cur], s->d_lat[1 - s->cur], axis, s->march_seg, s->stream);
         else if (axis > 0 && (s->yz_kernel == 0 || s->yz_kernel == 3))
-            e = lmx::launch_sweep_tile(p, s->d_lat[s->cur], s->d_lat[1 - s->cur], axis, react, s->rlist, s->stream);
+            e = lmx::launch_sweep_tile(p, s->d_lat[s->cur], s->d_lat[1 - s->cur], axis, react, rlist, s->stream);
         else if (axis > 0 && s->yz_kernel == 2)
-            e = lmx::launch_sweep_sparse(p, s->d_lat[s->cur], s->d_lat[1 - s->cur], axis, react, s->rlist,
+            e = lmx::launch_sweep_sparse(p, s->d_lat[s->cur], s->d_lat[1 - s->cur], axis, react, rlist,
                                          s->stream);
         else
             e = lmx::launch_sweep_fast(p, s->d_lat[s->cur], s->d_lat[1 - s->cur], axis, s->march_seg, s->stream);
@@ -436,7 +439,7 @@ int enqueue_step(lmx_solver *s, const lmx::Params &p0, hipEvent_t *ev, int *kind
         if (int rc = mark(LMX_K_SWEEP_X + axis)) return rc;
     }
     if (z_reacts) {
-        e = lmx::launch_reaction_list(p, s->d_lat[s->cur], s->rlist, (int)s->num_cu, s->stream);
+        e = lmx::launch_reaction_list(p, s->d_lat[s->cur], rlist, (int)s->num_cu, s->stream);
         if (e != hipSuccess) return hip_fail(e, "reaction launch");
         if (int rc = mark(LMX_K_REACTION)) return rc;
     } else if (s->R > 0) {
@@ -1331,6 +1334,7 @@ struct lmx_batch {
     std::vector<lmx_solver *> members;
     uint32_t W = 0;
     bool use_graph = true;
+    bool use_list = true;  // reaction phases over the members' lists (LMX_BATCH_REACTION_LIST)
     // device descriptors and block prefixes (x, y, z, reaction), sized for cap members
     uint32_t cap = 0;
     lmx::Member *d_members = nullptr, *h_members = nullptr;
@@ -1377,8 +1381,10 @@ int member_ready(lmx_solver *s, bool may_alloc)
 }
 
 // Descriptor of one member from its host-side state (buffer parity, tables)
-// and the grid sizes of its share of the four batched launches.
-void fill_member(const lmx_solver *s, lmx::Member &m, uint32_t blocks[4])
+// and the grid sizes of its share of the four batched launches.  With
+// use_list, a member without zero-order reactions runs its reaction phase
+// over the list its z sweep fills, as a single solver does by default.
+void fill_member(const lmx_solver *s, bool use_list, lmx::Member &m, uint32_t blocks[4])
 {
     m.p = make_params(s);
     if (s->flag_words > 0) m.p.plane_flags = 1u;  // the batched kernels are the default ones
@@ -1392,7 +1398,20 @@ void fill_member(const lmx_solver *s, lmx::Member &m, uint32_t blocks[4])
     blocks[0] = lmx::x4_blocks(m.p);
     blocks[1] = lmx::tile_blocks(m.geo[0]);
     blocks[2] = lmx::tile_blocks(m.geo[1]);
-    blocks[3] = lmx::reaction_blocks(m.p);
+    m.rl = lmx::ReactionList{};
+    m.nshards = 0;
+    if (s->R == 0) {
+        m.react = lmx::kReactNone;
+        blocks[3] = 0;
+    } else if (use_list && !s->any_qp0 && s->rlist.idx) {
+        m.react = lmx::kReactList;
+        m.rl = s->rlist;
+        m.nshards = lmx::batch_list_shards(m.geo[1], s->W);
+        blocks[3] = lmx::batch_list_blocks(m.nshards);
+    } else {
+        m.react = lmx::kReactFull;
+        blocks[3] = lmx::reaction_blocks(m.p);
+    }
 }
 
 // After nsteps batched steps were enqueued: the members' host-side state
@@ -1441,7 +1460,7 @@ int batch_prepare(lmx_batch *b)
         int rc = member_ready(s, true);
         if (rc) return rc;
         uint32_t blocks[4];
-        fill_member(s, b->h_members[r], blocks);
+        fill_member(s, b->use_list, b->h_members[r], blocks);
         for (int k = 0; k < 4; k++) {
             b->h_start[k * (b->cap + 1) + r] = acc[k];
             acc[k] += blocks[k];
@@ -1586,6 +1605,11 @@ int lmx_batch_set_option(lmx_batch *b, int option, int value)
     if (option == LMX_BATCH_GRAPH) {
         b->use_graph = value != 0;
         if (!b->use_graph) batch_drop_graph(b);
+        return LMX_OK;
+    }
+    if (option == LMX_BATCH_REACTION_LIST) {
+        if (b->use_list != (value != 0)) batch_drop_graph(b);
+        b->use_list = value != 0;
         return LMX_OK;
     }
     return fail(LMX_ERR_ARG, "unknown batch option");

@@ -181,6 +181,24 @@ __device__ inline unsigned long long group_occ(const Group4<W> &G)
     return m;
 }
 
+// group_occ for a step input that may be packed: a packed site's particles
+// fill slots 0..n-1, so its mask is the n lowest bits, with n counted from
+// the highest nonzero byte of each word.
+template <int W>
+__device__ inline unsigned long long group_occ_in(const Params &p, const Group4<W> &G)
+{
+    if (!p.packed) return group_occ<W>(G);
+    unsigned long long m = 0;
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+        uint32_t n = 0;
+#pragma unroll
+        for (int k = 0; k < W; k++) n += (39u - (uint32_t)__clz((int)lane_of(G.w[k], s))) >> 3;
+        m |= (unsigned long long)((1u << n) - 1u) << (s * 4 * W);
+    }
+    return m;
+}
+
 template <int W>
 __device__ inline void site_words(const Group4<W> &G, int s, uint32_t *out)
 {
@@ -395,7 +413,7 @@ __device__ inline void x4_body(const Params &p, const uint32_t *__restrict__ src
     }
 
     const unsigned long long h = timestep_hash(p.seed32, p.ctl->timestep, 0);
-    const unsigned long long occ = active ? group_occ<W>(G) : 0ull;
+    const unsigned long long occ = active ? group_occ_in<W>(p, G) : 0ull;
     unsigned long long minus = 0, plus = 0;
     if (occ) {
         const uint32_t tm = (G.t << 8) | lt;
@@ -497,8 +515,8 @@ __device__ inline bool load_row(const Params &p, const uint32_t *__restrict__ sr
     } else if (rr >= (int)len) {
         if (p.boundary) { zero_group<W>(G); return false; }
         rr -= (int)len;
+        if (rr >= (int)len) rr -= (int)len;  // r = len + 1 with len = 1
     }
-    rr %= (int)len;
     load_group<W>(p, src, (base + (uint32_t)rr * stride) >> 2, G);
     return true;
 }
@@ -516,8 +534,8 @@ __device__ inline uint32_t load_row_first(const Params &p, const uint32_t *__res
     } else if (rr >= (int)len) {
         if (p.boundary) { zero_group<W>(G); return ~0u; }
         rr -= (int)len;
+        if (rr >= (int)len) rr -= (int)len;  // r = len + 1 with len = 1
     }
-    rr %= (int)len;
     const uint32_t g = (base + (uint32_t)rr * stride) >> 2;
     load_group_first<W>(p, src, g, G);
     return g;
@@ -529,7 +547,7 @@ __device__ inline unsigned long long row_index(const Params &p, uint32_t base, u
     int rr = r;
     if (rr < 0) rr += (int)len;
     if (rr >= (int)len) rr -= (int)len;
-    rr %= (int)len;
+    if (rr >= (int)len) rr -= (int)len;  // r = len + 1 with len = 1
     return base + (uint32_t)rr * stride;
 }
 
@@ -753,12 +771,20 @@ __device__ inline uint32_t wave_reserve(uint32_t *counter, uint32_t c)
 }
 
 // Reserves c < 8 entries of the reaction list for the calling lane, in the
-// shard of the calling wave; *shard receives the shard.  Every lane of the
-// wave must call it.
-__device__ inline uint32_t list_reserve(const Params &p, uint32_t c, uint32_t *shard)
+// shard of the calling wave (wave w of the launch's block `block` of this
+// solver: shard w % kListShards); *shard receives the shard.  Every lane of
+// the wave must call it.
+__device__ inline uint32_t list_reserve(const Params &p, uint32_t c, uint32_t block, uint32_t *shard)
 {
-    *shard = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) % (uint32_t)kListShards;
+    *shard = (block * (blockDim.x >> 6) + (threadIdx.x >> 6)) % (uint32_t)kListShards;
     return wave_reserve(&p.ctl->list_count[*shard], c);
+}
+
+// Shards a launch of `blocks` blocks of `threads` threads fills.
+__host__ __device__ inline uint32_t list_shards_used(uint32_t blocks, uint32_t threads)
+{
+    const unsigned long long waves = (unsigned long long)blocks * (threads / 64u);
+    return waves < (unsigned long long)kListShards ? (uint32_t)waves : (uint32_t)kListShards;
 }
 
 // Writes one reaction-list entry; false (nothing written) when the shard is full.
@@ -787,39 +813,46 @@ __device__ inline void tile_body(const Params &p, const uint32_t *__restrict__ s
                                  const TileGeom &geo, uint32_t substep, uint32_t block, const ReactionList &rl)
 {
     const uint32_t stride = geo.stride, len = geo.len, ostride = geo.ostride, nouter = geo.nouter, Gx = geo.Gx,
-                   Ty = geo.Ty, nxchunk = geo.nxchunk, ntiles = geo.ntiles, tile_first = geo.tile_first;
+                   Ty = geo.Ty, nxchunk = geo.nxchunk, ntiles = geo.ntiles, tile_first = geo.tile_first,
+                   No = geo.No, nob = geo.nob;
     constexpr int kEntries = tile_threads<W>() + 128;
     __shared__ uint4 s_w[W][kEntries];
     __shared__ unsigned long long s_m[kEntries], s_p[kEntries];
 
-    const uint32_t nblocks = nxchunk * nouter * ntiles;
+    const uint32_t nblocks = nxchunk * nob * ntiles;
     const uint32_t per_xcd = (nblocks + 7) / 8;
     const uint32_t logical = (block & 7u) * per_xcd + (block >> 3);
     if (logical >= nblocks) return;  // whole block: no barrier is skipped by part of it
     // Adjacent logical blocks take the work nearest in memory: the next row
     // tile when tiles are closer than outer rows (y sweep), else the next
-    // outer row (z sweep).
-    uint32_t tile, xc, o;
+    // outer rows (z sweep).
+    uint32_t tile, xc, ob;
     if (tile_first) {
-        tile = logical % ntiles;
-        const uint32_t rest = logical / ntiles;
-        xc = rest % nxchunk;
-        o = rest / nxchunk;
+        const uint32_t rest = fast_div(logical, geo.div_ntiles);
+        tile = logical - rest * ntiles;
+        ob = fast_div(rest, geo.div_nxchunk);
+        xc = rest - ob * nxchunk;
     } else {
-        o = logical % nouter;
-        const uint32_t rest = logical / nouter;
-        xc = rest % nxchunk;
-        tile = rest / nxchunk;
+        const uint32_t rest = fast_div(logical, geo.div_nob);
+        ob = logical - rest * nob;
+        tile = fast_div(rest, geo.div_nxchunk);
+        xc = rest - tile * nxchunk;
     }
 
+    // Thread: tile sub of the block (outer row ob * No + sub), row ly, group lx.
     const uint32_t tid = threadIdx.x;
-    const uint32_t lx = tid % Gx, ly = tid / Gx;
+    const uint32_t sub = fast_div(tid, geo.div_tile);
+    const uint32_t q = tid - sub * (Gx * Ty);
+    const uint32_t ly = fast_div(q, geo.div_Gx), lx = q - ly * Gx;
+    const uint32_t o = ob * No + sub;
     const uint32_t gx = xc * Gx + lx;
-    const bool xvalid = gx < (p.X >> 2) && ly < Ty;
+    const bool in_block = sub < No;  // LDS rows exist for this thread's tile
+    const bool xvalid = gx < (p.X >> 2) && in_block && o < nouter;
     const uint32_t r0 = tile * Ty;
     const uint32_t nrows = min(Ty, len - r0);
     const uint32_t r = r0 + ly;
     const bool valid = xvalid && ly < nrows;
+    const uint32_t sb = sub * (Ty + 2u) * Gx;  // the tile's first LDS slot (its top apron row)
     const uint32_t base = o * ostride + gx * 4u;
     const bool periodic = p.boundary == 0;
     const unsigned long long h = timestep_hash(p.seed32, p.ctl->timestep, substep);
@@ -830,31 +863,33 @@ __device__ inline void tile_body(const Params &p, const uint32_t *__restrict__ s
     if (valid) load_group_first<W>(p, src, gG, G);
     const bool top = valid && ly == 0;
     const bool bottom = valid && ly == nrows - 1;
+    // Site types of the neighbouring rows along the axis, read directly (the
+    // block loads those rows itself, so these hit in cache) and issued with
+    // the group's own loads: the decisions then need no barrier, and their
+    // table loads overlap other waves' work.  For the edge rows of the tile
+    // these are the apron rows' types.
+    const bool hm_ok = valid && (r > 0 || periodic), hp_ok = valid && (r + 1 < len || periodic);
+    const uint32_t *sites32 = reinterpret_cast<const uint32_t *>(p.sites);
+    uint32_t tmy = 0, tpy = 0;
+    if (hm_ok) tmy = sites32[(base + (r > 0 ? r - 1 : len - 1) * stride) >> 2];
+    if (hp_ok) tpy = sites32[(base + (r + 1 < len ? r + 1 : 0u) * stride) >> 2];
     Group4<W> Ap, Bp;  // apron rows above and below the tile
     zero_group<W>(Ap);
     zero_group<W>(Bp);
     uint32_t gA = ~0u, gB = ~0u;
     if (top) gA = load_row_first<W>(p, src, base, stride, len, (int)r0 - 1, Ap);
     if (bottom) gB = load_row_first<W>(p, src, base, stride, len, (int)(r0 + nrows), Bp);
-    // Site types of the neighbouring rows along the axis, read directly (the
-    // block loads those rows itself, so these hit in cache): the decisions
-    // then need no barrier, and their table loads overlap other waves' work.
-    const bool hm_ok = valid && (r > 0 || periodic), hp_ok = valid && (r + 1 < len || periodic);
-    const uint32_t *sites32 = reinterpret_cast<const uint32_t *>(p.sites);
-    uint32_t tmy = 0, tpy = 0;
-    if (hm_ok) tmy = top ? Ap.t : sites32[(base + (r > 0 ? r - 1 : len - 1) * stride) >> 2];
-    if (hp_ok) tpy = bottom ? Bp.t : sites32[(base + (r + 1 < len ? r + 1 : 0u) * stride) >> 2];
     if (valid) load_group_rest<W>(p, src, gG, G);
     if (gA != ~0u) load_group_rest<W>(p, src, gA, Ap);
     if (gB != ~0u) load_group_rest<W>(p, src, gB, Bp);
 
-    const unsigned long long occ = valid ? group_occ<W>(G) : 0ull;
+    const unsigned long long occ = valid ? group_occ_in<W>(p, G) : 0ull;
     unsigned long long mi = 0, pl = 0;
     if (occ) decide<W>(p, G, tmy, tpy, hm_ok ? 0xFu : 0u, hp_ok ? 0xFu : 0u, base + r * stride, h, occ, mi, pl);
-    // Slot (row+1)*Gx + lx; rows at or beyond nrows write nothing, because
-    // slot nrows+1 holds the bottom apron row.
-    const uint32_t e = (ly + 1) * Gx + lx;
-    if (ly < nrows) {
+    // Slot sb + (row+1)*Gx + lx; rows at or beyond nrows write nothing,
+    // because slot nrows+1 holds the bottom apron row.
+    const uint32_t e = sb + (ly + 1) * Gx + lx;
+    if (in_block && ly < nrows) {
 #pragma unroll
         for (int k = 0; k < W; k++) s_w[k][e] = G.w[k];
         s_m[e] = mi;
@@ -863,24 +898,24 @@ __device__ inline void tile_body(const Params &p, const uint32_t *__restrict__ s
     if (top) {
         // Row r0-1: only its plus movers enter the tile.
         unsigned long long am = 0, ap = 0;
-        const unsigned long long ao = group_occ<W>(Ap);
+        const unsigned long long ao = group_occ_in<W>(p, Ap);
         if (ao) {
             const int ra = (int)r0 - 1;
             decide<W>(p, Ap, 0u, G.t, 0u, 0xFu, (uint32_t)row_index<W>(p, base, stride, len, ra), h, ao, am, ap);
         }
 #pragma unroll
-        for (int k = 0; k < W; k++) s_w[k][lx] = Ap.w[k];
-        s_p[lx] = ap;
+        for (int k = 0; k < W; k++) s_w[k][sb + lx] = Ap.w[k];
+        s_p[sb + lx] = ap;
     }
     if (bottom) {
         // Row r0+nrows: only its minus movers enter the tile.
         unsigned long long bm = 0, bpl = 0;
-        const unsigned long long bo = group_occ<W>(Bp);
+        const unsigned long long bo = group_occ_in<W>(p, Bp);
         if (bo) {
             const int rb = (int)(r0 + nrows);
             decide<W>(p, Bp, G.t, 0u, 0xFu, 0u, (uint32_t)row_index<W>(p, base, stride, len, rb), h, bo, bm, bpl);
         }
-        const uint32_t eb = (nrows + 1) * Gx + lx;
+        const uint32_t eb = sb + (nrows + 1) * Gx + lx;
 #pragma unroll
         for (int k = 0; k < W; k++) s_w[k][eb] = Bp.w[k];
         s_m[eb] = bm;
@@ -932,7 +967,7 @@ __device__ inline void tile_body(const Params &p, const uint32_t *__restrict__ s
             }
         }
         uint32_t shard;
-        uint32_t slot = list_reserve(p, (uint32_t)__builtin_popcount(occ_out), &shard);
+        uint32_t slot = list_reserve(p, (uint32_t)__builtin_popcount(occ_out), block, &shard);
         bool full = false;
         while (occ_out) {
             const uint32_t s = (uint32_t)__builtin_ctz(occ_out);
@@ -1170,7 +1205,7 @@ __global__ __launch_bounds__(kSparseThreads) void sweep_sparse_kernel(
         if (idx >= nwork) {
             if (REACT) {
                 uint32_t shard;
-                list_reserve(p, 0u, &shard);
+                list_reserve(p, 0u, blockIdx.x, &shard);
             }
             continue;
         }
@@ -1204,7 +1239,7 @@ __global__ __launch_bounds__(kSparseThreads) void sweep_sparse_kernel(
             uint32_t any = 0, shard;
 #pragma unroll
             for (int q = 0; q < W; q++) any |= out[q];
-            const uint32_t slot = list_reserve(p, any != 0u ? 1u : 0u, &shard);
+            const uint32_t slot = list_reserve(p, any != 0u ? 1u : 0u, blockIdx.x, &shard);
             if (any != 0u && !list_put<W>(rl, shard, slot, i, out, (s_t[e] >> (8 * sb)) & 0xFFu))
                 p.ctl->list_overflow = 1u;
         }
@@ -1331,20 +1366,27 @@ __global__ __launch_bounds__(256) void reaction4_kernel(Params p, uint32_t *__re
 // cannot react without zero-order reactions, so the result equals the full
 // pass).  Block b serves shards b, b + gridDim.x, ...  If any shard
 // overflowed, the kernel runs the full in-place pass instead.
+// Block `block` of `nblocks` serves the shards below nshards (the shards the
+// z sweep filled): one shard at a time with all its threads (shards of many
+// entries), or with by_wave one shard per wave (short shards, as the small
+// lattices of a batch fill them).
 template <int W>
-__global__ __launch_bounds__(256) void reaction_list_kernel(Params p, uint32_t *__restrict__ lat, ReactionList rl)
+__device__ inline void reaction_list_body(const Params &p, uint32_t *__restrict__ lat, const ReactionList &rl,
+                                          uint32_t block, uint32_t nblocks, uint32_t nshards, bool by_wave)
 {
     if (p.ctl->list_overflow) {
         const uint32_t nblk = ((p.N >> 2) + 256u * kRG - 1u) / (256u * kRG);
-        for (uint32_t blk = blockIdx.x; blk < nblk; blk += gridDim.x) reaction4_body<W>(p, lat, blk);
+        for (uint32_t blk = block; blk < nblk; blk += nblocks) reaction4_body<W>(p, lat, blk);
         return;
     }
     const float2 *tab = reinterpret_cast<const float2 *>(p.prob1);
     const unsigned long long h3 = timestep_hash(p.seed32, p.ctl->timestep, 3);
     const size_t total = (size_t)kListShards * rl.cap;
-    for (uint32_t shard = blockIdx.x; shard < (uint32_t)kListShards; shard += gridDim.x) {
+    const uint32_t units = by_wave ? (blockDim.x >> 6) : 1u, unit = by_wave ? (threadIdx.x >> 6) : 0u;
+    const uint32_t first = by_wave ? (threadIdx.x & 63u) : threadIdx.x, width = by_wave ? 64u : blockDim.x;
+    for (uint32_t shard = block * units + unit; shard < nshards; shard += nblocks * units) {
         const uint32_t n = min(p.ctl->list_count[shard], rl.cap);
-        for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) {
+        for (uint32_t k = first; k < n; k += width) {
             const size_t e = (size_t)shard * rl.cap + k;
             const uint32_t i = rl.idx[e];
             uint32_t w[W];
@@ -1359,6 +1401,13 @@ __global__ __launch_bounds__(256) void reaction_list_kernel(Params p, uint32_t *
             }
         }
     }
+}
+
+template <int W>
+__global__ __launch_bounds__(256) void reaction_list_kernel(Params p, uint32_t *__restrict__ lat, ReactionList rl,
+                                                            uint32_t nshards)
+{
+    reaction_list_body<W>(p, lat, rl, blockIdx.x, gridDim.x, nshards, false);
 }
 
 // Probability tables for sites with at most one particle, built with the
@@ -1466,7 +1515,13 @@ __global__ __launch_bounds__(fused_threads<W>()) void fused_step_kernel(Params p
                              h2 = timestep_hash(p.seed32, t, 2), h3 = timestep_hash(p.seed32, t, 3);
 
     auto plane_exists = [&](int z) { return periodic || (z >= 0 && z < (int)Z); };
-    auto row_base = [&](int z) { return (uint32_t)((z + (int)Z) % (int)Z) * XY + y * X + 4u * g; };
+    // z in [-1, Z]: wrapped without a division
+    auto row_base = [&](int z) {
+        uint32_t zz = (uint32_t)(z + (int)Z);
+        zz = zz >= Z ? zz - Z : zz;
+        zz = zz >= Z ? zz - Z : zz;
+        return zz * XY + y * X + 4u * g;
+    };
     auto fetch = [&](int z, Group4<W> &G) {
         zero_group<W>(G);
         if (row_exists && plane_exists(z)) load_group<W>(p, src, row_base(z) >> 2, G);
@@ -1798,7 +1853,7 @@ static hipError_t launch_tile(const Params &p, const uint32_t *src, uint32_t *ds
                               const ReactionList &rl, hipStream_t st)
 {
     const TileGeom geo = tile_geom(p, axis, (uint32_t)tile_threads<W>());
-    const dim3 grid(tile_blocks(geo)), block(geo.Gx * geo.Ty);
+    const dim3 grid(tile_blocks(geo)), block(geo.Gx * geo.Ty * geo.No);
     if (react)
         hipLaunchKernelGGL((sweep_tile_kernel<W, true>), grid, block, 0, st, p, src, dst, geo, (uint32_t)axis, rl);
     else
@@ -1899,9 +1954,9 @@ hipError_t launch_reaction_list(const Params &p, uint32_t *lat, const ReactionLi
     const unsigned nb = (unsigned)std::min(kListShards, (num_cu > 0 ? num_cu : 64) * 4);
     const dim3 grid(nb), block(256);
     switch (p.W) {
-    case 1: hipLaunchKernelGGL(reaction_list_kernel<1>, grid, block, 0, st, p, lat, rl); break;
-    case 2: hipLaunchKernelGGL(reaction_list_kernel<2>, grid, block, 0, st, p, lat, rl); break;
-    case 4: hipLaunchKernelGGL(reaction_list_kernel<4>, grid, block, 0, st, p, lat, rl); break;
+    case 1: hipLaunchKernelGGL(reaction_list_kernel<1>, grid, block, 0, st, p, lat, rl, (uint32_t)kListShards); break;
+    case 2: hipLaunchKernelGGL(reaction_list_kernel<2>, grid, block, 0, st, p, lat, rl, (uint32_t)kListShards); break;
+    case 4: hipLaunchKernelGGL(reaction_list_kernel<4>, grid, block, 0, st, p, lat, rl, (uint32_t)kListShards); break;
     default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -1943,6 +1998,8 @@ __global__ __launch_bounds__(256) void batch_x4_kernel(const Member *__restrict_
     x4_body<W>(M.p, M.lat[c], M.lat[c ^ 1u], blockIdx.x - start[r]);
 }
 
+// y (axis 1) or z (axis 2) sweeps; a z sweep of a member whose reaction
+// phase runs over its list fills that list.
 template <int W>
 __global__ __launch_bounds__(tile_threads<W>()) void batch_tile_kernel(const Member *__restrict__ m,
                                                                        const uint32_t *__restrict__ start,
@@ -1953,21 +2010,40 @@ __global__ __launch_bounds__(tile_threads<W>()) void batch_tile_kernel(const Mem
     const uint32_t c = M.cur;
     const uint32_t *src = axis == 1 ? M.lat[c ^ 1u] : M.lat[c];
     uint32_t *dst = axis == 1 ? M.lat[c] : M.lat[c ^ 1u];
-    const ReactionList none{};
-    tile_body<W, false>(M.p, src, dst, M.geo[axis - 1], axis, blockIdx.x - start[r], none);
+    const uint32_t block = blockIdx.x - start[r];
+    if (axis == 2 && M.react == kReactList) {
+        tile_body<W, true>(M.p, src, dst, M.geo[1], axis, block, M.rl);
+    } else {
+        const ReactionList none{};
+        tile_body<W, false>(M.p, src, dst, M.geo[axis - 1], axis, block, none);
+    }
 }
 
+// Reaction phase: the full pass, or the member's list (its share of blocks
+// walks the shards its z sweep filled).
 template <int W>
 __global__ __launch_bounds__(256) void batch_reaction_kernel(const Member *__restrict__ m,
                                                              const uint32_t *__restrict__ start, uint32_t n)
 {
     const uint32_t r = find_member(start, n, blockIdx.x);
     const Member &M = m[r];
-    reaction4_body<W>(M.p, M.lat[M.cur ^ 1u], blockIdx.x - start[r]);
+    const uint32_t block = blockIdx.x - start[r];
+    if (M.react == kReactList)
+        reaction_list_body<W>(M.p, M.lat[M.cur ^ 1u], M.rl, block, start[r + 1] - start[r], M.nshards, true);
+    else
+        reaction4_body<W>(M.p, M.lat[M.cur ^ 1u], block);
 }
 
 uint32_t x4_blocks(const Params &p) { return blocks_for(p.N >> 2, 256); }
 uint32_t reaction_blocks(const Params &p) { return p.R > 0 ? blocks_for(p.N >> 2, 256 * kRG) : 0u; }
+
+// List shards a batched z sweep of this geometry fills, and the list blocks
+// (four waves of about two shards each) that serve them.
+uint32_t batch_list_shards(const TileGeom &zgeo, uint32_t W)
+{
+    return list_shards_used(tile_blocks(zgeo), tile_threads_for(W));
+}
+uint32_t batch_list_blocks(uint32_t nshards) { return (nshards + 7u) / 8u; }
 
 hipError_t launch_batch_sweeps(const Member *m, const uint32_t *start_x, const uint32_t *start_y,
                                const uint32_t *start_z, const uint32_t *start_r, uint32_t n, const uint32_t *total,

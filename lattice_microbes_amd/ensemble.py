@@ -40,6 +40,11 @@ class Ensemble:
     def set_graph(self, on: bool) -> None:
         _lib.check(self._lib.lmx_batch_set_option(self._h, _lib.BATCH_GRAPH, 1 if on else 0))
 
+    def set_reaction_list(self, on: bool) -> None:
+        """Reaction phases over the occupied sites each member's z sweep lists
+        (default), or the full in-place pass; the results are identical."""
+        _lib.check(self._lib.lmx_batch_set_option(self._h, _lib.BATCH_REACTION_LIST, 1 if on else 0))
+
     def add(self, solver: MpdRdmeSolver) -> None:
         _lib.check(self._lib.lmx_batch_add(self._h, solver._h))
         self.members.append(solver)

@@ -11,11 +11,10 @@ alone: each member of an Ensemble is bit-identical to its plain run.
 """
 from __future__ import annotations
 
-import collections
 import hashlib
 import time
 from dataclasses import dataclass, field
-from typing import Callable, Deque, Dict, List, Optional
+from typing import Callable, Dict, List, Optional
 
 import numpy as np
 
@@ -75,15 +74,34 @@ class Server:
     """Admits, batches and retires jobs; see the module docstring."""
 
     def __init__(self, device: int = 0, hbm_budget: int = 64 << 30, chunk: int = 250, graph: bool = True,
-                 keep_digest: bool = True, max_active: Optional[int] = None,
+                 keep_digest: bool = True, max_active: Optional[int] = None, max_sites: Optional[int] = None,
+                 policy: str = "fcfs", starvation_s: float = 5.0,
                  clock: Callable[[], float] = time.perf_counter):
+        """hbm_budget: device bytes the admitted jobs may hold (footprint());
+        max_active: at most this many jobs at once; max_sites: at most this
+        many lattice sites stepped per batched step (beyond a few whole-cell
+        lattices the GPU is saturated and more jobs only slow every one);
+        policy: "fcfs" admits in arrival order, "sjf" the waiting job with the
+        least work (sites x steps) first, except that a job waiting longer
+        than starvation_s goes first."""
+        if policy not in ("fcfs", "sjf"):
+            raise ValueError(f"unknown admission policy {policy!r}")
         self.device = device
         self.hbm_budget = int(hbm_budget)
         self.chunk = int(chunk)
         self.graph = graph
         self.keep_digest = keep_digest
         self.max_active = max_active
+        self.max_sites = max_sites
+        self.policy = policy
+        self.starvation_s = float(starvation_s)
         self.clock = clock
+
+    def _candidates(self, waiting: List[Job], now: float) -> List[Job]:
+        """Waiting jobs in the order admission considers them."""
+        if self.policy == "fcfs":
+            return list(waiting)
+        return sorted(waiting, key=lambda j: (now - j.arrival <= self.starvation_s, j.site_steps, j.arrival))
 
     def _admit(self, job: Job, ens: Ensemble, now: float) -> None:
         s = MpdRdmeSolver(self.device)
@@ -106,9 +124,10 @@ class Server:
         """Serves jobs (arrival times relative to the call); returns summary
         statistics.  Every job gets finished, counts and digest set."""
         order = sorted(jobs, key=lambda j: j.arrival)
-        waiting: Deque[Job] = collections.deque()
+        waiting: List[Job] = []
         active: List[Job] = []
         used = 0
+        sites = 0
         nxt = 0
         ens = Ensemble(self.device, graph=self.graph)
         busy = 0.0
@@ -125,15 +144,26 @@ class Server:
                 while nxt < len(order) and order[nxt].arrival <= now:
                     waiting.append(order[nxt])
                     nxt += 1
-                # Admission: first come, first served, while the footprint fits
-                # (a job larger than the budget runs when nothing else does);
-                # admitted jobs join from the next chunk on.
+                # Admission, in the policy's order, of every candidate whose
+                # footprint and sites fit (a job larger than the limits runs
+                # when nothing else does; under fcfs a job that does not fit
+                # holds back the ones behind it); admitted jobs join from the
+                # next chunk on.
                 joining = []
-                while waiting and (self.max_active is None or len(active) + len(joining) < self.max_active) and \
-                        (used + footprint(waiting[0].spec) <= self.hbm_budget or not (active or joining)):
-                    job = waiting.popleft()
+                for job in self._candidates(waiting, now):
+                    if self.max_active is not None and len(active) + len(joining) >= self.max_active:
+                        break
+                    alone = not (active or joining)
+                    fits = used + footprint(job.spec) <= self.hbm_budget and \
+                        (self.max_sites is None or sites + job.spec.N <= self.max_sites)
+                    if not (fits or alone):
+                        if self.policy == "fcfs":
+                            break
+                        continue
+                    waiting.remove(job)
                     self._admit(job, ens, self.clock() - t0)
                     used += footprint(job.spec)
+                    sites += job.spec.N
                     joining.append(job)
                 if k:
                     ens.synchronize()
@@ -143,6 +173,7 @@ class Server:
                     for j in [j for j in active if j.done_steps >= j.steps]:
                         self._retire(j, ens, self.clock() - t0)
                         used -= footprint(j.spec)
+                        sites -= j.spec.N
                         active.remove(j)
                 active.extend(joining)
                 if not active and not waiting and nxt < len(order):

@@ -284,39 +284,45 @@ def test_fused_whole_cell_reduced(gpu):
     run_pair(spec, steps=6, seed=3, chunks=2, path=lmx._lib.PATH_FUSED)
 
 
-def test_ensemble_members_bit_exact(gpu):
+@pytest.mark.parametrize("graph,reaction_list", [(True, True), (False, True), (True, False)],
+                         ids=["graph-list", "nograph-list", "graph-fullpass"])
+def test_ensemble_members_bit_exact(gpu, graph, reaction_list):
     """A batch of uneven lattices, models and seeds: after every run each
     member equals its own oracle run (and so its plain run), with and
-    without the captured graph, also after members leave and join."""
+    without the captured graph, also after members leave and join.  Members
+    without zero-order reactions run their reaction phase over the sites
+    their z sweep lists (or, with the list off, the full pass); the others
+    always the full pass."""
     shapes = [(8, 6, 5), (12, 7, 9), (32, 12, 20), (16, 1, 3), (4, 33, 3), (64, 16, 16), (300, 4, 6), (512, 3, 4)]
-    for graph in (True, False):
-        ens = lmx.Ensemble(0, graph=graph)
-        pairs = []
-        for k, shape in enumerate(shapes):
-            spec = models.random_model(shape=shape, P=8, nS=5, nST=3, R=7, density=0.25, seed=40 + k,
-                                       boundary="reflecting" if k % 2 else "periodic")
-            s = lmx.MpdRdmeSolver(0)
-            spec.load_into(s, replicate=k, seed=9 + k)
-            orc = oracle.Oracle(spec, seed_top=9 + k, replicate=k)
-            ens.add(s)
-            pairs.append((s, orc))
-        for chunk in (1, 4, 3):
-            ens.run(chunk)
-            for s, orc in pairs:
-                assert orc.run(chunk) == 0
-                assert np.array_equal(s.getLatticeNative(), orc.native()), f"graph={graph} {s.shape}"
-        # a member leaves, steps alone, and rejoins
-        s0, o0 = pairs[0]
-        ens.remove(s0)
-        s0.runTimesteps(2)
-        assert o0.run(2) == 0
-        ens.add(s0)
-        ens.run(2)
+    ens = lmx.Ensemble(0, graph=graph)
+    ens.set_reaction_list(reaction_list)
+    pairs = []
+    for k, shape in enumerate(shapes):
+        spec = models.random_model(shape=shape, P=8, nS=5, nST=3, R=7, density=0.25, seed=40 + k,
+                                   boundary="reflecting" if k % 2 else "periodic",
+                                   include_zero_order=k % 3 == 2)
+        s = lmx.MpdRdmeSolver(0)
+        spec.load_into(s, replicate=k, seed=9 + k)
+        orc = oracle.Oracle(spec, seed_top=9 + k, replicate=k)
+        ens.add(s)
+        pairs.append((s, orc))
+    for chunk in (1, 4, 3):
+        ens.run(chunk)
         for s, orc in pairs:
-            assert orc.run(2) == 0
-            assert np.array_equal(s.getLatticeNative(), orc.native())
-            assert s.timestep == orc.timestep
-        ens.close()
+            assert orc.run(chunk) == 0
+            assert np.array_equal(s.getLatticeNative(), orc.native()), f"graph={graph} {s.shape}"
+    # a member leaves, steps alone, and rejoins
+    s0, o0 = pairs[0]
+    ens.remove(s0)
+    s0.runTimesteps(2)
+    assert o0.run(2) == 0
+    ens.add(s0)
+    ens.run(2)
+    for s, orc in pairs:
+        assert orc.run(2) == 0
+        assert np.array_equal(s.getLatticeNative(), orc.native())
+        assert s.timestep == orc.timestep
+    ens.close()
 
 
 def test_served_jobs_equal_their_solo_runs(gpu):
