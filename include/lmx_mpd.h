@@ -177,6 +177,17 @@ int lmx_autotune(lmx_solver *s, uint32_t nsteps, double *best_ms, int *chosen);
  * synchronizes every step, :1091).  Device-side errors are sticky and are
  * reported by the next lmx_synchronize. */
 int lmx_run_steps(lmx_solver *s, uint32_t nsteps);
+/* No counterpart in the reference: lmx_run_steps for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops).
+ * The same kernels and results, with the z sweep's reaction list in a static
+ * device array of the library (1024 shards x 256 entries, 5.5 MB per GPU,
+ * held until the process ends) instead of the solver's own list; the call
+ * allocates nothing.  Lattices of at most 2,097,152 sites (128^3); larger
+ * ones are rejected (LMX_ERR_ARG), as is a call after a reaction-rate change
+ * whose tables lmx_run_steps has not uploaded yet (LMX_ERR_STATE).  Every
+ * call on one GPU shares the array: calls that overlap on one GPU (solvers
+ * stepping concurrently on different streams) race. */
+int lmx_run_steps_static(lmx_solver *s, uint32_t nsteps);
 int lmx_synchronize(lmx_solver *s);
 /* Current timestep counter (number of steps taken since the lattice was set). */
 int lmx_get_timestep(lmx_solver *s, uint32_t *timestep);

@@ -66,6 +66,7 @@ SIGNATURES = {
     "lmx_set_path": (c_int, [c_void_p, c_int]),
     "lmx_set_launch_option": (c_int, [c_void_p, c_int, c_int]),
     "lmx_run_steps": (c_int, [c_void_p, c_uint32]),
+    "lmx_run_steps_static": (c_int, [c_void_p, c_uint32]),
     "lmx_synchronize": (c_int, [c_void_p]),
     "lmx_get_timestep": (c_int, [c_void_p, _u32p]),
     "lmx_set_timestep": (c_int, [c_void_p, c_uint32]),

@@ -26,6 +26,13 @@ hipError_t launch_sweep_tile(const Params &p, const uint32_t *src, uint32_t *dst
 hipError_t launch_sweep_sparse(const Params &p, const uint32_t *src, uint32_t *dst, int axis, bool react,
                                const ReactionList &rl, hipStream_t st);
 hipError_t launch_reaction_list(const Params &p, uint32_t *lat, const ReactionList &rl, int num_cu, hipStream_t st);
+// The static reaction list (lmx_run_steps_static): kListShards shards of
+// kStaticListCap entries in a device array of the library; `device` must be
+// current.  Lattices of up to kStaticListMaxSites sites use it.
+constexpr uint32_t kStaticListCap = 256;
+constexpr size_t kStaticListMaxSites = (size_t)kListShards * kStaticListCap * 8;
+constexpr int kMaxDevices = 64;
+hipError_t static_reaction_list(int device, uint32_t W, ReactionList *out);
 bool fused_ok(const Params &p);
 uint32_t fused_tile_rows(uint32_t X, uint32_t W);
 hipError_t launch_fused(const Params &p, const uint32_t *src, uint32_t *dst, uint32_t zseg, hipStream_t st);

@@ -838,6 +838,28 @@ int lmx_run_steps(lmx_solver *s, uint32_t nsteps)
     return LMX_OK;
 }
 
+int lmx_run_steps_static(lmx_solver *s, uint32_t nsteps)
+{
+    int rc = require_ready(s);
+    if (rc) return rc;
+    if (s->N > lmx::kStaticListMaxSites)
+        return fail(LMX_ERR_ARG, "lmx_run_steps_static: the lattice has " + std::to_string(s->N) +
+                                     " sites; its static reaction list serves at most " +
+                                     std::to_string(lmx::kStaticListMaxSites));
+    if (s->tables_dirty)
+        return fail(LMX_ERR_STATE, "lmx_run_steps_static: reaction rates changed since the tables were uploaded; "
+                                   "lmx_run_steps uploads them (it allocates)");
+    lmx::ReactionList rl{};
+    if (s->bpp == 1) {
+        const hipError_t e = lmx::static_reaction_list(s->device, s->W, &rl);
+        if (e != hipSuccess) return hip_fail(e, "static reaction list");
+    }
+    const lmx::Params p = make_params(s);
+    for (uint32_t i = 0; i < nsteps; i++)
+        if ((rc = enqueue_step(s, p, nullptr, nullptr, nullptr, &rl))) return rc;
+    return LMX_OK;
+}
+
 int lmx_synchronize(lmx_solver *s)
 {
     if (!s) return fail(LMX_ERR_ARG, "null solver");

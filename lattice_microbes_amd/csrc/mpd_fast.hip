@@ -30,6 +30,7 @@
 // the minus neighbour's plus movers, then the plus neighbour's minus movers,
 // each in slot order, first P kept, the rest recorded as overflow.
 #include <algorithm>
+#include <mutex>
 #include <type_traits>
 
 #include "lmx_batch.h"
@@ -1947,6 +1948,43 @@ hipError_t launch_sweep_sparse(const Params &p, const uint32_t *src, uint32_t *d
     case 4: return launch_sparse<4>(p, src, dst, axis, react, rl, st);
     }
     return hipErrorInvalidValue;
+}
+
+// ---------------------------------------------------------------------------
+// The reaction list of lmx_run_steps_static, for callers that cannot allocate
+// device memory: a file-scope device array (each GPU holds its own copy until
+// the process ends) of kListShards shards x kStaticListCap entries, laid out
+// as a solver's own list: the site indices, then W word planes (W <= 4), then
+// the type bytes.  Its device address comes from hipGetSymbolAddress, once
+// per device with that device current; host code never passes the symbol
+// itself to a kernel.  Every call on one GPU shares the array; the list
+// counters live in each solver's control block and are cleared by the
+// resolver after every step, so the array needs no clearing, but calls that
+// overlap on one GPU race on it.
+// ---------------------------------------------------------------------------
+constexpr size_t kStaticListEntries = (size_t)kListShards * kStaticListCap;
+constexpr size_t kStaticListWords = kStaticListEntries * 5 + kStaticListEntries / 4;
+static_assert(kStaticListEntries % 4 == 0, "type bytes fill whole words");
+__device__ __align__(256) static uint32_t scratch[kStaticListWords];
+
+hipError_t static_reaction_list(int device, uint32_t W, ReactionList *out)
+{
+    static std::mutex mu;
+    static uint32_t *ptr[kMaxDevices] = {};
+    if (device < 0 || device >= kMaxDevices || W < 1 || W > 4) return hipErrorInvalidValue;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!ptr[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        if (!p) return hipErrorInvalidSymbol;
+        ptr[device] = static_cast<uint32_t *>(p);
+    }
+    out->idx = ptr[device];
+    out->words = ptr[device] + kStaticListEntries;
+    out->type = reinterpret_cast<uint8_t *>(ptr[device] + kStaticListEntries * (1 + W));
+    out->cap = kStaticListCap;
+    return hipSuccess;
 }
 
 hipError_t launch_reaction_list(const Params &p, uint32_t *lat, const ReactionList &rl, int num_cu, hipStream_t st)
