@@ -308,6 +308,7 @@ def ensemble_bench(nrep: int, steps: int, seed: int, sync_steps: int):
     for s in batch:
         ens.add(s)
     ens.run(1)
+    tuned = ens.autotune(steps=8)  # launch configuration from the batch autotuner (state restored)
     ms = ens.time_steps(steps)
     batched = [digest(s) for s in batch]
     ens.close()
@@ -335,6 +336,21 @@ def ensemble_bench(nrep: int, steps: int, seed: int, sync_steps: int):
             s.runTimesteps(1)
         sync_wall += time.perf_counter() - t0
         s.close()
+    # one persistent fused diffuse+react launch per step against the sweeps,
+    # on a single replicate of each lattice size: a batch of fused steps does
+    # the same per-site work, so it can only win where the fused kernel does
+    paths = {}
+    for L in models.ENSEMBLE_SIZES:
+        spec = next(m[0] for m in mix if m[0].shape[0] == L)
+        row = {}
+        for name, path in (("sweeps", lmx._lib.PATH_SWEEPS), ("fused", lmx._lib.PATH_FUSED)):
+            s = make(spec, 0, 7)
+            s.setPath(path)
+            s.runTimesteps(2)
+            t, _ = s.timeSteps(200)
+            row[name + "_us_per_step"] = t / 200 * 1e3
+            s.close()
+        paths[f"{L}^3"] = row
     b_rate = sites * steps / (ms / 1e3)
     p_rate = sites * steps / (plain_ms / 1e3)
     sync_rate = sites * sync_steps / sync_wall
@@ -352,6 +368,8 @@ def ensemble_bench(nrep: int, steps: int, seed: int, sync_steps: int):
         "sims_per_hr_plain_stepsync": nrep / (sync_wall / sync_steps * sim_steps) * 3600.0,
         "sim_steps": sim_steps,
         "bit_exact_vs_plain": {"checked": nrep, "identical": same},
+        "fused_vs_sweeps_single_replicate": paths,
+        "autotune": tuned,
     }
 
 

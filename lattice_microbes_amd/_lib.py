@@ -36,6 +36,7 @@ OPT_FUSED_ZSEG = 4
 OPT_FUSE_REACTION = 5
 BATCH_GRAPH = 1
 BATCH_REACTION_LIST = 2
+BATCH_TILE_THREADS = 3
 
 PATH_SWEEPS = 0
 PATH_FUSED = 1
@@ -93,6 +94,7 @@ SIGNATURES = {
     "lmx_batch_run_steps": (c_int, [c_void_p, c_uint32]),
     "lmx_batch_synchronize": (c_int, [c_void_p]),
     "lmx_batch_time_steps": (c_int, [c_void_p, c_uint32, _dp]),
+    "lmx_batch_autotune": (c_int, [c_void_p, c_uint32, _dp, POINTER(c_int)]),
     "lmx_get_table_f32": (c_int, [c_void_p, c_int, _fp, c_size_t, POINTER(c_size_t)]),
     "lmx_device_exp": (c_int, [c_int, _dp, _dp, c_size_t]),
     "lmx_device_reaction_probability": (c_int, [c_int, _fp, _fp, c_size_t]),

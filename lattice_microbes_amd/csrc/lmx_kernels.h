@@ -41,14 +41,16 @@ hipError_t launch_prob_tables(const Params &p, float *prob1, float *prob0, hipSt
 uint32_t x4_blocks(const Params &p);
 uint32_t reaction_blocks(const Params &p);
 struct TileGeom;
-uint32_t batch_list_shards(const TileGeom &zgeo, uint32_t W);
+uint32_t batch_list_shards(const TileGeom &zgeo, uint32_t threads);
 uint32_t batch_list_blocks(uint32_t nshards);
 // One batched step without the resolver: x, y, z sweeps and (if total[3] > 0)
 // the reaction phase; start_* are device prefix arrays of n+1 block offsets,
 // total[] the four grid sizes.
+// Tiled sweeps in blocks of tile_threads (a multiple of 64, at most
+// tile_threads_for(W)), the size the members' tile geometry was made for.
 hipError_t launch_batch_sweeps(const Member *m, const uint32_t *start_x, const uint32_t *start_y,
                                const uint32_t *start_z, const uint32_t *start_r, uint32_t n, const uint32_t *total,
-                               uint32_t W, hipStream_t st);
+                               uint32_t W, uint32_t tile_threads, hipStream_t st);
 hipError_t launch_batch_resolve(Member *m, uint32_t n, hipStream_t st);
 hipError_t launch_copy(const void *src, void *dst, size_t bytes, int num_cu, hipStream_t st);
 hipError_t launch_fill(uint32_t *ptr, size_t n, uint32_t value, hipStream_t st);

@@ -1357,6 +1357,7 @@ struct lmx_batch {
     uint32_t W = 0;
     bool use_graph = true;
     bool use_list = true;  // reaction phases over the members' lists (LMX_BATCH_REACTION_LIST)
+    uint32_t tile_threads = 0;  // threads of the tiled sweeps' blocks (LMX_BATCH_TILE_THREADS; 0: the default)
     // device descriptors and block prefixes (x, y, z, reaction), sized for cap members
     uint32_t cap = 0;
     lmx::Member *d_members = nullptr, *h_members = nullptr;
@@ -1402,11 +1403,16 @@ int member_ready(lmx_solver *s, bool may_alloc)
     return LMX_OK;
 }
 
+uint32_t batch_tile_threads(const lmx_batch *b)
+{
+    return b->tile_threads ? b->tile_threads : lmx::tile_threads_for(b->W);
+}
+
 // Descriptor of one member from its host-side state (buffer parity, tables)
 // and the grid sizes of its share of the four batched launches.  With
 // use_list, a member without zero-order reactions runs its reaction phase
 // over the list its z sweep fills, as a single solver does by default.
-void fill_member(const lmx_solver *s, bool use_list, lmx::Member &m, uint32_t blocks[4])
+void fill_member(const lmx_solver *s, bool use_list, uint32_t threads, lmx::Member &m, uint32_t blocks[4])
 {
     m.p = make_params(s);
     if (s->flag_words > 0) m.p.plane_flags = 1u;  // the batched kernels are the default ones
@@ -1414,7 +1420,6 @@ void fill_member(const lmx_solver *s, bool use_list, lmx::Member &m, uint32_t bl
     m.lat[1] = s->d_lat[1];
     m.cur = (uint32_t)s->cur;
     m.pad = 0;
-    const uint32_t threads = lmx::tile_threads_for(s->W);
     m.geo[0] = lmx::tile_geom(m.p, 1, threads);
     m.geo[1] = lmx::tile_geom(m.p, 2, threads);
     blocks[0] = lmx::x4_blocks(m.p);
@@ -1428,7 +1433,7 @@ void fill_member(const lmx_solver *s, bool use_list, lmx::Member &m, uint32_t bl
     } else if (use_list && !s->any_qp0 && s->rlist.idx) {
         m.react = lmx::kReactList;
         m.rl = s->rlist;
-        m.nshards = lmx::batch_list_shards(m.geo[1], s->W);
+        m.nshards = lmx::batch_list_shards(m.geo[1], threads);
         blocks[3] = lmx::batch_list_blocks(m.nshards);
     } else {
         m.react = lmx::kReactFull;
@@ -1482,7 +1487,7 @@ int batch_prepare(lmx_batch *b)
         int rc = member_ready(s, true);
         if (rc) return rc;
         uint32_t blocks[4];
-        fill_member(s, b->use_list, b->h_members[r], blocks);
+        fill_member(s, b->use_list, batch_tile_threads(b), b->h_members[r], blocks);
         for (int k = 0; k < 4; k++) {
             b->h_start[k * (b->cap + 1) + r] = acc[k];
             acc[k] += blocks[k];
@@ -1505,7 +1510,8 @@ int batch_enqueue_step(lmx_batch *b)
     const uint32_t n = (uint32_t)b->members.size();
     const uint32_t stride = b->cap + 1;
     hipError_t e = lmx::launch_batch_sweeps(b->d_members, b->d_start, b->d_start + stride, b->d_start + 2 * stride,
-                                            b->d_start + 3 * stride, n, b->total, b->W, b->stream);
+                                            b->d_start + 3 * stride, n, b->total, b->W, batch_tile_threads(b),
+                                            b->stream);
     if (e != hipSuccess) return hip_fail(e, "batched sweep launch");
     e = lmx::launch_batch_resolve(b->d_members, n, b->stream);
     if (e != hipSuccess) return hip_fail(e, "batched resolve launch");
@@ -1593,6 +1599,8 @@ int lmx_batch_add(lmx_batch *b, lmx_solver *s)
         if (s->bpp != 1) return fail(LMX_ERR_ARG, "batched solvers need the byte lattice");
         if (!lmx::fast_path_ok(p)) return fail(LMX_ERR_ARG, "batched solvers need X % 4 == 0");
         if (!b->members.empty() && s->W != b->W) return fail(LMX_ERR_ARG, "batched solvers need the same particles per site");
+        if (b->tile_threads > lmx::tile_threads_for(s->W))
+            return fail(LMX_ERR_ARG, "the batch's tile threads exceed what solvers with this many slots per site allow");
         b->W = s->W;
         b->members.push_back(s);
         s->batch = b;
@@ -1634,6 +1642,14 @@ int lmx_batch_set_option(lmx_batch *b, int option, int value)
         b->use_list = value != 0;
         return LMX_OK;
     }
+    if (option == LMX_BATCH_TILE_THREADS) {
+        const uint32_t t = (uint32_t)value;
+        if (value != 0 && (value < 0 || t % 64u || t < 256u || (b->W && t > lmx::tile_threads_for(b->W)) || t > 1024u))
+            return fail(LMX_ERR_ARG, "tile threads: 0 (default) or a multiple of 64 from 256 up to the default");
+        if (b->tile_threads != t) batch_drop_graph(b);
+        b->tile_threads = t;
+        return LMX_OK;
+    }
     return fail(LMX_ERR_ARG, "unknown batch option");
 }
 
@@ -1667,6 +1683,120 @@ int lmx_batch_synchronize(lmx_batch *b)
         if (rc) return rc;
     }
     return LMX_OK;
+}
+
+int lmx_batch_autotune(lmx_batch *b, uint32_t nsteps, double *best_ms, int *chosen)
+{
+    return guarded([&]() -> int {
+        if (!b) return fail(LMX_ERR_ARG, "null batch");
+        if (nsteps == 0 || nsteps > 10000) return fail(LMX_ERR_ARG, "nsteps must be in 1..10000");
+        if (b->members.empty()) return fail(LMX_ERR_STATE, "the batch has no members");
+        LMX_HIP(hipSetDevice(b->device));
+        int rc = batch_prepare(b);  // members' own work done, tables current
+        if (rc) return rc;
+        LMX_HIP(hipStreamSynchronize(b->stream));
+
+        // Every member's state a step reads or advances: both buffers
+        // (planes and chunk flags), the control block, the host bookkeeping.
+        struct Saved {
+            size_t off, buf_bytes;
+            uint32_t timestep;
+            int cur;
+            bool packed, fresh;
+        };
+        std::vector<Saved> saved;
+        size_t total = 0;
+        for (lmx_solver *s : b->members) {
+            const size_t bb = (s->N * s->W + s->flag_words) * sizeof(uint32_t);
+            saved.push_back({total, bb, s->timestep, s->cur, s->packed, s->flags_fresh});
+            total += 2 * bb + sizeof(lmx::Control);
+            total = (total + 255) & ~(size_t)255;
+        }
+        struct Scratch {
+            void *p = nullptr;
+            ~Scratch() { if (p) (void)hipFree(p); }
+        } snap;
+        LMX_HIP(hipMalloc(&snap.p, total));
+        char *base = (char *)snap.p;
+        auto copy_state = [&](bool save) -> int {
+            for (size_t k = 0; k < b->members.size(); k++) {
+                lmx_solver *s = b->members[k];
+                const Saved &v = saved[k];
+                char *o = base + v.off;
+                for (int i = 0; i < 2; i++) {
+                    void *dev = s->d_lat[i], *cp = o + i * v.buf_bytes;
+                    LMX_HIP(hipMemcpyAsync(save ? cp : dev, save ? dev : cp, v.buf_bytes, hipMemcpyDeviceToDevice,
+                                           b->stream));
+                }
+                void *cc = o + 2 * v.buf_bytes;
+                LMX_HIP(hipMemcpyAsync(save ? cc : (void *)s->d_ctl, save ? (void *)s->d_ctl : cc,
+                                       sizeof(lmx::Control), hipMemcpyDeviceToDevice, b->stream));
+                if (!save) {
+                    s->timestep = v.timestep;
+                    s->cur = v.cur;
+                    s->packed = v.packed;
+                    s->flags_fresh = v.fresh;
+                }
+            }
+            return LMX_OK;
+        };
+        if ((rc = copy_state(true))) return rc;
+
+        struct Config {
+            bool list;
+            uint32_t threads;
+        };
+        const uint32_t full = lmx::tile_threads_for(b->W);
+        const Config before{b->use_list, batch_tile_threads(b)};
+        std::vector<Config> list{before};
+        for (bool l : {true, false})
+            for (uint32_t t : {full, full / 2})
+                if (l != before.list || t != before.threads) list.push_back({l, t});
+        hipEvent_t e0, e1;
+        LMX_HIP(hipEventCreate(&e0));
+        LMX_HIP(hipEventCreate(&e1));
+        struct Events {
+            hipEvent_t a, b;
+            ~Events() { (void)hipEventDestroy(a); (void)hipEventDestroy(b); }
+        } events{e0, e1};
+        std::vector<double> times(list.size(), -1.0);
+        for (int pass = 0; pass < 2 && !rc; pass++) {
+            for (size_t k = 0; k < list.size() && !rc; k++) {
+                if (b->use_list != list[k].list || batch_tile_threads(b) != list[k].threads) batch_drop_graph(b);
+                b->use_list = list[k].list;
+                b->tile_threads = list[k].threads == full ? 0u : list[k].threads;
+                if ((rc = copy_state(false))) break;
+                if ((rc = lmx_batch_run_steps(b, 1))) break;  // untimed first step (captures the graph)
+                LMX_HIP(hipEventRecord(e0, b->stream));
+                if ((rc = lmx_batch_run_steps(b, nsteps))) break;
+                LMX_HIP(hipEventRecord(e1, b->stream));
+                LMX_HIP(hipEventSynchronize(e1));
+                float ms = 0.0f;
+                LMX_HIP(hipEventElapsedTime(&ms, e0, e1));
+                const double per = (double)ms / nsteps;
+                if (times[k] < 0.0 || per < times[k]) times[k] = per;
+            }
+        }
+        size_t win = 0;
+        for (size_t k = 1; k < times.size(); k++)
+            if (times[k] >= 0.0 && times[k] < times[win]) win = k;
+        if (times[win] > 0.98 * times[0]) win = 0;
+        const Config keep = rc ? before : list[win];
+        if (b->use_list != keep.list || batch_tile_threads(b) != keep.threads) batch_drop_graph(b);
+        b->use_list = keep.list;
+        b->tile_threads = keep.threads == full ? 0u : keep.threads;
+        const int rc2 = copy_state(false);
+        LMX_HIP(hipStreamSynchronize(b->stream));
+        for (lmx_solver *s : b->members) LMX_HIP(hipStreamWaitEvent(s->stream, b->done, 0));
+        if (rc) return rc;
+        if (rc2) return rc2;
+        if (best_ms) *best_ms = times[win];
+        if (chosen) {
+            chosen[0] = keep.list ? 1 : 0;
+            chosen[1] = (int)keep.threads;
+        }
+        return lmx_batch_synchronize(b);
+    });
 }
 
 int lmx_batch_time_steps(lmx_batch *b, uint32_t nsteps, double *total_ms)

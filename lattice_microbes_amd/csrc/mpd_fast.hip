@@ -809,16 +809,25 @@ static_assert(tile_threads<2>() == 1024 && tile_threads<4>() == 512, "tile_threa
 // With REACT (z sweep) the output sites that hold particles are appended to
 // the reaction list, and reaction_list_kernel evaluates their reactions after
 // the sweep instead of a full reaction pass.
+// LDS of a tile block: the rows' words and decisions, apron rows included.
+template <int W>
+struct TileLds {
+    static constexpr int kEntries = tile_threads<W>() + 128;
+    uint4 w[W][kEntries];
+    unsigned long long m[kEntries], p[kEntries];
+};
+
 template <int W, bool REACT>
 __device__ inline void tile_body(const Params &p, const uint32_t *__restrict__ src, uint32_t *__restrict__ dst,
-                                 const TileGeom &geo, uint32_t substep, uint32_t block, const ReactionList &rl)
+                                 const TileGeom &geo, uint32_t substep, uint32_t block, const ReactionList &rl,
+                                 TileLds<W> &lds)
 {
     const uint32_t stride = geo.stride, len = geo.len, ostride = geo.ostride, nouter = geo.nouter, Gx = geo.Gx,
                    Ty = geo.Ty, nxchunk = geo.nxchunk, ntiles = geo.ntiles, tile_first = geo.tile_first,
                    No = geo.No, nob = geo.nob;
-    constexpr int kEntries = tile_threads<W>() + 128;
-    __shared__ uint4 s_w[W][kEntries];
-    __shared__ unsigned long long s_m[kEntries], s_p[kEntries];
+    auto &s_w = lds.w;
+    auto &s_m = lds.m;
+    auto &s_p = lds.p;
 
     const uint32_t nblocks = nxchunk * nob * ntiles;
     const uint32_t per_xcd = (nblocks + 7) / 8;
@@ -987,7 +996,8 @@ __global__ __launch_bounds__(tile_threads<W>()) void sweep_tile_kernel(Params p,
                                                                        uint32_t *__restrict__ dst, TileGeom geo,
                                                                        uint32_t substep, ReactionList rl)
 {
-    tile_body<W, REACT>(p, src, dst, geo, substep, blockIdx.x, rl);
+    __shared__ TileLds<W> lds;
+    tile_body<W, REACT>(p, src, dst, geo, substep, blockIdx.x, rl, lds);
 }
 
 // ---------------------------------------------------------------------------
@@ -1371,15 +1381,12 @@ __global__ __launch_bounds__(256) void reaction4_kernel(Params p, uint32_t *__re
 // z sweep filled): one shard at a time with all its threads (shards of many
 // entries), or with by_wave one shard per wave (short shards, as the small
 // lattices of a batch fill them).
+// (A shard that overflowed makes the reaction phase the full pass instead:
+// the callers check Control::list_overflow.)
 template <int W>
 __device__ inline void reaction_list_body(const Params &p, uint32_t *__restrict__ lat, const ReactionList &rl,
                                           uint32_t block, uint32_t nblocks, uint32_t nshards, bool by_wave)
 {
-    if (p.ctl->list_overflow) {
-        const uint32_t nblk = ((p.N >> 2) + 256u * kRG - 1u) / (256u * kRG);
-        for (uint32_t blk = block; blk < nblk; blk += nblocks) reaction4_body<W>(p, lat, blk);
-        return;
-    }
     const float2 *tab = reinterpret_cast<const float2 *>(p.prob1);
     const unsigned long long h3 = timestep_hash(p.seed32, p.ctl->timestep, 3);
     const size_t total = (size_t)kListShards * rl.cap;
@@ -1408,6 +1415,11 @@ template <int W>
 __global__ __launch_bounds__(256) void reaction_list_kernel(Params p, uint32_t *__restrict__ lat, ReactionList rl,
                                                             uint32_t nshards)
 {
+    if (p.ctl->list_overflow) {
+        const uint32_t nblk = ((p.N >> 2) + 256u * kRG - 1u) / (256u * kRG);
+        for (uint32_t blk = blockIdx.x; blk < nblk; blk += gridDim.x) reaction4_body<W>(p, lat, blk);
+        return;
+    }
     reaction_list_body<W>(p, lat, rl, blockIdx.x, gridDim.x, nshards, false);
 }
 
@@ -2049,11 +2061,12 @@ __global__ __launch_bounds__(tile_threads<W>()) void batch_tile_kernel(const Mem
     const uint32_t *src = axis == 1 ? M.lat[c ^ 1u] : M.lat[c];
     uint32_t *dst = axis == 1 ? M.lat[c] : M.lat[c ^ 1u];
     const uint32_t block = blockIdx.x - start[r];
+    __shared__ TileLds<W> lds;  // one LDS block for both instantiations
     if (axis == 2 && M.react == kReactList) {
-        tile_body<W, true>(M.p, src, dst, M.geo[1], axis, block, M.rl);
+        tile_body<W, true>(M.p, src, dst, M.geo[1], axis, block, M.rl, lds);
     } else {
         const ReactionList none{};
-        tile_body<W, false>(M.p, src, dst, M.geo[axis - 1], axis, block, none);
+        tile_body<W, false>(M.p, src, dst, M.geo[axis - 1], axis, block, none, lds);
     }
 }
 
@@ -2065,36 +2078,41 @@ __global__ __launch_bounds__(256) void batch_reaction_kernel(const Member *__res
 {
     const uint32_t r = find_member(start, n, blockIdx.x);
     const Member &M = m[r];
-    const uint32_t block = blockIdx.x - start[r];
-    if (M.react == kReactList)
-        reaction_list_body<W>(M.p, M.lat[M.cur ^ 1u], M.rl, block, start[r + 1] - start[r], M.nshards, true);
-    else
-        reaction4_body<W>(M.p, M.lat[M.cur ^ 1u], block);
+    const uint32_t block = blockIdx.x - start[r], nb = start[r + 1] - start[r];
+    uint32_t *lat = M.lat[M.cur ^ 1u];
+    if (M.react == kReactList && !M.p.ctl->list_overflow) {
+        reaction_list_body<W>(M.p, lat, M.rl, block, nb, M.nshards, true);
+        return;
+    }
+    // the full pass (this member's blocks stride over its reaction blocks)
+    const uint32_t nblk = ((M.p.N >> 2) + 256u * kRG - 1u) / (256u * kRG);
+    for (uint32_t blk = block; blk < nblk; blk += nb) reaction4_body<W>(M.p, lat, blk);
 }
 
 uint32_t x4_blocks(const Params &p) { return blocks_for(p.N >> 2, 256); }
 uint32_t reaction_blocks(const Params &p) { return p.R > 0 ? blocks_for(p.N >> 2, 256 * kRG) : 0u; }
 
-// List shards a batched z sweep of this geometry fills, and the list blocks
-// (four waves of about two shards each) that serve them.
-uint32_t batch_list_shards(const TileGeom &zgeo, uint32_t W)
+// List shards a batched z sweep of this geometry fills (blocks of `threads`),
+// and the list blocks (four waves of about two shards each) that serve them.
+uint32_t batch_list_shards(const TileGeom &zgeo, uint32_t threads)
 {
-    return list_shards_used(tile_blocks(zgeo), tile_threads_for(W));
+    return list_shards_used(tile_blocks(zgeo), threads);
 }
 uint32_t batch_list_blocks(uint32_t nshards) { return (nshards + 7u) / 8u; }
 
 hipError_t launch_batch_sweeps(const Member *m, const uint32_t *start_x, const uint32_t *start_y,
                                const uint32_t *start_z, const uint32_t *start_r, uint32_t n, const uint32_t *total,
-                               uint32_t W, hipStream_t st)
+                               uint32_t W, uint32_t tile_threads_used, hipStream_t st)
 {
+    if (tile_threads_used == 0 || tile_threads_used % 64u || tile_threads_used > tile_threads_for(W))
+        return hipErrorInvalidValue;
+    const dim3 tb(tile_threads_used);
     switch (W) {
 #define LMX_BATCH_CASE(WW)                                                                                        \
     case WW:                                                                                                      \
         hipLaunchKernelGGL(batch_x4_kernel<WW>, dim3(total[0]), dim3(256), 0, st, m, start_x, n);                  \
-        hipLaunchKernelGGL(batch_tile_kernel<WW>, dim3(total[1]), dim3(tile_threads<WW>()), 0, st, m, start_y, n,  \
-                           1u);                                                                                   \
-        hipLaunchKernelGGL(batch_tile_kernel<WW>, dim3(total[2]), dim3(tile_threads<WW>()), 0, st, m, start_z, n,  \
-                           2u);                                                                                   \
+        hipLaunchKernelGGL(batch_tile_kernel<WW>, dim3(total[1]), tb, 0, st, m, start_y, n, 1u);                   \
+        hipLaunchKernelGGL(batch_tile_kernel<WW>, dim3(total[2]), tb, 0, st, m, start_z, n, 2u);                   \
         if (total[3]) hipLaunchKernelGGL(batch_reaction_kernel<WW>, dim3(total[3]), dim3(256), 0, st, m, start_r, n); \
         break;
         LMX_BATCH_CASE(1)

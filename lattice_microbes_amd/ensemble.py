@@ -8,7 +8,7 @@ member's lattice stays bit-identical to running that solver alone.
 """
 from __future__ import annotations
 
-from ctypes import byref, c_double, c_uint32, c_void_p
+from ctypes import byref, c_double, c_int, c_uint32, c_void_p
 from typing import List
 
 from . import _lib
@@ -44,6 +44,20 @@ class Ensemble:
         """Reaction phases over the occupied sites each member's z sweep lists
         (default), or the full in-place pass; the results are identical."""
         _lib.check(self._lib.lmx_batch_set_option(self._h, _lib.BATCH_REACTION_LIST, 1 if on else 0))
+
+    def set_tile_threads(self, threads: int) -> None:
+        """Threads per block of the tiled y/z sweeps (0: the default); the
+        results are identical."""
+        _lib.check(self._lib.lmx_batch_set_option(self._h, _lib.BATCH_TILE_THREADS, int(threads)))
+
+    def autotune(self, steps: int = 8) -> dict:
+        """Times the batched launch configurations (reaction list on/off, tile
+        blocks of the default and half the default threads) on the members'
+        own state, restores it, and keeps the fastest."""
+        ms = c_double()
+        chosen = (c_int * 2)()
+        _lib.check(self._lib.lmx_batch_autotune(self._h, int(steps), byref(ms), chosen))
+        return {"reaction_list": bool(chosen[0]), "tile_threads": int(chosen[1]), "ms_per_step": ms.value}
 
     def add(self, solver: MpdRdmeSolver) -> None:
         _lib.check(self._lib.lmx_batch_add(self._h, solver._h))

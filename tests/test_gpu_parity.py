@@ -284,9 +284,10 @@ def test_fused_whole_cell_reduced(gpu):
     run_pair(spec, steps=6, seed=3, chunks=2, path=lmx._lib.PATH_FUSED)
 
 
-@pytest.mark.parametrize("graph,reaction_list", [(True, True), (False, True), (True, False)],
-                         ids=["graph-list", "nograph-list", "graph-fullpass"])
-def test_ensemble_members_bit_exact(gpu, graph, reaction_list):
+@pytest.mark.parametrize("graph,reaction_list,tile_threads", [(True, True, 0), (False, True, 0), (True, False, 0),
+                                                               (True, True, 512), (True, True, 256)],
+                         ids=["graph-list", "nograph-list", "graph-fullpass", "tiles512", "tiles256"])
+def test_ensemble_members_bit_exact(gpu, graph, reaction_list, tile_threads):
     """A batch of uneven lattices, models and seeds: after every run each
     member equals its own oracle run (and so its plain run), with and
     without the captured graph, also after members leave and join.  Members
@@ -296,6 +297,7 @@ def test_ensemble_members_bit_exact(gpu, graph, reaction_list):
     shapes = [(8, 6, 5), (12, 7, 9), (32, 12, 20), (16, 1, 3), (4, 33, 3), (64, 16, 16), (300, 4, 6), (512, 3, 4)]
     ens = lmx.Ensemble(0, graph=graph)
     ens.set_reaction_list(reaction_list)
+    ens.set_tile_threads(tile_threads)
     pairs = []
     for k, shape in enumerate(shapes):
         spec = models.random_model(shape=shape, P=8, nS=5, nST=3, R=7, density=0.25, seed=40 + k,
@@ -322,6 +324,36 @@ def test_ensemble_members_bit_exact(gpu, graph, reaction_list):
         assert orc.run(2) == 0
         assert np.array_equal(s.getLatticeNative(), orc.native())
         assert s.timestep == orc.timestep
+    ens.close()
+
+
+def test_ensemble_autotune_keeps_state_and_results(gpu):
+    """The batch autotuner runs every launch configuration on the members'
+    own state and restores it: afterwards each member's lattice, step counter
+    and overflow statistics are those before the call, and the steps that
+    follow (with the configuration it kept) equal the oracle's."""
+    shapes = [(16, 12, 10), (32, 8, 6), (64, 16, 16), (12, 7, 9)]
+    ens = lmx.Ensemble(0)
+    pairs = []
+    for k, shape in enumerate(shapes):
+        spec = models.random_model(shape=shape, P=8, nS=5, nST=3, R=7, density=0.4, seed=70 + k,
+                                   include_zero_order=k == 3)
+        s = lmx.MpdRdmeSolver(0)
+        spec.load_into(s, replicate=k, seed=3 + k)
+        ens.add(s)
+        pairs.append((s, oracle.Oracle(spec, seed_top=3 + k, replicate=k)))
+    ens.run(3)
+    before = [(s.getLatticeNative(), s.timestep, s.getOverflowStats()) for s, _ in pairs]
+    tuned = ens.autotune(steps=4)
+    assert tuned["tile_threads"] in (512, 1024) and tuned["ms_per_step"] > 0
+    for (s, _), (lat, t, ovf) in zip(pairs, before):
+        assert np.array_equal(s.getLatticeNative(), lat)
+        assert s.timestep == t and s.getOverflowStats() == ovf
+    ens.run(5)
+    for s, orc in pairs:
+        assert orc.run(8) == 0
+        assert np.array_equal(s.getLatticeNative(), orc.native()), f"{s.shape}"
+        assert s.getOverflowStats() == (orc.stats.overflow_entries, orc.stats.overflow_steps)
     ens.close()
 
 
