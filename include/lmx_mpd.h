@@ -180,7 +180,7 @@ int lmx_run_steps(lmx_solver *s, uint32_t nsteps);
 /* No counterpart in the reference: lmx_run_steps for callers that cannot
  * allocate device memory (code under HIP graph capture, tight serving loops).
  * The same kernels and results, with the z sweep's reaction list in a static
- * device array of the library (1024 shards x 256 entries, 5.5 MB per GPU,
+ * device array of the library (1024 shards x 256 entries, 8 MiB per GPU,
  * held until the process ends) instead of the solver's own list; the call
  * allocates nothing.  Lattices of at most 2,097,152 sites (128^3); larger
  * ones are rejected (LMX_ERR_ARG), as is a call after a reaction-rate change

@@ -44,17 +44,17 @@ struct Control {
     uint32_t list_count[kListShards];
 };
 
-// Sites queued for the reaction phase by a fused z sweep: the site index, its
-// W words after the sweep and its type.  Shard s holds entries
-// [s*cap, (s+1)*cap) of idx and type and of each word plane
-// words[k*kListShards*cap ...]; a wave appends to shard (global wave index) %
-// kListShards.
+// Sites queued for the reaction phase by a z sweep that lists its output: one
+// record per site of list_units(W) 16-byte units, {site index, word 0, word 1,
+// type} and for W = 4 a second unit {word 2, word 3, 0, 0}, so that a list
+// entry is one vector store and one vector load.  Shard s holds records
+// [s*cap, (s+1)*cap); a wave appends to shard (wave index) % kListShards.
 struct ReactionList {
-    uint32_t *idx;
-    uint32_t *words;
-    uint8_t *type;
+    uint4 *rec;
     uint32_t cap;
 };
+
+__host__ __device__ constexpr uint32_t list_units(uint32_t W) { return W <= 2 ? 1u : 2u; }
 
 // Overflow entry key: canonical order is phase, then site index, then
 // emission order within the site; the low 11 bits hold the entry's position

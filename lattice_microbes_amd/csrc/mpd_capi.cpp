@@ -138,7 +138,7 @@ struct lmx_solver {
     lmx_batch *batch = nullptr;  // ensemble this solver is a member of, if any
     uint32_t fused_zseg = 0;  // 0: automatic
     bool fuse_reaction = true;  // the tiled z sweep lists its occupied output sites for the reaction phase
-    uint32_t *d_rl = nullptr;   // the reaction list: index and word planes, then the type bytes
+    uint4 *d_rl = nullptr;      // the reaction list: one record of list_units(W) 16-byte units per entry
     lmx::ReactionList rlist{};
     uint32_t num_cu = 256;
 };
@@ -420,7 +420,7 @@ int enqueue_step(lmx_solver *s, const lmx::Params &p0, hipEvent_t *ev, int *kind
     // phase evaluates only those (not with zero-order reactions, which make
     // every site a candidate).
     const bool z_reacts = fast && (s->yz_kernel == 0 || s->yz_kernel == 2) && s->fuse_reaction && s->R > 0 &&
-                          !s->any_qp0 && rlist.idx;
+                          !s->any_qp0 && rlist.rec;
     for (int axis = 0; axis < 3; axis++) {
         const bool react = axis == 2 && z_reacts;
         if (!fast)
@@ -712,18 +712,16 @@ int lmx_build_diffusion_model(lmx_solver *s, uint32_t nST, const double *DF, con
             e = hipMalloc((void **)&s->d_counts, (size_t)s->nS * sizeof(uint32_t));
             s->counts_cap = e == hipSuccess ? s->nS : 0u;
         }
-        // The reaction list: kListShards shards of cap entries (index, W words,
-        // type byte).  Any capacity is correct (an overflowing shard makes the
+        // The reaction list: kListShards shards of cap records (index, W words,
+        // type).  Any capacity is correct (an overflowing shard makes the
         // reaction phase a full pass); N/8 entries in total hold every occupied
         // site up to 12.5% occupancy.
         const size_t rcap = std::max<size_t>(64, (s->N / 8 + lmx::kListShards - 1) / lmx::kListShards);
         const size_t rtotal = rcap * lmx::kListShards;
         // (the byte lattice's vector kernels only)
-        if (e == hipSuccess && s->bpp == 1) e = hipMalloc((void **)&s->d_rl, rtotal * (1 + s->W) * sizeof(uint32_t) + rtotal);
+        if (e == hipSuccess && s->bpp == 1) e = hipMalloc((void **)&s->d_rl, rtotal * lmx::list_units(s->W) * sizeof(uint4));
         if (e == hipSuccess && s->d_rl) {
-            s->rlist.idx = s->d_rl;
-            s->rlist.words = s->d_rl + rtotal;
-            s->rlist.type = reinterpret_cast<uint8_t *>(s->d_rl + rtotal * (1 + s->W));
+            s->rlist.rec = s->d_rl;
             s->rlist.cap = (uint32_t)rcap;
         }
         if (e != hipSuccess) {
@@ -1430,7 +1428,7 @@ void fill_member(const lmx_solver *s, bool use_list, uint32_t threads, lmx::Memb
     if (s->R == 0) {
         m.react = lmx::kReactNone;
         blocks[3] = 0;
-    } else if (use_list && !s->any_qp0 && s->rlist.idx) {
+    } else if (use_list && !s->any_qp0 && s->rlist.rec) {
         m.react = lmx::kReactList;
         m.rl = s->rlist;
         m.nshards = lmx::batch_list_shards(m.geo[1], threads);

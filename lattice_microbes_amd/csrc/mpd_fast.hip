@@ -794,11 +794,9 @@ __device__ inline bool list_put(const ReactionList &rl, uint32_t shard, uint32_t
                                 const uint32_t *w, uint32_t st)
 {
     if (slot >= rl.cap) return false;
-    const size_t e = (size_t)shard * rl.cap + slot, total = (size_t)kListShards * rl.cap;
-    rl.idx[e] = i;
-#pragma unroll
-    for (int k = 0; k < W; k++) rl.words[(size_t)k * total + e] = w[k];
-    rl.type[e] = (uint8_t)st;
+    const size_t e = ((size_t)shard * rl.cap + slot) * list_units(W);
+    rl.rec[e] = make_uint4(i, w[0], W > 1 ? w[W > 1 ? 1 : 0] : 0u, st);
+    if constexpr (W == 4) rl.rec[e + 1] = make_uint4(w[2], w[3], 0u, 0u);
     return true;
 }
 
@@ -1371,12 +1369,11 @@ __global__ __launch_bounds__(256) void reaction4_kernel(Params p, uint32_t *__re
     reaction4_body<W>(p, lat, blockIdx.x);
 }
 
-// Reaction phase of the sites a fused z sweep listed: one evaluation per
-// entry, from the words and type in the list (the same per-site evaluation as
+// Reaction phase of the sites a z sweep listed: one evaluation per entry,
+// from the words and type in the list (the same per-site evaluation as
 // reaction4_kernel; a site's reaction reads only that site, and an empty site
 // cannot react without zero-order reactions, so the result equals the full
-// pass).  Block b serves shards b, b + gridDim.x, ...  If any shard
-// overflowed, the kernel runs the full in-place pass instead.
+// pass).
 // Block `block` of `nblocks` serves the shards below nshards (the shards the
 // z sweep filled): one shard at a time with all its threads (shards of many
 // entries), or with by_wave one shard per wave (short shards, as the small
@@ -1389,18 +1386,23 @@ __device__ inline void reaction_list_body(const Params &p, uint32_t *__restrict_
 {
     const float2 *tab = reinterpret_cast<const float2 *>(p.prob1);
     const unsigned long long h3 = timestep_hash(p.seed32, p.ctl->timestep, 3);
-    const size_t total = (size_t)kListShards * rl.cap;
     const uint32_t units = by_wave ? (blockDim.x >> 6) : 1u, unit = by_wave ? (threadIdx.x >> 6) : 0u;
     const uint32_t first = by_wave ? (threadIdx.x & 63u) : threadIdx.x, width = by_wave ? 64u : blockDim.x;
     for (uint32_t shard = block * units + unit; shard < nshards; shard += nblocks * units) {
         const uint32_t n = min(p.ctl->list_count[shard], rl.cap);
         for (uint32_t k = first; k < n; k += width) {
-            const size_t e = (size_t)shard * rl.cap + k;
-            const uint32_t i = rl.idx[e];
+            const size_t e = ((size_t)shard * rl.cap + k) * list_units(W);
+            const uint4 a = rl.rec[e];
+            const uint32_t i = a.x;
             uint32_t w[W];
-#pragma unroll
-            for (int q = 0; q < W; q++) w[q] = rl.words[(size_t)q * total + e];
-            if (react_site_table<W>(p, i, rl.type[e], w, h3, tab)) {
+            w[0] = a.y;
+            if constexpr (W > 1) w[W > 1 ? 1 : 0] = a.z;
+            if constexpr (W == 4) {
+                const uint4 b = rl.rec[e + 1];
+                w[2] = b.x;
+                w[3] = b.y;
+            }
+            if (react_site_table<W>(p, i, a.w, w, h3, tab)) {
 #pragma unroll
                 for (int q = 0; q < W; q++) {
                     lat[(size_t)q * p.N + i] = w[q];
@@ -1916,7 +1918,7 @@ hipError_t launch_sweep_col(const Params &p, const uint32_t *src, uint32_t *dst,
 hipError_t launch_sweep_tile(const Params &p, const uint32_t *src, uint32_t *dst, int axis, bool react,
                              const ReactionList &rl, hipStream_t st)
 {
-    if (axis < 1 || axis > 2 || (react && (axis != 2 || p.any_qp0 || !rl.idx || !rl.cap))) return hipErrorInvalidValue;
+    if (axis < 1 || axis > 2 || (react && (axis != 2 || p.any_qp0 || !rl.rec || !rl.cap))) return hipErrorInvalidValue;
     switch (p.W) {
     case 1: return launch_tile<1>(p, src, dst, axis, react, rl, st);
     case 2: return launch_tile<2>(p, src, dst, axis, react, rl, st);
@@ -1955,7 +1957,7 @@ static hipError_t launch_sparse(const Params &p, const uint32_t *src, uint32_t *
 hipError_t launch_sweep_sparse(const Params &p, const uint32_t *src, uint32_t *dst, int axis, bool react,
                                const ReactionList &rl, hipStream_t st)
 {
-    if (axis < 1 || axis > 2 || (react && (axis != 2 || p.any_qp0 || !rl.idx || !rl.cap))) return hipErrorInvalidValue;
+    if (axis < 1 || axis > 2 || (react && (axis != 2 || p.any_qp0 || !rl.rec || !rl.cap))) return hipErrorInvalidValue;
     switch (p.W) {
     case 1: return launch_sparse<1>(p, src, dst, axis, react, rl, st);
     case 2: return launch_sparse<2>(p, src, dst, axis, react, rl, st);
@@ -1967,9 +1969,8 @@ hipError_t launch_sweep_sparse(const Params &p, const uint32_t *src, uint32_t *d
 // ---------------------------------------------------------------------------
 // The reaction list of lmx_run_steps_static, for callers that cannot allocate
 // device memory: a file-scope device array (each GPU holds its own copy until
-// the process ends) of kListShards shards x kStaticListCap entries, laid out
-// as a solver's own list: the site indices, then W word planes (W <= 4), then
-// the type bytes.  Its device address comes from hipGetSymbolAddress, once
+// the process ends) of kListShards shards x kStaticListCap records of up to
+// two 16-byte units, laid out as a solver's own list.  Its device address comes from hipGetSymbolAddress, once
 // per device with that device current; host code never passes the symbol
 // itself to a kernel.  Every call on one GPU shares the array; the list
 // counters live in each solver's control block and are cleared by the
@@ -1977,14 +1978,12 @@ hipError_t launch_sweep_sparse(const Params &p, const uint32_t *src, uint32_t *d
 // overlap on one GPU race on it.
 // ---------------------------------------------------------------------------
 constexpr size_t kStaticListEntries = (size_t)kListShards * kStaticListCap;
-constexpr size_t kStaticListWords = kStaticListEntries * 5 + kStaticListEntries / 4;
-static_assert(kStaticListEntries % 4 == 0, "type bytes fill whole words");
-__device__ __align__(256) static uint32_t scratch[kStaticListWords];
+__device__ __align__(256) static uint4 scratch[kStaticListEntries * 2];  // records of up to 2 units (W <= 4)
 
 hipError_t static_reaction_list(int device, uint32_t W, ReactionList *out)
 {
     static std::mutex mu;
-    static uint32_t *ptr[kMaxDevices] = {};
+    static uint4 *ptr[kMaxDevices] = {};
     if (device < 0 || device >= kMaxDevices || W < 1 || W > 4) return hipErrorInvalidValue;
     std::lock_guard<std::mutex> lock(mu);
     if (!ptr[device]) {
@@ -1992,11 +1991,9 @@ hipError_t static_reaction_list(int device, uint32_t W, ReactionList *out)
         const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
         if (e != hipSuccess) return e;
         if (!p) return hipErrorInvalidSymbol;
-        ptr[device] = static_cast<uint32_t *>(p);
+        ptr[device] = static_cast<uint4 *>(p);
     }
-    out->idx = ptr[device];
-    out->words = ptr[device] + kStaticListEntries;
-    out->type = reinterpret_cast<uint8_t *>(ptr[device] + kStaticListEntries * (1 + W));
+    out->rec = ptr[device];
     out->cap = kStaticListCap;
     return hipSuccess;
 }
