@@ -964,26 +964,25 @@ __device__ inline void tile_body(const Params &p, const uint32_t *__restrict__ s
         // The output sites that hold particles go to the reaction list with
         // their words and types (without zero-order reactions an empty site
         // cannot react).
+        // Output sites are packed (particles fill word 0 first), so a site
+        // holds particles exactly when its word 0 is nonzero.
         uint32_t occ_out = 0;
         if (valid) {
 #pragma unroll
-            for (int s = 0; s < 4; s++) {
-                uint32_t a = 0;
-#pragma unroll
-                for (int k = 0; k < W; k++) a |= lane_of(res[k], s);
-                occ_out |= (a != 0u ? 1u : 0u) << s;
-            }
+            for (int s = 0; s < 4; s++) occ_out |= (lane_of(res[0], s) != 0u ? 1u : 0u) << s;
         }
         uint32_t shard;
-        uint32_t slot = list_reserve(p, (uint32_t)__builtin_popcount(occ_out), block, &shard);
+        const uint32_t slot = list_reserve(p, (uint32_t)__builtin_popcount(occ_out), block, &shard);
         bool full = false;
-        while (occ_out) {
-            const uint32_t s = (uint32_t)__builtin_ctz(occ_out);
-            occ_out &= occ_out - 1;
-            uint32_t w[W];
 #pragma unroll
-            for (int k = 0; k < W; k++) w[k] = lane_dyn(res[k], s);
-            full |= !list_put<W>(rl, shard, slot++, i0 + s, w, (G.t >> (8 * s)) & 0xFFu);
+        for (int s = 0; s < 4; s++) {
+            if (occ_out & (1u << s)) {
+                uint32_t w[W];
+#pragma unroll
+                for (int k = 0; k < W; k++) w[k] = lane_of(res[k], s);
+                full |= !list_put<W>(rl, shard, slot + (uint32_t)__builtin_popcount(occ_out & ((1u << s) - 1u)),
+                                     i0 + s, w, (G.t >> (8 * s)) & 0xFFu);
+            }
         }
         if (full) p.ctl->list_overflow = 1u;
     }
