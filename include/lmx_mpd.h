@@ -281,15 +281,16 @@ int lmx_batch_size(const lmx_batch *b, uint32_t *n);
  * reactions runs its reaction phase over the occupied sites its z sweep lists
  * (as a single solver does); 0 = the full in-place pass for every member.
  * LMX_BATCH_TILE_THREADS: threads per block of the tiled y/z sweeps, which
- * sets every member's tile shape (rows per tile, tiles per block); 0
- * (default) = 1024 (512 with 16 slots per site), else a multiple of 64 from
- * 256 up to that.  Results are identical whatever the options. */
+ * sets every member's tile shape: 0 (default) = 256 (tiles 32 sites wide and
+ * 32 rows tall), else a multiple of 64 from 256 up to 1024 (512 with 16
+ * slots per site); above 256, tiles 256 sites wide.  Results are identical
+ * whatever the options. */
 enum lmx_batch_option { LMX_BATCH_GRAPH = 1, LMX_BATCH_REACTION_LIST = 2, LMX_BATCH_TILE_THREADS = 3 };
 int lmx_batch_set_option(lmx_batch *b, int option, int value);
 /* No reference counterpart (the reference's launch shapes are build-time
  * TUNE_MPD_* constants): times nsteps batched steps (after one untimed step)
  * for each launch configuration -- reaction list on and off, tile blocks of
- * the default and half the default threads -- in two interleaved passes,
+ * 256, half the maximum and the maximum threads -- in two interleaved passes,
  * restoring every member's lattice buffers, control block and step counter
  * before each and at the end, and keeps the fastest (the current one unless
  * another is at least 2% faster).  chosen[0] = reaction list (0/1),

@@ -53,6 +53,13 @@ struct TileGeom {
 
 __host__ __device__ inline uint32_t tile_threads_for(uint32_t W) { return W == 4 ? 512u : 1024u; }
 
+// Blocks of at most kNarrowTileThreads threads take tiles kNarrowTileGx
+// groups (32 sites) wide and threads / kNarrowTileGx rows tall: eight such
+// blocks share a CU, so one block's row loads overlap another's barrier and
+// stores.  Larger blocks take tiles one wave (64 groups) wide.  The narrow
+// tile is the default of the single-solver sweeps and of batches.
+constexpr uint32_t kNarrowTileThreads = 256u, kNarrowTileGx = 8u;
+
 __host__ __device__ inline TileGeom tile_geom(const Params &p, int axis, uint32_t threads)
 {
     TileGeom g;
@@ -61,7 +68,8 @@ __host__ __device__ inline TileGeom tile_geom(const Params &p, int axis, uint32_
     g.ostride = axis == 1 ? p.XY : p.X;
     g.nouter = axis == 1 ? p.Z : p.Y;
     const uint32_t gxn = p.X >> 2;
-    g.Gx = gxn < 64u ? gxn : 64u;
+    const uint32_t gx_max = threads <= kNarrowTileThreads ? kNarrowTileGx : 64u;
+    g.Gx = gxn < gx_max ? gxn : gx_max;
     g.Ty = threads / g.Gx < g.len ? threads / g.Gx : g.len;
     g.nxchunk = (gxn + g.Gx - 1) / g.Gx;
     g.ntiles = (g.len + g.Ty - 1) / g.Ty;

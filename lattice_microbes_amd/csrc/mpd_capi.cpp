@@ -1403,7 +1403,7 @@ int member_ready(lmx_solver *s, bool may_alloc)
 
 uint32_t batch_tile_threads(const lmx_batch *b)
 {
-    return b->tile_threads ? b->tile_threads : lmx::tile_threads_for(b->W);
+    return b->tile_threads ? b->tile_threads : lmx::kNarrowTileThreads;
 }
 
 // Descriptor of one member from its host-side state (buffer parity, tables)
@@ -1643,7 +1643,7 @@ int lmx_batch_set_option(lmx_batch *b, int option, int value)
     if (option == LMX_BATCH_TILE_THREADS) {
         const uint32_t t = (uint32_t)value;
         if (value != 0 && (value < 0 || t % 64u || t < 256u || (b->W && t > lmx::tile_threads_for(b->W)) || t > 1024u))
-            return fail(LMX_ERR_ARG, "tile threads: 0 (default) or a multiple of 64 from 256 up to the default");
+            return fail(LMX_ERR_ARG, "tile threads: 0 (default, 256) or a multiple of 64 from 256 up to 1024 (512 for P = 16)");
         if (b->tile_threads != t) batch_drop_graph(b);
         b->tile_threads = t;
         return LMX_OK;
@@ -1744,12 +1744,15 @@ int lmx_batch_autotune(lmx_batch *b, uint32_t nsteps, double *best_ms, int *chos
             bool list;
             uint32_t threads;
         };
-        const uint32_t full = lmx::tile_threads_for(b->W);
+        const uint32_t full = lmx::tile_threads_for(b->W), narrow = lmx::kNarrowTileThreads;
         const Config before{b->use_list, batch_tile_threads(b)};
         std::vector<Config> list{before};
         for (bool l : {true, false})
-            for (uint32_t t : {full, full / 2})
-                if (l != before.list || t != before.threads) list.push_back({l, t});
+            for (uint32_t t : {narrow, full / 2, full}) {
+                bool seen = false;
+                for (const Config &c : list) seen = seen || (c.list == l && c.threads == t);
+                if (!seen) list.push_back({l, t});
+            }
         hipEvent_t e0, e1;
         LMX_HIP(hipEventCreate(&e0));
         LMX_HIP(hipEventCreate(&e1));
@@ -1762,7 +1765,7 @@ int lmx_batch_autotune(lmx_batch *b, uint32_t nsteps, double *best_ms, int *chos
             for (size_t k = 0; k < list.size() && !rc; k++) {
                 if (b->use_list != list[k].list || batch_tile_threads(b) != list[k].threads) batch_drop_graph(b);
                 b->use_list = list[k].list;
-                b->tile_threads = list[k].threads == full ? 0u : list[k].threads;
+                b->tile_threads = list[k].threads == narrow ? 0u : list[k].threads;
                 if ((rc = copy_state(false))) break;
                 if ((rc = lmx_batch_run_steps(b, 1))) break;  // untimed first step (captures the graph)
                 LMX_HIP(hipEventRecord(e0, b->stream));
@@ -1782,7 +1785,7 @@ int lmx_batch_autotune(lmx_batch *b, uint32_t nsteps, double *best_ms, int *chos
         const Config keep = rc ? before : list[win];
         if (b->use_list != keep.list || batch_tile_threads(b) != keep.threads) batch_drop_graph(b);
         b->use_list = keep.list;
-        b->tile_threads = keep.threads == full ? 0u : keep.threads;
+        b->tile_threads = keep.threads == narrow ? 0u : keep.threads;
         const int rc2 = copy_state(false);
         LMX_HIP(hipStreamSynchronize(b->stream));
         for (lmx_solver *s : b->members) LMX_HIP(hipStreamWaitEvent(s->stream, b->done, 0));

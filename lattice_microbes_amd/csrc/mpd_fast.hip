@@ -807,18 +807,19 @@ static_assert(tile_threads<2>() == 1024 && tile_threads<4>() == 512, "tile_threa
 // With REACT (z sweep) the output sites that hold particles are appended to
 // the reaction list, and reaction_list_kernel evaluates their reactions after
 // the sweep instead of a full reaction pass.
-// LDS of a tile block: the rows' words and decisions, apron rows included.
-template <int W>
+// LDS of a tile block of T threads: the rows' words and decisions, apron
+// rows included (No * Gx <= 64, so the apron rows take at most 128 entries).
+template <int W, int T>
 struct TileLds {
-    static constexpr int kEntries = tile_threads<W>() + 128;
+    static constexpr int kEntries = T + 128;
     uint4 w[W][kEntries];
     unsigned long long m[kEntries], p[kEntries];
 };
 
-template <int W, bool REACT>
+template <int W, bool REACT, int T>
 __device__ inline void tile_body(const Params &p, const uint32_t *__restrict__ src, uint32_t *__restrict__ dst,
                                  const TileGeom &geo, uint32_t substep, uint32_t block, const ReactionList &rl,
-                                 TileLds<W> &lds)
+                                 TileLds<W, T> &lds)
 {
     const uint32_t stride = geo.stride, len = geo.len, ostride = geo.ostride, nouter = geo.nouter, Gx = geo.Gx,
                    Ty = geo.Ty, nxchunk = geo.nxchunk, ntiles = geo.ntiles, tile_first = geo.tile_first,
@@ -989,12 +990,12 @@ __device__ inline void tile_body(const Params &p, const uint32_t *__restrict__ s
 }
 
 template <int W, bool REACT>
-__global__ __launch_bounds__(tile_threads<W>()) void sweep_tile_kernel(Params p, const uint32_t *__restrict__ src,
-                                                                       uint32_t *__restrict__ dst, TileGeom geo,
-                                                                       uint32_t substep, ReactionList rl)
+__global__ __launch_bounds__(kNarrowTileThreads) void sweep_tile_kernel(Params p, const uint32_t *__restrict__ src,
+                                                                        uint32_t *__restrict__ dst, TileGeom geo,
+                                                                        uint32_t substep, ReactionList rl)
 {
-    __shared__ TileLds<W> lds;
-    tile_body<W, REACT>(p, src, dst, geo, substep, blockIdx.x, rl, lds);
+    __shared__ TileLds<W, kNarrowTileThreads> lds;
+    tile_body<W, REACT, kNarrowTileThreads>(p, src, dst, geo, substep, blockIdx.x, rl, lds);
 }
 
 // ---------------------------------------------------------------------------
@@ -1868,7 +1869,7 @@ template <int W>
 static hipError_t launch_tile(const Params &p, const uint32_t *src, uint32_t *dst, int axis, bool react,
                               const ReactionList &rl, hipStream_t st)
 {
-    const TileGeom geo = tile_geom(p, axis, (uint32_t)tile_threads<W>());
+    const TileGeom geo = tile_geom(p, axis, kNarrowTileThreads);
     const dim3 grid(tile_blocks(geo)), block(geo.Gx * geo.Ty * geo.No);
     if (react)
         hipLaunchKernelGGL((sweep_tile_kernel<W, true>), grid, block, 0, st, p, src, dst, geo, (uint32_t)axis, rl);
@@ -2048,10 +2049,9 @@ __global__ __launch_bounds__(256) void batch_x4_kernel(const Member *__restrict_
 
 // y (axis 1) or z (axis 2) sweeps; a z sweep of a member whose reaction
 // phase runs over its list fills that list.
-template <int W>
-__global__ __launch_bounds__(tile_threads<W>()) void batch_tile_kernel(const Member *__restrict__ m,
-                                                                       const uint32_t *__restrict__ start,
-                                                                       uint32_t n, uint32_t axis)
+template <int W, int T>
+__global__ __launch_bounds__(T) void batch_tile_kernel(const Member *__restrict__ m, const uint32_t *__restrict__ start,
+                                                       uint32_t n, uint32_t axis)
 {
     const uint32_t r = find_member(start, n, blockIdx.x);
     const Member &M = m[r];
@@ -2059,12 +2059,12 @@ __global__ __launch_bounds__(tile_threads<W>()) void batch_tile_kernel(const Mem
     const uint32_t *src = axis == 1 ? M.lat[c ^ 1u] : M.lat[c];
     uint32_t *dst = axis == 1 ? M.lat[c] : M.lat[c ^ 1u];
     const uint32_t block = blockIdx.x - start[r];
-    __shared__ TileLds<W> lds;  // one LDS block for both instantiations
+    __shared__ TileLds<W, T> lds;  // one LDS block for both instantiations
     if (axis == 2 && M.react == kReactList) {
-        tile_body<W, true>(M.p, src, dst, M.geo[1], axis, block, M.rl, lds);
+        tile_body<W, true, T>(M.p, src, dst, M.geo[1], axis, block, M.rl, lds);
     } else {
         const ReactionList none{};
-        tile_body<W, false>(M.p, src, dst, M.geo[axis - 1], axis, block, none, lds);
+        tile_body<W, false, T>(M.p, src, dst, M.geo[axis - 1], axis, block, none, lds);
     }
 }
 
@@ -2105,12 +2105,22 @@ hipError_t launch_batch_sweeps(const Member *m, const uint32_t *start_x, const u
     if (tile_threads_used == 0 || tile_threads_used % 64u || tile_threads_used > tile_threads_for(W))
         return hipErrorInvalidValue;
     const dim3 tb(tile_threads_used);
+    const bool narrow = tile_threads_used <= kNarrowTileThreads;  // LDS sized for the narrow tile
     switch (W) {
 #define LMX_BATCH_CASE(WW)                                                                                        \
     case WW:                                                                                                      \
         hipLaunchKernelGGL(batch_x4_kernel<WW>, dim3(total[0]), dim3(256), 0, st, m, start_x, n);                  \
-        hipLaunchKernelGGL(batch_tile_kernel<WW>, dim3(total[1]), tb, 0, st, m, start_y, n, 1u);                   \
-        hipLaunchKernelGGL(batch_tile_kernel<WW>, dim3(total[2]), tb, 0, st, m, start_z, n, 2u);                   \
+        if (narrow) {                                                                                             \
+            hipLaunchKernelGGL((batch_tile_kernel<WW, kNarrowTileThreads>), dim3(total[1]), tb, 0, st, m, start_y, \
+                               n, 1u);                                                                            \
+            hipLaunchKernelGGL((batch_tile_kernel<WW, kNarrowTileThreads>), dim3(total[2]), tb, 0, st, m, start_z, \
+                               n, 2u);                                                                            \
+        } else {                                                                                                  \
+            hipLaunchKernelGGL((batch_tile_kernel<WW, tile_threads<WW>()>), dim3(total[1]), tb, 0, st, m, start_y, \
+                               n, 1u);                                                                            \
+            hipLaunchKernelGGL((batch_tile_kernel<WW, tile_threads<WW>()>), dim3(total[2]), tb, 0, st, m, start_z, \
+                               n, 2u);                                                                            \
+        }                                                                                                         \
         if (total[3]) hipLaunchKernelGGL(batch_reaction_kernel<WW>, dim3(total[3]), dim3(256), 0, st, m, start_r, n); \
         break;
         LMX_BATCH_CASE(1)
