@@ -81,9 +81,12 @@ __host__ __device__ inline TileGeom tile_geom(const Params &p, int axis, uint32_
     if (no > g.nouter) no = g.nouter;
     g.No = no > 0u ? no : 1u;
     g.nob = (g.nouter + g.No - 1) / g.No;
-    // Adjacent tiles share apron rows: number them consecutively when they
-    // are closer in memory than the outer rows (y sweep).
-    g.tile_first = (unsigned long long)g.Ty * g.stride < (unsigned long long)g.ostride ? 1u : 0u;
+    // Adjacent tiles share apron rows: wide tiles number them consecutively
+    // when they are closer in memory than the outer rows (y sweep).  Narrow
+    // tiles run over the outer rows first in both sweeps (on the whole-cell
+    // lattice 1.5 us faster per y and per z sweep than tiles first).
+    g.tile_first = threads > kNarrowTileThreads &&
+                   (unsigned long long)g.Ty * g.stride < (unsigned long long)g.ostride ? 1u : 0u;
     g.div_nob = make_fastdiv(g.nob);
     g.div_Gx = make_fastdiv(g.Gx);
     g.div_nxchunk = make_fastdiv(g.nxchunk);
