@@ -111,8 +111,13 @@ __host__ __device__ inline uint32_t hash32(unsigned long long v, unsigned long l
     v ^= v >> 21;
     v ^= v << 37;
     v ^= v >> 4;
-    v *= 2685821657736338717ULL;
-    return (uint32_t)(v >> 32);
+    // High word of v * C (mod 2^64) in 32-bit parts, so that the result is a
+    // 32-bit value to the compiler (its float conversion is then one
+    // instruction, not the 64-bit conversion sequence).
+    constexpr unsigned long long C = 2685821657736338717ULL;
+    const uint32_t lo = (uint32_t)v, hi = (uint32_t)(v >> 32);
+    const uint32_t clo = (uint32_t)C, chi = (uint32_t)(C >> 32);
+    return (uint32_t)(((unsigned long long)lo * clo) >> 32) + lo * chi + hi * clo;
 }
 
 __host__ __device__ inline unsigned long long timestep_hash(uint32_t seed32, uint32_t t, uint32_t substep)
@@ -176,18 +181,27 @@ __device__ inline float reaction_probability(float a)
     return (float)(1.0 - exp_proj(-(double)a));
 }
 
-// lookupTransitionProbability, byte_diffusion_1d_dev.cu:70-79.
-__device__ inline float lookup_T(const Params &p, uint32_t particle, uint32_t src, uint32_t dst)
+// lookupTransitionProbability, byte_diffusion_1d_dev.cu:70-79, in table Tt
+// (p.T or a copy of it, e.g. in LDS).  The three range tests are combined
+// into one condition, so that one branch guards the load.
+// With en false the result is 0 (a neighbour that does not exist).
+__device__ inline float lookup_T_if(const Params &p, const float *Tt, bool en, uint32_t particle, uint32_t src,
+                                    uint32_t dst)
 {
-    if ((uint32_t)(particle - 1) >= p.nS || dst >= p.nST || src >= p.nST) return 0.0f;
-    return p.T[(src * p.nST + dst) * p.nS + (particle - 1)];
+    const uint32_t sp = particle - 1u;
+    const bool ok = en & (sp < p.nS) & (dst < p.nST) & (src < p.nST);
+    return ok ? Tt[(src * p.nST + dst) * p.nS + sp] : 0.0f;
 }
 
-// The same lookup in a copy of the table (e.g. in LDS).
 __device__ inline float lookup_T_in(const Params &p, const float *Tt, uint32_t particle, uint32_t src, uint32_t dst)
 {
-    if ((uint32_t)(particle - 1) >= p.nS || dst >= p.nST || src >= p.nST) return 0.0f;
-    return Tt[(src * p.nST + dst) * p.nS + (particle - 1)];
+    return lookup_T_if(p, Tt, true, particle, src, dst);
+}
+
+// The same lookup in p.T.
+__device__ inline float lookup_T(const Params &p, uint32_t particle, uint32_t src, uint32_t dst)
+{
+    return lookup_T_in(p, p.T, particle, src, dst);
 }
 
 // ---------------------------------------------------------------------------

@@ -36,8 +36,8 @@ __device__ inline void pull_site_l(const Params &p, uint32_t i, uint32_t im, uin
             const uint32_t part = L::part(wo[w], q);
             if (part == 0) continue;
             const float u = uniform01(i, p.pcb, Q * w + q, h);
-            const float qm = has_m ? lookup_T(p, part, st, stm) : 0.0f;
-            const float qp = has_p ? lookup_T(p, part, st, stp) : 0.0f;
+            const float qm = lookup_T_if(p, p.T, has_m, part, st, stm);
+            const float qp = lookup_T_if(p, p.T, has_p, part, st, stp);
             const bool leaves = (u < qm) || (u >= 0.5f && u < (qp + 0.5f));
             if (!leaves && !b.push(part)) record_overflow(p.ctl, phase, i, b.n - 1 - L::P, part);
         }

@@ -230,9 +230,13 @@ __device__ inline uint32_t group_slot(const Group4<W> &G, uint32_t s, uint32_t j
     return (word >> (8 * (j & 3u))) & 0xFFu;
 }
 
+// Lane s (0..3) of v by two selects on the bits of s (a chain of equality
+// tests compiles to branches).
 __device__ inline uint32_t lane_dyn(const uint4 &v, uint32_t s)
 {
-    return s == 0 ? v.x : (s == 1 ? v.y : (s == 2 ? v.z : v.w));
+    const uint32_t lo = (s & 1u) ? v.y : v.x;
+    const uint32_t hi = (s & 1u) ? v.w : v.z;
+    return (s & 2u) ? hi : lo;
 }
 
 template <int W>
@@ -270,8 +274,8 @@ __device__ inline void decide_t(const Params &p, const float *Tt, const Group4<W
         const uint32_t stm = (tm >> (8 * s)) & 0xFFu;
         const uint32_t stp = (tp >> (8 * s)) & 0xFFu;
         const float u = uniform01(base + s, p.pcb, j, h);
-        const float qm = ((hm >> s) & 1u) ? lookup_T_in(p, Tt, part, st, stm) : 0.0f;
-        const float qp = ((hp >> s) & 1u) ? lookup_T_in(p, Tt, part, st, stp) : 0.0f;
+        const float qm = lookup_T_if(p, Tt, (hm >> s) & 1u, part, st, stm);
+        const float qp = lookup_T_if(p, Tt, (hp >> s) & 1u, part, st, stp);
         if (u < qm) mi |= 1ull << b;
         else if (u >= 0.5f && u < (qp + 0.5f)) pl |= 1ull << b;
     }
@@ -1180,8 +1184,8 @@ __global__ __launch_bounds__(kSparseThreads) void sweep_sparse_kernel(
             m &= m - 1;
             const uint32_t part = slot_dyn<W>(w, jb);
             const float u = uniform01(i, p.pcb, jb, h);
-            const float qm = hm ? lookup_T(p, part, st, stm) : 0.0f;
-            const float qp = hp ? lookup_T(p, part, st, stp) : 0.0f;
+            const float qm = lookup_T_if(p, p.T, hm, part, st, stm);
+            const float qp = lookup_T_if(p, p.T, hp, part, st, stp);
             if (u < qm) mi |= 1u << jb;
             else if (u >= 0.5f && u < (qp + 0.5f)) pl |= 1u << jb;
         }
