@@ -184,13 +184,17 @@ __device__ inline float reaction_probability(float a)
 // lookupTransitionProbability, byte_diffusion_1d_dev.cu:70-79, in table Tt
 // (p.T or a copy of it, e.g. in LDS).  The three range tests are combined
 // into one condition, so that one branch guards the load.
-// With en false the result is 0 (a neighbour that does not exist).
+// With en false the result is 0 (a neighbour that does not exist).  The
+// index products use 24-bit multiplies (full rate; 32-bit integer multiplies
+// are quarter rate).  Both are exact: nST <= 256 gives src * nST + dst <
+// 2^16, and the model check nST * nS^2 < 2^32 gives nS < 2^16 and a table of
+// nST^2 * nS < 2^32 entries.
 __device__ inline float lookup_T_if(const Params &p, const float *Tt, bool en, uint32_t particle, uint32_t src,
                                     uint32_t dst)
 {
     const uint32_t sp = particle - 1u;
     const bool ok = en & (sp < p.nS) & (dst < p.nST) & (src < p.nST);
-    return ok ? Tt[(src * p.nST + dst) * p.nS + sp] : 0.0f;
+    return ok ? Tt[__umul24(__umul24(src, p.nST) + dst, p.nS) + sp] : 0.0f;
 }
 
 __device__ inline float lookup_T_in(const Params &p, const float *Tt, uint32_t particle, uint32_t src, uint32_t dst)
