@@ -230,22 +230,31 @@ __device__ inline uint32_t group_slot(const Group4<W> &G, uint32_t s, uint32_t j
     return (word >> (8 * (j & 3u))) & 0xFFu;
 }
 
-// Lane s (0..3) of v by two selects on the bits of s (a chain of equality
-// tests compiles to branches).
+// Lane s (0..3) of v.  SEL: two selects on the bits of s (a chain of
+// equality tests compiles to branches, which cost the sweeps 4-6 us each).
+// The fused step and the full reaction pass keep the chain: there the
+// selects' live values cause register spills (fused step 36 -> 176 bytes
+// per lane; the reaction kernels, which hold the full pass as their
+// overflow fallback, 0 -> 144).
+template <bool SEL = true>
 __device__ inline uint32_t lane_dyn(const uint4 &v, uint32_t s)
 {
-    const uint32_t lo = (s & 1u) ? v.y : v.x;
-    const uint32_t hi = (s & 1u) ? v.w : v.z;
-    return (s & 2u) ? hi : lo;
+    if constexpr (SEL) {
+        const uint32_t lo = (s & 1u) ? v.y : v.x;
+        const uint32_t hi = (s & 1u) ? v.w : v.z;
+        return (s & 2u) ? hi : lo;
+    } else {
+        return s == 0 ? v.x : (s == 1 ? v.y : (s == 2 ? v.z : v.w));
+    }
 }
 
-template <int W>
+template <int W, bool SEL = true>
 __device__ inline uint32_t group_slot_dyn(const Group4<W> &G, uint32_t s, uint32_t j)
 {
     uint32_t word = 0;
 #pragma unroll
     for (int k = 0; k < W; k++) {
-        const uint32_t v = lane_dyn(G.w[k], s);
+        const uint32_t v = lane_dyn<SEL>(G.w[k], s);
         word = ((j >> 2) == (uint32_t)k) ? v : word;
     }
     return (word >> (8 * (j & 3u))) & 0xFFu;
@@ -257,7 +266,7 @@ __device__ inline uint32_t group_slot_dyn(const Group4<W> &G, uint32_t s, uint32
 // exists.  Bit b = s*P + j of minus/plus is set for a particle moving minus /
 // plus.
 // Tt is the transition table (p.T, or its copy in LDS).
-template <int W>
+template <int W, bool SEL = true>
 __device__ inline void decide_t(const Params &p, const float *Tt, const Group4<W> &G, uint32_t tm, uint32_t tp,
                                 uint32_t hm, uint32_t hp, uint32_t base, unsigned long long h,
                                 unsigned long long pending, unsigned long long &minus, unsigned long long &plus)
@@ -269,7 +278,7 @@ __device__ inline void decide_t(const Params &p, const float *Tt, const Group4<W
         const uint32_t b = (uint32_t)__builtin_ctzll(pending);
         pending &= pending - 1;
         const uint32_t s = b >> LOGP, j = b & (P - 1);
-        const uint32_t part = group_slot_dyn<W>(G, s, j);
+        const uint32_t part = group_slot_dyn<W, SEL>(G, s, j);
         const uint32_t st = (G.t >> (8 * s)) & 0xFFu;
         const uint32_t stm = (tm >> (8 * s)) & 0xFFu;
         const uint32_t stp = (tp >> (8 * s)) & 0xFFu;
@@ -283,12 +292,14 @@ __device__ inline void decide_t(const Params &p, const float *Tt, const Group4<W
     plus = pl;
 }
 
-template <int W>
+// SEL as in lane_dyn: the P = 4 kernels and the register march and columns
+// keep the chain (the selects cost them register spills).
+template <int W, bool SEL = (W != 1)>
 __device__ inline void decide(const Params &p, const Group4<W> &G, uint32_t tm, uint32_t tp, uint32_t hm,
                               uint32_t hp, uint32_t base, unsigned long long h, unsigned long long pending,
                               unsigned long long &minus, unsigned long long &plus)
 {
-    decide_t<W>(p, p.T, G, tm, tp, hm, hp, base, h, pending, minus, plus);
+    decide_t<W, SEL>(p, p.T, G, tm, tp, hm, hp, base, h, pending, minus, plus);
 }
 
 // New content of site i: own stayers, minus-source movers, plus-source movers.
@@ -599,10 +610,10 @@ __global__ __launch_bounds__(256) void sweep_march_kernel(Params p, const uint32
         // A's minus neighbour (row r0-2) matters only for its minus movers,
         // which never reach row r0: pass "no minus neighbour".
         const int ra = (int)r0 - 1;
-        decide<W>(p, A, 0u, B.t, 0u, hp_of(ra), (uint32_t)row_index<W>(p, base, stride, len, ra), h, occA, mA, pA);
-        decide<W>(p, B, A.t, C.t, hm_of((int)r0), hp_of((int)r0), (uint32_t)row_index<W>(p, base, stride, len, (int)r0),
+        decide<W, false>(p, A, 0u, B.t, 0u, hp_of(ra), (uint32_t)row_index<W>(p, base, stride, len, ra), h, occA, mA, pA);
+        decide<W, false>(p, B, A.t, C.t, hm_of((int)r0), hp_of((int)r0), (uint32_t)row_index<W>(p, base, stride, len, (int)r0),
                   h, occB, mB, pB);
-        decide<W>(p, C, B.t, D.t, hm_of((int)r0 + 1), hp_of((int)r0 + 1),
+        decide<W, false>(p, C, B.t, D.t, hm_of((int)r0 + 1), hp_of((int)r0 + 1),
                   (uint32_t)row_index<W>(p, base, stride, len, (int)r0 + 1), h, occC, mC, pC);
     }
     (void)mA;
@@ -645,7 +656,7 @@ __global__ __launch_bounds__(256) void sweep_march_kernel(Params p, const uint32
         unsigned long long mD = 0, pD = 0;
         if (occD) {
             const int rd = (int)r + 2;
-            decide<W>(p, D, C.t, E.t, hm_of(rd), hp_of(rd), (uint32_t)row_index<W>(p, base, stride, len, rd), h,
+            decide<W, false>(p, D, C.t, E.t, hm_of(rd), hp_of(rd), (uint32_t)row_index<W>(p, base, stride, len, rd), h,
                       occD, mD, pD);
         }
         A = B;
@@ -718,7 +729,7 @@ __global__ __launch_bounds__(256) void sweep_col_kernel(Params p, const uint32_t
             const uint32_t tp = k == SEG + 1 ? 0u : R[k + 1].t;
             const uint32_t ri = (k == 0 || k == SEG + 1) ? (uint32_t)row_index<W>(p, base, stride, len, r)
                                                          : base + (uint32_t)r * stride;
-            decide<W>(p, R[k], tm, tp, hm, hp, ri, h, oc, m, q);
+            decide<W, false>(p, R[k], tm, tp, hm, hp, ri, h, oc, m, q);
         }
         occ[k] = (mask_t)oc;
         mi[k] = (mask_t)m;
@@ -1352,7 +1363,7 @@ __device__ inline void reaction4_body(const Params &p, uint32_t *__restrict__ la
         for (int qq = 0; qq < kRG; qq++) {
             if (qq == (int)q) {
 #pragma unroll
-                for (int k = 0; k < W; k++) w[k] = lane_dyn(w4[qq][k], s);
+                for (int k = 0; k < W; k++) w[k] = lane_dyn<false>(w4[qq][k], s);
                 tq = t4[qq];
             }
         }
@@ -1647,7 +1658,7 @@ __global__ __launch_bounds__(fused_threads<W>()) void fused_step_kernel(Params p
         unsigned long long xm = 0, xp = 0;
         if (occ) {
             const uint32_t tm = (G.t << 8) | lt, tp = (G.t >> 8) | (rt << 24);
-            decide_t<W>(p, s_T, G, tm, tp, hmx ? 0xFu : 0xEu, hpx ? 0xFu : 0x7u, base, h0, occ, xm, xp);
+            decide_t<W, false>(p, s_T, G, tm, tp, hmx ? 0xFu : 0xEu, hpx ? 0xFu : 0x7u, base, h0, occ, xm, xp);
         }
         uint32_t lplus = __shfl(site_bits<W>(xp, 3), ll, 64);
         uint32_t rminus = __shfl(site_bits<W>(xm, 0), lr, 64);
@@ -1706,7 +1717,7 @@ __global__ __launch_bounds__(fused_threads<W>()) void fused_step_kernel(Params p
             const bool hp = k <= nrows && hpy;
             const uint32_t tmy = hm ? s_t[par][e - gx] : 0u;
             const uint32_t tpy = hp ? s_t[par][e + gx] : 0u;
-            decide_t<W>(p, s_T, XO, tmy, tpy, hm ? 0xFu : 0u, hp ? 0xFu : 0u, base, h1, yocc, ym, yp);
+            decide_t<W, false>(p, s_T, XO, tmy, tpy, hm ? 0xFu : 0u, hp ? 0xFu : 0u, base, h1, yocc, ym, yp);
         }
         if (work) {
             s_m[e] = (mask_t)ym;
@@ -1754,7 +1765,7 @@ __global__ __launch_bounds__(fused_threads<W>()) void fused_step_kernel(Params p
             const bool hm = zi > (int)z0 - 1 && (periodic || zi > 0);
             const bool hp = zi < (int)z1 && (periodic || zi + 1 < (int)Z);
             unsigned long long m64, p64;
-            decide_t<W>(p, s_T, Yc, hm ? Yb.t : 0u, hp ? Gn.t : 0u, hm ? 0xFu : 0u, hp ? 0xFu : 0u, base, h2, occC,
+            decide_t<W, false>(p, s_T, Yc, hm ? Yb.t : 0u, hp ? Gn.t : 0u, hm ? 0xFu : 0u, hp ? 0xFu : 0u, base, h2, occC,
                         m64, p64);
             mC = (mask_t)m64;
             pC = (mask_t)p64;
@@ -1797,7 +1808,7 @@ __global__ __launch_bounds__(fused_threads<W>()) void fused_step_kernel(Params p
                     todo &= todo - 1;
                     uint32_t w[W];
 #pragma unroll
-                    for (int q = 0; q < W; q++) w[q] = lane_dyn(res[q], s);
+                    for (int q = 0; q < W; q++) w[q] = lane_dyn<false>(res[q], s);
                     if (react_site_table<W>(p, obase + s, (Yb.t >> (8 * s)) & 0xFFu, w, h3, s_tab)) {
 #pragma unroll
                         for (int q = 0; q < W; q++) set_lane(res[q], (int)s, w[q]);
