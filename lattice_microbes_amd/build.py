@@ -18,6 +18,9 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OUT_DIR = os.path.join(HERE, "_build")
 LIB = os.path.join(OUT_DIR, "liblmx_mpd.so")
+# The compiler's per-kernel resource report (VGPRs, scratch, occupancy) of
+# the last build, for tests/test_kernel_resources.py.
+RESOURCES = os.path.join(OUT_DIR, "kernel_resources.txt")
 IO_LIB = os.path.join(OUT_DIR, "liblmx_io.so")
 
 SOURCES = ["mpd_kernels.hip", "mpd_fast.hip", "mpd_capi.cpp"]
@@ -40,7 +43,8 @@ COMMON_FLAGS = [
 def build_library(verbose: bool = False) -> str:
     os.makedirs(OUT_DIR, exist_ok=True)
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, *COMMON_FLAGS, "-shared", "-Wl,-z,defs", *[os.path.join(CSRC, s) for s in SOURCES], "-o", tmp]
+    cmd = [HIPCC, *COMMON_FLAGS, "-Rpass-analysis=kernel-resource-usage", "-shared", "-Wl,-z,defs",
+           *[os.path.join(CSRC, s) for s in SOURCES], "-o", tmp]
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
@@ -48,7 +52,28 @@ def build_library(verbose: bool = False) -> str:
         sys.stderr.write(res.stdout + res.stderr)
         raise RuntimeError("hipcc failed building liblmx_mpd.so")
     os.replace(tmp, LIB)
+    with open(RESOURCES, "w") as fh:
+        fh.write("\n".join(l for l in res.stderr.splitlines() if "kernel-resource-usage" in l) + "\n")
     return LIB
+
+
+def kernel_resources(path: str = RESOURCES) -> dict:
+    """{mangled kernel name: {"VGPRs": n, "ScratchSize": bytes per lane,
+    "Occupancy": waves per SIMD, ...}} from the last build's report."""
+    out, cur = {}, None
+    with open(path) as fh:
+        for line in fh:
+            text = line.split("remark:", 1)[-1].split("[-Rpass", 1)[0].strip()
+            if text.startswith("Function Name:"):
+                cur = out.setdefault(text.split(":", 1)[1].strip(), {})
+            elif cur is not None and ":" in text:
+                key, val = text.rsplit(":", 1)
+                key = key.split("[", 1)[0].strip()
+                try:
+                    cur[key] = int(val.strip())
+                except ValueError:
+                    cur[key] = val.strip()
+    return out
 
 
 def build_io_library(verbose: bool = False) -> str:
