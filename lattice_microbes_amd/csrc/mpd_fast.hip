@@ -1418,19 +1418,18 @@ struct ListEntry {
     float2 ap;
 };
 
-// Phase 1 of react_site_table for a list record: decode it, count its
-// particles and load the tabulated propensity of a site with at most one.
+// Phase 1 of react_site_table for a list record (units a and, for W = 4,
+// b): decode it, count its particles and load the tabulated propensity of a
+// site with at most one.
 template <int W>
-__device__ inline void list_entry_load(const Params &p, const ReactionList &rl, size_t e, const float2 *tab,
+__device__ inline void list_entry_load(const Params &p, const uint4 &a, const uint4 &b, const float2 *tab,
                                        ListEntry<W> &E)
 {
-    const uint4 a = rl.rec[e];
     E.i = a.x;
     E.st = a.w;
     E.w[0] = a.y;
     if constexpr (W > 1) E.w[W > 1 ? 1 : 0] = a.z;
     if constexpr (W == 4) {
-        const uint4 b = rl.rec[e + 1];
         E.w[2] = b.x;
         E.w[3] = b.y;
     }
@@ -1479,13 +1478,30 @@ __device__ inline void reaction_list_body(const Params &p, uint32_t *__restrict_
     const uint32_t units = by_wave ? (blockDim.x >> 6) : 1u, unit = by_wave ? (threadIdx.x >> 6) : 0u;
     const uint32_t first = by_wave ? (threadIdx.x & 63u) : threadIdx.x, width = by_wave ? 64u : blockDim.x;
     for (uint32_t shard = block * units + unit; shard < nshards; shard += nblocks * units) {
+        const uint4 *rec = rl.rec + (size_t)shard * rl.cap * list_units(W);
+        // The thread's first record is read before the shard's count
+        // arrives (every slot below the capacity is allocated; a slot at or
+        // past the count is not used), and each next one while the current
+        // one is evaluated.  (Loading two or four entries before evaluating
+        // any measured slower: 32.6-33.3 against 30.8 us at the whole-cell
+        // size.)
+        uint4 a = make_uint4(0u, 0u, 0u, 0u), b = a;
+        if (first < rl.cap) {
+            a = rec[(size_t)first * list_units(W)];
+            if constexpr (W == 4) b = rec[(size_t)first * 2 + 1];
+        }
         const uint32_t n = min(p.ctl->list_count[shard], rl.cap);
         for (uint32_t k = first; k < n; k += width) {
-            // (Loading two or four entries before evaluating any measured
-            // slower: 32.6-33.3 against 30.8 us at the whole-cell size.)
+            uint4 na = make_uint4(0u, 0u, 0u, 0u), nb = na;
+            if (k + width < n) {
+                na = rec[(size_t)(k + width) * list_units(W)];
+                if constexpr (W == 4) nb = rec[(size_t)(k + width) * 2 + 1];
+            }
             ListEntry<W> E;
-            list_entry_load<W>(p, rl, ((size_t)shard * rl.cap + k) * list_units(W), tab, E);
+            list_entry_load<W>(p, a, b, tab, E);
             list_entry_react<W>(p, lat, E, h3);
+            a = na;
+            b = nb;
         }
     }
 }
