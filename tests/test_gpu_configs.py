@@ -81,14 +81,20 @@ def test_ensemble_256_replicates_match_plain_runs(gpu):
         assert _digest(orc.native()) == batched[k], f"replicate {r} {shape}: batched != oracle"
 
 
-def test_served_template_mix_matches_solo_and_oracle(gpu):
+@pytest.mark.parametrize("policy", ["fcfs", "sjf"])
+def test_served_template_mix_matches_solo_and_oracle(gpu, policy):
+    """The served mix of the three templates: every job equals its solo run
+    and one job per template its oracle run, in arrival order and with the
+    size-aware admission (smallest work first under a site limit) that the
+    benchmark serves with."""
     templates = models.serving_templates(seed=1)
     jobs = serving.poisson_jobs(templates, 9, 20.0, (4, 12), seed=5)
     # every template is requested
     assert {id(j.spec) for j in jobs} == {id(t) for t in templates}
     # a budget that holds two of the largest jobs at once: some requests wait
     budget = 2 * serving.footprint(templates[2]) + serving.footprint(templates[1])
-    serving.Server(0, hbm_budget=budget, chunk=3).run(jobs)
+    limit = None if policy == "fcfs" else templates[2].N + templates[1].N
+    serving.Server(0, hbm_budget=budget, chunk=3, max_sites=limit, policy=policy).run(jobs)
     for j in jobs:
         assert j.digest == serving.run_alone(j), f"job {j.id} ({j.spec.name}): served != solo"
     seen = set()
