@@ -117,6 +117,22 @@ struct Member {
     uint32_t pad;
 };
 
+// Work units (x4 blocks, sweep tiles) per batched block: a block loops over
+// its member's units with the member's block count as stride, so that the
+// member lookup and descriptor reads at a block's start are paid once per
+// kBatchUnits units of a smaller member.  A member of at least
+// kBatchOwnUnits units fills the device alone and keeps one unit per block
+// (its blocks then overlap each other's latencies).  A member's block count
+// is a multiple of 8, so every unit of a block keeps the XCD of the
+// single-solver block order.
+constexpr uint32_t kBatchUnits = 4, kBatchOwnUnits = 8192;
+__host__ __device__ inline uint32_t batch_blocks(uint32_t units)
+{
+    const uint32_t k = units >= kBatchOwnUnits ? 1u : kBatchUnits;
+    const uint32_t b = (units + k - 1) / k;
+    return (b + 7) / 8 * 8;
+}
+
 // Member of global block b: start[] holds, for n members, the first block of
 // each member and start[n] the total (members with no blocks repeat a start).
 __device__ inline uint32_t find_member(const uint32_t *__restrict__ start, uint32_t n, uint32_t b)

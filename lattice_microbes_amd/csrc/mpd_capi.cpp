@@ -1423,9 +1423,9 @@ void fill_member(const lmx_solver *s, bool use_list, uint32_t threads, lmx::Memb
     m.pad = 0;
     m.geo[0] = lmx::tile_geom(m.p, 1, threads);
     m.geo[1] = lmx::tile_geom(m.p, 2, threads);
-    blocks[0] = lmx::x4_blocks(m.p);
-    blocks[1] = lmx::tile_blocks(m.geo[0]);
-    blocks[2] = lmx::tile_blocks(m.geo[1]);
+    blocks[0] = lmx::batch_blocks(lmx::x4_blocks(m.p));
+    blocks[1] = lmx::batch_blocks(lmx::tile_blocks(m.geo[0]));
+    blocks[2] = lmx::batch_blocks(lmx::tile_blocks(m.geo[1]));
     m.rl = lmx::ReactionList{};
     m.nshards = 0;
     if (s->R == 0) {

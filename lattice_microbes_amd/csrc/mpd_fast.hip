@@ -976,7 +976,7 @@ __device__ inline void tile_body(const Params &p, const uint32_t *__restrict__ s
         // With X % 256 == 0 a wave is one aligned chunk of a row.
         store_group_sweep<W>(p, dst, i0 >> 2, res, Gx == kChunkGroups && (p.X & 255u) == 0u);
     }
-    if (REACT) {
+    if (REACT && rl.rec) {
         // The output sites that hold particles go to the reaction list with
         // their words and types (without zero-order reactions an empty site
         // cannot react).  Output sites are packed (particles fill word 0
@@ -2137,27 +2137,28 @@ __global__ __launch_bounds__(256) void batch_x4_kernel(const Member *__restrict_
     const uint32_t r = find_member(start, n, blockIdx.x);
     const Member &M = m[r];
     const uint32_t c = M.cur;
-    x4_body<W>(M.p, M.lat[c], M.lat[c ^ 1u], blockIdx.x - start[r]);
+    const uint32_t nb = start[r + 1] - start[r], units = ((M.p.N >> 2) + 255u) / 256u;  // x4_blocks
+    for (uint32_t u = blockIdx.x - start[r]; u < units; u += nb) x4_body<W>(M.p, M.lat[c], M.lat[c ^ 1u], u);
 }
 
-// y (axis 1) or z (axis 2) sweeps; a z sweep of a member whose reaction
-// phase runs over its list fills that list.
-template <int W, int T>
-__global__ __launch_bounds__(T) void batch_tile_kernel(const Member *__restrict__ m, const uint32_t *__restrict__ start,
-                                                       uint32_t n, uint32_t axis)
+// y (AXIS 1) or z (AXIS 2) sweeps, each block looping over units of its
+// member (batch_blocks); a z sweep of a member whose reaction phase runs over
+// its list fills that list (the others pass no list).
+template <int W, int T, int AXIS>
+__global__ __launch_bounds__(T, 8) void batch_tile_kernel(const Member *__restrict__ m,
+                                                          const uint32_t *__restrict__ start, uint32_t n)
 {
     const uint32_t r = find_member(start, n, blockIdx.x);
     const Member &M = m[r];
     const uint32_t c = M.cur;
-    const uint32_t *src = axis == 1 ? M.lat[c ^ 1u] : M.lat[c];
-    uint32_t *dst = axis == 1 ? M.lat[c] : M.lat[c ^ 1u];
-    const uint32_t block = blockIdx.x - start[r];
-    __shared__ TileLds<W, T> lds;  // one LDS block for both instantiations
-    if (axis == 2 && M.react == kReactList) {
-        tile_body<W, true, T>(M.p, src, dst, M.geo[1], axis, block, M.rl, lds);
-    } else {
-        const ReactionList none{};
-        tile_body<W, false, T>(M.p, src, dst, M.geo[axis - 1], axis, block, none, lds);
+    const uint32_t *src = AXIS == 1 ? M.lat[c ^ 1u] : M.lat[c];
+    uint32_t *dst = AXIS == 1 ? M.lat[c] : M.lat[c ^ 1u];
+    const uint32_t nb = start[r + 1] - start[r], units = tile_blocks(M.geo[AXIS - 1]);
+    const ReactionList rl = AXIS == 2 && M.react == kReactList ? M.rl : ReactionList{};
+    __shared__ TileLds<W, T> lds;
+    for (uint32_t u = blockIdx.x - start[r]; u < units; u += nb) {
+        tile_body<W, AXIS == 2, T>(M.p, src, dst, M.geo[AXIS - 1], AXIS, u, rl, lds);
+        if (u + nb < units) __syncthreads();  // the next unit rewrites the LDS rows
     }
 }
 
@@ -2204,15 +2205,15 @@ hipError_t launch_batch_sweeps(const Member *m, const uint32_t *start_x, const u
     case WW:                                                                                                      \
         hipLaunchKernelGGL(batch_x4_kernel<WW>, dim3(total[0]), dim3(256), 0, st, m, start_x, n);                  \
         if (narrow) {                                                                                             \
-            hipLaunchKernelGGL((batch_tile_kernel<WW, kNarrowTileThreads>), dim3(total[1]), tb, 0, st, m, start_y, \
-                               n, 1u);                                                                            \
-            hipLaunchKernelGGL((batch_tile_kernel<WW, kNarrowTileThreads>), dim3(total[2]), tb, 0, st, m, start_z, \
-                               n, 2u);                                                                            \
+            hipLaunchKernelGGL((batch_tile_kernel<WW, kNarrowTileThreads, 1>), dim3(total[1]), tb, 0, st, m,      \
+                               start_y, n);                                                                       \
+            hipLaunchKernelGGL((batch_tile_kernel<WW, kNarrowTileThreads, 2>), dim3(total[2]), tb, 0, st, m,      \
+                               start_z, n);                                                                       \
         } else {                                                                                                  \
-            hipLaunchKernelGGL((batch_tile_kernel<WW, tile_threads<WW>()>), dim3(total[1]), tb, 0, st, m, start_y, \
-                               n, 1u);                                                                            \
-            hipLaunchKernelGGL((batch_tile_kernel<WW, tile_threads<WW>()>), dim3(total[2]), tb, 0, st, m, start_z, \
-                               n, 2u);                                                                            \
+            hipLaunchKernelGGL((batch_tile_kernel<WW, tile_threads<WW>(), 1>), dim3(total[1]), tb, 0, st, m,      \
+                               start_y, n);                                                                       \
+            hipLaunchKernelGGL((batch_tile_kernel<WW, tile_threads<WW>(), 2>), dim3(total[2]), tb, 0, st, m,      \
+                               start_z, n);                                                                       \
         }                                                                                                         \
         if (total[3]) hipLaunchKernelGGL(batch_reaction_kernel<WW>, dim3(total[3]), dim3(256), 0, st, m, start_r, n); \
         break;

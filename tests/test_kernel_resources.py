@@ -15,16 +15,22 @@ pytestmark = pytest.mark.skipif(not os.path.exists(build.RESOURCES),
                                 reason="no kernel resource report (build the library first)")
 
 # Kernels of the default step and of batches, for 4 and 8 slots per site
-# (W = 1, 2): (mangled-name fragment, least occupancy in waves per SIMD).
+# (W = 1, 2): (mangled-name fragment, least occupancy in waves per SIMD,
+# most scratch bytes per lane).  The batched tile sweeps, bounded to eight
+# waves per SIMD, keep one 8-byte spill: their loop over several tiles per
+# block needs a few registers more than the single-solver body (measured:
+# the batched ensemble 1.59 -> 1.47 ms per step against 1.50 ms unbounded at
+# seven waves per SIMD without the spill).
 DEFAULT_KERNELS = [
-    ("sweep_x4_kernelILi1E", 8), ("sweep_x4_kernelILi2E", 8),
-    ("sweep_tile_kernelILi1ELb0E", 8), ("sweep_tile_kernelILi1ELb1E", 8),
-    ("sweep_tile_kernelILi2ELb0E", 8), ("sweep_tile_kernelILi2ELb1E", 8),
-    ("reaction_list_kernelILi1E", 4), ("reaction_list_kernelILi2E", 4),
-    ("batch_x4_kernelILi1E", 8), ("batch_x4_kernelILi2E", 8),
-    ("batch_tile_kernelILi1ELi256E", 8), ("batch_tile_kernelILi2ELi256E", 8),
-    ("batch_reaction_kernelILi1E", 4), ("batch_reaction_kernelILi2E", 4),
-    ("3lmx14resolve_kernel", 1), ("batch_resolve_kernel", 1),
+    ("sweep_x4_kernelILi1E", 8, 0), ("sweep_x4_kernelILi2E", 8, 0),
+    ("sweep_tile_kernelILi1ELb0E", 8, 0), ("sweep_tile_kernelILi1ELb1E", 8, 0),
+    ("sweep_tile_kernelILi2ELb0E", 8, 0), ("sweep_tile_kernelILi2ELb1E", 8, 0),
+    ("reaction_list_kernelILi1E", 4, 0), ("reaction_list_kernelILi2E", 4, 0),
+    ("batch_x4_kernelILi1E", 8, 0), ("batch_x4_kernelILi2E", 8, 0),
+    ("batch_tile_kernelILi1ELi256ELi1E", 8, 0), ("batch_tile_kernelILi1ELi256ELi2E", 8, 0),
+    ("batch_tile_kernelILi2ELi256ELi1E", 8, 8), ("batch_tile_kernelILi2ELi256ELi2E", 8, 8),
+    ("batch_reaction_kernelILi1E", 4, 0), ("batch_reaction_kernelILi2E", 4, 0),
+    ("3lmx14resolve_kernel", 1, 0), ("batch_resolve_kernel", 1, 0),
 ]
 
 
@@ -34,10 +40,10 @@ def _find(res, frag):
     return res[names[0]]
 
 
-@pytest.mark.parametrize("frag,occupancy", DEFAULT_KERNELS, ids=[k[0] for k in DEFAULT_KERNELS])
-def test_default_kernels_do_not_spill(frag, occupancy):
+@pytest.mark.parametrize("frag,occupancy,scratch", DEFAULT_KERNELS, ids=[k[0] for k in DEFAULT_KERNELS])
+def test_default_kernels_do_not_spill(frag, occupancy, scratch):
     r = _find(build.kernel_resources(), frag)
-    assert r["ScratchSize"] == 0, f"{frag} spills {r['ScratchSize']} bytes per lane ({r['VGPRs']} VGPRs)"
+    assert r["ScratchSize"] <= scratch, f"{frag} spills {r['ScratchSize']} bytes per lane ({r['VGPRs']} VGPRs)"
     assert r["Occupancy"] >= occupancy, f"{frag}: occupancy {r['Occupancy']} < {occupancy}"
 
 
