@@ -114,7 +114,7 @@ struct Member {
     uint32_t cur;        // lat[cur] holds the state at the start of a step; the batched resolver flips it
     uint32_t react;      // kReactNone, kReactFull or kReactList
     uint32_t nshards;    // list shards its z sweep fills (kReactList)
-    uint32_t pad;
+    uint32_t list_by_block;  // 1: a block per list shard (large members); 0: a wave per several shards
 };
 
 // Work units (x4 blocks, sweep tiles) per batched block: a block loops over

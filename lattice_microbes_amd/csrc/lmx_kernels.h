@@ -42,7 +42,9 @@ uint32_t x4_blocks(const Params &p);
 uint32_t reaction_blocks(const Params &p);
 struct TileGeom;
 uint32_t batch_list_shards(const TileGeom &zgeo, uint32_t threads);
-uint32_t batch_list_blocks(uint32_t nshards);
+constexpr uint32_t kBatchListBlockSites = 1u << 22;
+bool batch_list_by_block(uint32_t sites);
+uint32_t batch_list_blocks(uint32_t nshards, uint32_t sites);
 // One batched step without the resolver: x, y, z sweeps and (if total[3] > 0)
 // the reaction phase; start_* are device prefix arrays of n+1 block offsets,
 // total[] the four grid sizes.

@@ -1420,7 +1420,7 @@ void fill_member(const lmx_solver *s, bool use_list, uint32_t threads, lmx::Memb
     m.lat[0] = s->d_lat[0];
     m.lat[1] = s->d_lat[1];
     m.cur = (uint32_t)s->cur;
-    m.pad = 0;
+    m.list_by_block = lmx::batch_list_by_block((uint32_t)s->N) ? 1u : 0u;
     m.geo[0] = lmx::tile_geom(m.p, 1, threads);
     m.geo[1] = lmx::tile_geom(m.p, 2, threads);
     blocks[0] = lmx::batch_blocks(lmx::x4_blocks(m.p));
@@ -1435,7 +1435,7 @@ void fill_member(const lmx_solver *s, bool use_list, uint32_t threads, lmx::Memb
         m.react = lmx::kReactList;
         m.rl = s->rlist;
         m.nshards = lmx::batch_list_shards(m.geo[1], threads);
-        blocks[3] = lmx::batch_list_blocks(m.nshards);
+        blocks[3] = lmx::batch_list_blocks(m.nshards, (uint32_t)s->N);
     } else {
         m.react = lmx::kReactFull;
         blocks[3] = lmx::reaction_blocks(m.p);

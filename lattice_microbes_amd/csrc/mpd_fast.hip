@@ -2173,7 +2173,7 @@ __global__ __launch_bounds__(256, 4) void batch_reaction_kernel(const Member *__
     const uint32_t block = blockIdx.x - start[r], nb = start[r + 1] - start[r];
     uint32_t *lat = M.lat[M.cur ^ 1u];
     if (M.react == kReactList && !M.p.ctl->list_overflow) {
-        reaction_list_body<W>(M.p, lat, M.rl, block, nb, M.nshards, true);
+        reaction_list_body<W>(M.p, lat, M.rl, block, nb, M.nshards, !M.list_by_block);
         return;
     }
     // the full pass (this member's blocks stride over its reaction blocks)
@@ -2185,12 +2185,19 @@ uint32_t x4_blocks(const Params &p) { return blocks_for(p.N >> 2, 256); }
 uint32_t reaction_blocks(const Params &p) { return p.R > 0 ? blocks_for(p.N >> 2, 256 * kRG) : 0u; }
 
 // List shards a batched z sweep of this geometry fills (blocks of `threads`),
-// and the list blocks (four waves of about two shards each) that serve them.
+// and the list blocks that serve them: a block per shard for a member of at
+// least kBatchListBlockSites sites (its shards hold many entries), else four
+// waves of about eight shards each (small lattices fill many shards with a
+// few entries each, and fewer blocks pay fewer member lookups).
 uint32_t batch_list_shards(const TileGeom &zgeo, uint32_t threads)
 {
     return list_shards_used(tile_blocks(zgeo), threads);
 }
-uint32_t batch_list_blocks(uint32_t nshards) { return (nshards + 7u) / 8u; }
+bool batch_list_by_block(uint32_t sites) { return sites >= kBatchListBlockSites; }
+uint32_t batch_list_blocks(uint32_t nshards, uint32_t sites)
+{
+    return batch_list_by_block(sites) ? nshards : (nshards + 31u) / 32u;
+}
 
 hipError_t launch_batch_sweeps(const Member *m, const uint32_t *start_x, const uint32_t *start_y,
                                const uint32_t *start_z, const uint32_t *start_r, uint32_t n, const uint32_t *total,
