@@ -108,10 +108,12 @@ constexpr uint32_t kReactNone = 0, kReactFull = 1, kReactList = 2;
 
 struct Member {
     Params p;
+    // lat[0] holds the state at the start of a step, lat[1] the other
+    // buffer; the batched resolver swaps them (reading a parity first would
+    // put one more dependent load before every block's first lattice load)
     uint32_t *lat[2];
     TileGeom geo[2];     // y and z tiled sweeps
     ReactionList rl;     // the member's own reaction list (kReactList)
-    uint32_t cur;        // lat[cur] holds the state at the start of a step; the batched resolver flips it
     uint32_t react;      // kReactNone, kReactFull or kReactList
     uint32_t nshards;    // list shards its z sweep fills (kReactList)
     uint32_t list_by_block;  // 1: a block per list shard (large members); 0: a wave per several shards
@@ -133,17 +135,13 @@ __host__ __device__ inline uint32_t batch_blocks(uint32_t units)
     return (b + 7) / 8 * 8;
 }
 
-// Member of global block b: start[] holds, for n members, the first block of
-// each member and start[n] the total (members with no blocks repeat a start).
-__device__ inline uint32_t find_member(const uint32_t *__restrict__ start, uint32_t n, uint32_t b)
+// Member of block b of a batched launch: map[b], written by the host with the
+// block offsets (one scalar load; a binary search over the offsets was a
+// chain of eight dependent loads at the start of every block of a batch of
+// 256 members).  A batch of one member reads no map.
+__device__ inline uint32_t member_of(const uint32_t *__restrict__ map, uint32_t n, uint32_t b)
 {
-    uint32_t lo = 0, hi = n;  // start[lo] <= b < start[hi]
-    while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (start[mid] <= b) lo = mid;
-        else hi = mid;
-    }
-    return lo;
+    return n == 1u ? 0u : map[b];
 }
 
 }  // namespace lmx

@@ -46,13 +46,14 @@ constexpr uint32_t kBatchListBlockSites = 1u << 22;
 bool batch_list_by_block(uint32_t sites);
 uint32_t batch_list_blocks(uint32_t nshards, uint32_t sites);
 // One batched step without the resolver: x, y, z sweeps and (if total[3] > 0)
-// the reaction phase; start_* are device prefix arrays of n+1 block offsets,
-// total[] the four grid sizes.
+// the reaction phase.  For launch k (x, y, z, reaction), start[k] is a device
+// prefix array of n+1 block offsets and map[k] a device array of total[k]
+// member indices (block b belongs to member map[k][b]); total[] holds the
+// four grid sizes.
 // Tiled sweeps in blocks of tile_threads (a multiple of 64, at most
 // tile_threads_for(W)), the size the members' tile geometry was made for.
-hipError_t launch_batch_sweeps(const Member *m, const uint32_t *start_x, const uint32_t *start_y,
-                               const uint32_t *start_z, const uint32_t *start_r, uint32_t n, const uint32_t *total,
-                               uint32_t W, uint32_t tile_threads, hipStream_t st);
+hipError_t launch_batch_sweeps(const Member *m, const uint32_t *const start[4], const uint32_t *const map[4],
+                               uint32_t n, const uint32_t *total, uint32_t W, uint32_t tile_threads, hipStream_t st);
 hipError_t launch_batch_resolve(Member *m, uint32_t n, hipStream_t st);
 hipError_t launch_copy(const void *src, void *dst, size_t bytes, int num_cu, hipStream_t st);
 hipError_t launch_fill(uint32_t *ptr, size_t n, uint32_t value, hipStream_t st);

@@ -1364,6 +1364,9 @@ struct lmx_batch {
     lmx::Member *d_members = nullptr, *h_members = nullptr;
     uint32_t *d_start = nullptr, *h_start = nullptr;
     uint32_t total[4] = {0, 0, 0, 0};
+    // member of every block of the four launches, concatenated (map_cap entries)
+    size_t map_cap = 0;
+    uint32_t *d_map = nullptr, *h_map = nullptr;
     // captured step, valid for the member list and totals it was made with
     hipGraphExec_t exec = nullptr;
     std::vector<lmx_solver *> graph_members;
@@ -1417,9 +1420,8 @@ void fill_member(const lmx_solver *s, bool use_list, uint32_t threads, lmx::Memb
 {
     m.p = make_params(s);
     if (s->flag_words > 0) m.p.plane_flags = 1u;  // the batched kernels are the default ones
-    m.lat[0] = s->d_lat[0];
-    m.lat[1] = s->d_lat[1];
-    m.cur = (uint32_t)s->cur;
+    m.lat[0] = s->d_lat[s->cur];
+    m.lat[1] = s->d_lat[1 - s->cur];
     m.list_by_block = lmx::batch_list_by_block((uint32_t)s->N) ? 1u : 0u;
     m.geo[0] = lmx::tile_geom(m.p, 1, threads);
     m.geo[1] = lmx::tile_geom(m.p, 2, threads);
@@ -1494,13 +1496,34 @@ int batch_prepare(lmx_batch *b)
             acc[k] += blocks[k];
         }
     }
+    size_t blocks = 0;
     for (int k = 0; k < 4; k++) {
         b->h_start[k * (b->cap + 1) + n] = acc[k];
         b->total[k] = acc[k];
+        blocks += acc[k];
+    }
+    if (blocks > b->map_cap) {
+        size_t cap = b->map_cap ? b->map_cap : 4096;
+        while (cap < blocks) cap *= 2;
+        dev_free(b->d_map);
+        if (b->h_map) hipHostFree(b->h_map);
+        b->h_map = nullptr;
+        b->map_cap = 0;
+        LMX_HIP(hipMalloc((void **)&b->d_map, sizeof(uint32_t) * cap));
+        LMX_HIP(hipHostMalloc((void **)&b->h_map, sizeof(uint32_t) * cap, hipHostMallocDefault));
+        b->map_cap = cap;
+        batch_drop_graph(b);
+    }
+    uint32_t *map = b->h_map;
+    for (int k = 0; k < 4; k++) {
+        const uint32_t *st = b->h_start + k * (b->cap + 1);
+        for (uint32_t r = 0; r < n; r++) std::fill(map + st[r], map + st[r + 1], r);
+        map += acc[k];
     }
     LMX_HIP(hipMemcpyAsync(b->d_members, b->h_members, sizeof(lmx::Member) * n, hipMemcpyHostToDevice, b->stream));
     LMX_HIP(hipMemcpyAsync(b->d_start, b->h_start, sizeof(uint32_t) * 4 * (b->cap + 1), hipMemcpyHostToDevice,
                            b->stream));
+    if (blocks) LMX_HIP(hipMemcpyAsync(b->d_map, b->h_map, sizeof(uint32_t) * blocks, hipMemcpyHostToDevice, b->stream));
     LMX_HIP(hipEventRecord(b->staged, b->stream));
     b->staged_pending = true;
     return LMX_OK;
@@ -1510,8 +1533,14 @@ int batch_enqueue_step(lmx_batch *b)
 {
     const uint32_t n = (uint32_t)b->members.size();
     const uint32_t stride = b->cap + 1;
-    hipError_t e = lmx::launch_batch_sweeps(b->d_members, b->d_start, b->d_start + stride, b->d_start + 2 * stride,
-                                            b->d_start + 3 * stride, n, b->total, b->W, batch_tile_threads(b),
+    const uint32_t *start[4], *map[4];
+    size_t off = 0;
+    for (int k = 0; k < 4; k++) {
+        start[k] = b->d_start + k * stride;
+        map[k] = b->d_map + off;
+        off += b->total[k];
+    }
+    hipError_t e = lmx::launch_batch_sweeps(b->d_members, start, map, n, b->total, b->W, batch_tile_threads(b),
                                             b->stream);
     if (e != hipSuccess) return hip_fail(e, "batched sweep launch");
     e = lmx::launch_batch_resolve(b->d_members, n, b->stream);
@@ -1579,8 +1608,10 @@ int lmx_batch_destroy(lmx_batch *b)
     batch_drop_graph(b);
     dev_free(b->d_members);
     dev_free(b->d_start);
+    dev_free(b->d_map);
     if (b->h_members) hipHostFree(b->h_members);
     if (b->h_start) hipHostFree(b->h_start);
+    if (b->h_map) hipHostFree(b->h_map);
     if (b->done) hipEventDestroy(b->done);
     if (b->staged) hipEventDestroy(b->staged);
     if (b->stream) hipStreamDestroy(b->stream);

@@ -2125,20 +2125,20 @@ hipError_t launch_prob_tables(const Params &p, float *prob1, float *prob0, hipSt
 }
 
 // ---------------------------------------------------------------------------
-// Batched launches: block b of a launch belongs to member find_member(start, b)
-// and runs that member's share with the single-solver body.  Buffer roles
-// follow the member's parity: x reads lat[cur], y writes it back, z and the
-// reaction phase leave the step in lat[cur ^ 1].
+// Batched launches: block b of a launch belongs to member map[b] (its blocks
+// are start[r] .. start[r+1]-1) and runs that member's share with the
+// single-solver body.  x reads lat[0], y writes it back, z and the reaction
+// phase leave the step in lat[1].
 // ---------------------------------------------------------------------------
 template <int W>
 __global__ __launch_bounds__(256) void batch_x4_kernel(const Member *__restrict__ m,
-                                                       const uint32_t *__restrict__ start, uint32_t n)
+                                                       const uint32_t *__restrict__ start,
+                                                       const uint32_t *__restrict__ map, uint32_t n)
 {
-    const uint32_t r = find_member(start, n, blockIdx.x);
+    const uint32_t r = member_of(map, n, blockIdx.x);
     const Member &M = m[r];
-    const uint32_t c = M.cur;
     const uint32_t nb = start[r + 1] - start[r], units = ((M.p.N >> 2) + 255u) / 256u;  // x4_blocks
-    for (uint32_t u = blockIdx.x - start[r]; u < units; u += nb) x4_body<W>(M.p, M.lat[c], M.lat[c ^ 1u], u);
+    for (uint32_t u = blockIdx.x - start[r]; u < units; u += nb) x4_body<W>(M.p, M.lat[0], M.lat[1], u);
 }
 
 // y (AXIS 1) or z (AXIS 2) sweeps, each block looping over units of its
@@ -2146,13 +2146,13 @@ __global__ __launch_bounds__(256) void batch_x4_kernel(const Member *__restrict_
 // its list fills that list (the others pass no list).
 template <int W, int T, int AXIS>
 __global__ __launch_bounds__(T, 8) void batch_tile_kernel(const Member *__restrict__ m,
-                                                          const uint32_t *__restrict__ start, uint32_t n)
+                                                          const uint32_t *__restrict__ start,
+                                                          const uint32_t *__restrict__ map, uint32_t n)
 {
-    const uint32_t r = find_member(start, n, blockIdx.x);
+    const uint32_t r = member_of(map, n, blockIdx.x);
     const Member &M = m[r];
-    const uint32_t c = M.cur;
-    const uint32_t *src = AXIS == 1 ? M.lat[c ^ 1u] : M.lat[c];
-    uint32_t *dst = AXIS == 1 ? M.lat[c] : M.lat[c ^ 1u];
+    const uint32_t *src = M.lat[AXIS == 1 ? 1 : 0];
+    uint32_t *dst = M.lat[AXIS == 1 ? 0 : 1];
     const uint32_t nb = start[r + 1] - start[r], units = tile_blocks(M.geo[AXIS - 1]);
     const ReactionList rl = AXIS == 2 && M.react == kReactList ? M.rl : ReactionList{};
     __shared__ TileLds<W, T> lds;
@@ -2166,12 +2166,13 @@ __global__ __launch_bounds__(T, 8) void batch_tile_kernel(const Member *__restri
 // walks the shards its z sweep filled).
 template <int W>
 __global__ __launch_bounds__(256, 4) void batch_reaction_kernel(const Member *__restrict__ m,
-                                                             const uint32_t *__restrict__ start, uint32_t n)
+                                                             const uint32_t *__restrict__ start,
+                                                             const uint32_t *__restrict__ map, uint32_t n)
 {
-    const uint32_t r = find_member(start, n, blockIdx.x);
+    const uint32_t r = member_of(map, n, blockIdx.x);
     const Member &M = m[r];
     const uint32_t block = blockIdx.x - start[r], nb = start[r + 1] - start[r];
-    uint32_t *lat = M.lat[M.cur ^ 1u];
+    uint32_t *lat = M.lat[1];
     if (M.react == kReactList && !M.p.ctl->list_overflow) {
         reaction_list_body<W>(M.p, lat, M.rl, block, nb, M.nshards, !M.list_by_block);
         return;
@@ -2199,9 +2200,9 @@ uint32_t batch_list_blocks(uint32_t nshards, uint32_t sites)
     return batch_list_by_block(sites) ? nshards : (nshards + 31u) / 32u;
 }
 
-hipError_t launch_batch_sweeps(const Member *m, const uint32_t *start_x, const uint32_t *start_y,
-                               const uint32_t *start_z, const uint32_t *start_r, uint32_t n, const uint32_t *total,
-                               uint32_t W, uint32_t tile_threads_used, hipStream_t st)
+hipError_t launch_batch_sweeps(const Member *m, const uint32_t *const start[4], const uint32_t *const map[4],
+                               uint32_t n, const uint32_t *total, uint32_t W, uint32_t tile_threads_used,
+                               hipStream_t st)
 {
     if (tile_threads_used == 0 || tile_threads_used % 64u || tile_threads_used > tile_threads_for(W))
         return hipErrorInvalidValue;
@@ -2210,19 +2211,20 @@ hipError_t launch_batch_sweeps(const Member *m, const uint32_t *start_x, const u
     switch (W) {
 #define LMX_BATCH_CASE(WW)                                                                                        \
     case WW:                                                                                                      \
-        hipLaunchKernelGGL(batch_x4_kernel<WW>, dim3(total[0]), dim3(256), 0, st, m, start_x, n);                  \
+        hipLaunchKernelGGL(batch_x4_kernel<WW>, dim3(total[0]), dim3(256), 0, st, m, start[0], map[0], n);        \
         if (narrow) {                                                                                             \
             hipLaunchKernelGGL((batch_tile_kernel<WW, kNarrowTileThreads, 1>), dim3(total[1]), tb, 0, st, m,      \
-                               start_y, n);                                                                       \
+                               start[1], map[1], n);                                                              \
             hipLaunchKernelGGL((batch_tile_kernel<WW, kNarrowTileThreads, 2>), dim3(total[2]), tb, 0, st, m,      \
-                               start_z, n);                                                                       \
+                               start[2], map[2], n);                                                              \
         } else {                                                                                                  \
             hipLaunchKernelGGL((batch_tile_kernel<WW, tile_threads<WW>(), 1>), dim3(total[1]), tb, 0, st, m,      \
-                               start_y, n);                                                                       \
+                               start[1], map[1], n);                                                              \
             hipLaunchKernelGGL((batch_tile_kernel<WW, tile_threads<WW>(), 2>), dim3(total[2]), tb, 0, st, m,      \
-                               start_z, n);                                                                       \
+                               start[2], map[2], n);                                                              \
         }                                                                                                         \
-        if (total[3]) hipLaunchKernelGGL(batch_reaction_kernel<WW>, dim3(total[3]), dim3(256), 0, st, m, start_r, n); \
+        if (total[3])                                                                                             \
+            hipLaunchKernelGGL(batch_reaction_kernel<WW>, dim3(total[3]), dim3(256), 0, st, m, start[3], map[3], n); \
         break;
         LMX_BATCH_CASE(1)
         LMX_BATCH_CASE(2)

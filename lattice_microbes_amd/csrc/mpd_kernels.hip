@@ -306,13 +306,17 @@ __global__ __launch_bounds__(kResolveThreads) void resolve_kernel(Params p, uint
 }
 
 // Batched resolver: block r resolves member r's step on its post-step buffer
-// and flips the member's buffer parity for the next step.
+// (lat[1]) and swaps the member's two buffers for the next step.
 __global__ __launch_bounds__(kResolveThreads) void batch_resolve_kernel(Member *__restrict__ m)
 {
     Member &M = m[blockIdx.x];
-    const uint32_t c = M.cur;
-    resolve_body(M.p, M.lat[c ^ 1u]);
-    if (threadIdx.x == 0) M.cur = c ^ 1u;
+    uint32_t *const before = M.lat[0], *const after = M.lat[1];
+    resolve_body(M.p, after);
+    __syncthreads();  // every wave has read both pointers (resolve_body returns block-uniformly)
+    if (threadIdx.x == 0) {
+        M.lat[0] = after;
+        M.lat[1] = before;
+    }
 }
 
 // ---------------------------------------------------------------------------
