@@ -359,10 +359,12 @@ __device__ inline uint32_t site_bits(unsigned long long m, int s)
 // x sweep
 // ---------------------------------------------------------------------------
 // Body of the x sweep for 256-thread block number `block` (the kernel below,
-// or one replicate's share of a batched launch).
+// or one replicate's share of a batched launch); h is the step's x-sweep hash
+// (timestep_hash substep 0), which a batched block computes once for all the
+// units it loops over.
 template <int W>
 __device__ inline void x4_body(const Params &p, const uint32_t *__restrict__ src, uint32_t *__restrict__ dst,
-                               uint32_t block)
+                               unsigned long long h, uint32_t block)
 {
     constexpr uint32_t P = 4 * W;
     const uint32_t ngroups = p.N >> 2;
@@ -428,7 +430,6 @@ __device__ inline void x4_body(const Params &p, const uint32_t *__restrict__ src
         for (int k = 0; k < W; k++) rw[k] = 0u;
     }
 
-    const unsigned long long h = timestep_hash(p.seed32, p.ctl->timestep, 0);
     const unsigned long long occ = active ? group_occ_in<W>(p, G) : 0ull;
     unsigned long long minus = 0, plus = 0;
     if (occ) {
@@ -509,7 +510,7 @@ template <int W>
 __global__ __launch_bounds__(256) void sweep_x4_kernel(Params p, const uint32_t *__restrict__ src,
                                                        uint32_t *__restrict__ dst)
 {
-    x4_body<W>(p, src, dst, blockIdx.x);
+    x4_body<W>(p, src, dst, timestep_hash(p.seed32, p.ctl->timestep, 0), blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -831,10 +832,14 @@ struct TileLds {
     unsigned long long m[kEntries], p[kEntries];
 };
 
+// Body of a y or z tile sweep (substep 1 or 2) for block number `block`; h
+// is timestep_hash(seed32, timestep, substep).  A batched block computes h
+// once before its unit loop: computed inside the loop, its constants stay in
+// registers across the loop and cost the tile kernels a spill.
 template <int W, bool REACT, int T>
 __device__ inline void tile_body(const Params &p, const uint32_t *__restrict__ src, uint32_t *__restrict__ dst,
-                                 const TileGeom &geo, uint32_t substep, uint32_t block, const ReactionList &rl,
-                                 TileLds<W, T> &lds)
+                                 const TileGeom &geo, uint32_t substep, unsigned long long h, uint32_t block,
+                                 const ReactionList &rl, TileLds<W, T> &lds)
 {
     const uint32_t stride = geo.stride, len = geo.len, ostride = geo.ostride, nouter = geo.nouter, Gx = geo.Gx,
                    Ty = geo.Ty, nxchunk = geo.nxchunk, ntiles = geo.ntiles, tile_first = geo.tile_first,
@@ -879,7 +884,6 @@ __device__ inline void tile_body(const Params &p, const uint32_t *__restrict__ s
     const uint32_t sb = sub * (Ty + 2u) * Gx;  // the tile's first LDS slot (its top apron row)
     const uint32_t base = o * ostride + gx * 4u;
     const bool periodic = p.boundary == 0;
-    const unsigned long long h = timestep_hash(p.seed32, p.ctl->timestep, substep);
 
     Group4<W> G;
     zero_group<W>(G);
@@ -1024,7 +1028,8 @@ __global__ __launch_bounds__(kNarrowTileThreads) void sweep_tile_kernel(Params p
                                                                         uint32_t substep, ReactionList rl)
 {
     __shared__ TileLds<W, kNarrowTileThreads> lds;
-    tile_body<W, REACT, kNarrowTileThreads>(p, src, dst, geo, substep, blockIdx.x, rl, lds);
+    const unsigned long long h = timestep_hash(p.seed32, p.ctl->timestep, substep);
+    tile_body<W, REACT, kNarrowTileThreads>(p, src, dst, geo, substep, h, blockIdx.x, rl, lds);
 }
 
 // ---------------------------------------------------------------------------
@@ -2138,7 +2143,8 @@ __global__ __launch_bounds__(256) void batch_x4_kernel(const Member *__restrict_
     const uint32_t r = member_of(map, n, blockIdx.x);
     const Member &M = m[r];
     const uint32_t nb = start[r + 1] - start[r], units = ((M.p.N >> 2) + 255u) / 256u;  // x4_blocks
-    for (uint32_t u = blockIdx.x - start[r]; u < units; u += nb) x4_body<W>(M.p, M.lat[0], M.lat[1], u);
+    const unsigned long long h = timestep_hash(M.p.seed32, M.p.ctl->timestep, 0);  // once per block
+    for (uint32_t u = blockIdx.x - start[r]; u < units; u += nb) x4_body<W>(M.p, M.lat[0], M.lat[1], h, u);
 }
 
 // y (AXIS 1) or z (AXIS 2) sweeps, each block looping over units of its
@@ -2155,9 +2161,10 @@ __global__ __launch_bounds__(T, 8) void batch_tile_kernel(const Member *__restri
     uint32_t *dst = M.lat[AXIS == 1 ? 0 : 1];
     const uint32_t nb = start[r + 1] - start[r], units = tile_blocks(M.geo[AXIS - 1]);
     const ReactionList rl = AXIS == 2 && M.react == kReactList ? M.rl : ReactionList{};
+    const unsigned long long h = timestep_hash(M.p.seed32, M.p.ctl->timestep, AXIS);  // once per block
     __shared__ TileLds<W, T> lds;
     for (uint32_t u = blockIdx.x - start[r]; u < units; u += nb) {
-        tile_body<W, AXIS == 2, T>(M.p, src, dst, M.geo[AXIS - 1], AXIS, u, rl, lds);
+        tile_body<W, AXIS == 2, T>(M.p, src, dst, M.geo[AXIS - 1], AXIS, h, u, rl, lds);
         if (u + nb < units) __syncthreads();  // the next unit rewrites the LDS rows
     }
 }

@@ -16,11 +16,10 @@ pytestmark = pytest.mark.skipif(not os.path.exists(build.RESOURCES),
 
 # Kernels of the default step and of batches, for 4 and 8 slots per site
 # (W = 1, 2): (mangled-name fragment, least occupancy in waves per SIMD,
-# most scratch bytes per lane).  The batched tile sweeps, bounded to eight
-# waves per SIMD, keep one 8-byte spill: their loop over several tiles per
-# block needs a few registers more than the single-solver body (measured:
-# the batched ensemble 1.59 -> 1.47 ms per step against 1.50 ms unbounded at
-# seven waves per SIMD without the spill).
+# most scratch bytes per lane).  The batched tile sweeps loop over several
+# tiles per block within a budget of eight waves per SIMD; computing the
+# step hash inside that loop once cost them an 8-byte spill (ensemble step
+# 1.44 against 1.38 ms without it).
 DEFAULT_KERNELS = [
     ("sweep_x4_kernelILi1E", 8, 0), ("sweep_x4_kernelILi2E", 8, 0),
     ("sweep_tile_kernelILi1ELb0E", 8, 0), ("sweep_tile_kernelILi1ELb1E", 8, 0),
@@ -28,7 +27,7 @@ DEFAULT_KERNELS = [
     ("reaction_list_kernelILi1E", 4, 0), ("reaction_list_kernelILi2E", 4, 0),
     ("batch_x4_kernelILi1E", 8, 0), ("batch_x4_kernelILi2E", 8, 0),
     ("batch_tile_kernelILi1ELi256ELi1E", 8, 0), ("batch_tile_kernelILi1ELi256ELi2E", 8, 0),
-    ("batch_tile_kernelILi2ELi256ELi1E", 8, 8), ("batch_tile_kernelILi2ELi256ELi2E", 8, 8),
+    ("batch_tile_kernelILi2ELi256ELi1E", 8, 0), ("batch_tile_kernelILi2ELi256ELi2E", 8, 0),
     ("batch_reaction_kernelILi1E", 4, 0), ("batch_reaction_kernelILi2E", 4, 0),
     ("3lmx14resolve_kernel", 1, 0), ("batch_resolve_kernel", 1, 0),
 ]
