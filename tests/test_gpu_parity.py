@@ -328,6 +328,38 @@ def test_ensemble_members_bit_exact(gpu, graph, reaction_list, tile_threads):
     ens.close()
 
 
+def test_ensemble_grows_past_block_map(gpu):
+    """A member joining after the batch has run raises the launches' block
+    count past the block -> member map the batch first allocated (4096
+    entries; a 256x256x128 lattice alone has 8192 x-sweep blocks): the map
+    is reallocated, the captured step recaptured, and every member still
+    equals its oracle run."""
+    ens = lmx.Ensemble(0)
+    pairs = []
+
+    def join(k, shape, density):
+        spec = models.random_model(shape=shape, P=8, nS=5, nST=3, R=7, density=density, seed=70 + k)
+        s = lmx.MpdRdmeSolver(0)
+        spec.load_into(s, replicate=k, seed=5 + k)
+        ens.add(s)
+        pairs.append((s, oracle.Oracle(spec, seed_top=5 + k, replicate=k)))
+
+    join(0, (32, 12, 20), 0.25)
+    join(1, (64, 16, 16), 0.25)
+    ens.run(2)
+    for s, orc in pairs:
+        assert orc.run(2) == 0
+    join(2, (256, 256, 128), 0.002)
+    join(3, (12, 7, 9), 0.25)
+    for chunk in (2, 1):
+        ens.run(chunk)
+        for s, orc in pairs:
+            assert orc.run(chunk) == 0
+            assert np.array_equal(s.getLatticeNative(), orc.native()), f"{s.shape}"
+            assert s.timestep == orc.timestep
+    ens.close()
+
+
 def test_ensemble_autotune_keeps_state_and_results(gpu):
     """The batch autotuner runs every launch configuration on the members'
     own state and restores it: afterwards each member's lattice, step counter
