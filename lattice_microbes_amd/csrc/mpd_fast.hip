@@ -98,9 +98,26 @@ __device__ inline void load_group(const Params &p, const uint32_t *__restrict__ 
 // (the wave's 64 lanes hold the 64 groups of one aligned chunk, inactive lanes
 // only beyond the lattice) lets the wave also set the flag after writing only
 // zeros.  Every lane of the wave that holds a group must call it.
+// The chunk flags of group g's planes k >= 1 in dst (0 without flags).  A
+// sweep reads them at the start of the group's work, so that their latency
+// overlaps it rather than delaying the stores.  Within a sweep only the
+// writers of a chunk change its flag: a wave that writes a whole chunk is its
+// only writer, and otherwise a flag only goes from 1 (chunk known zero) to 0
+// when another group of the chunk is written nonzero; so a 1 read early
+// still says that this group is zero in dst.
 template <int W>
-__device__ inline void store_group_sweep(const Params &p, uint32_t *__restrict__ dst, uint32_t g, const uint4 *v,
-                                         bool whole_chunk)
+__device__ inline void load_known(const Params &p, uint32_t *__restrict__ dst, uint32_t g, uint32_t *known)
+{
+#pragma unroll
+    for (int k = 1; k < W; k++)
+        known[k] = p.plane_flags ? __hip_atomic_load(chunk_flag(p, dst, k, g), __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_AGENT)
+                                 : 0u;
+}
+
+template <int W>
+__device__ inline void store_group_known(const Params &p, uint32_t *__restrict__ dst, uint32_t g, const uint4 *v,
+                                         bool whole_chunk, const uint32_t *known_k)
 {
     reinterpret_cast<uint4 *>(dst)[g] = v[0];
 #pragma unroll
@@ -111,7 +128,7 @@ __device__ inline void store_group_sweep(const Params &p, uint32_t *__restrict__
             continue;
         }
         uint32_t *f = chunk_flag(p, dst, k, g);
-        const uint32_t known = __hip_atomic_load(f, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t known = known_k[k];
         if (nz || known == 0u) reinterpret_cast<uint4 *>(dst + (size_t)k * p.N)[g] = v[k];
         if (whole_chunk) {
             const unsigned long long act = __ballot(1);
@@ -125,6 +142,15 @@ __device__ inline void store_group_sweep(const Params &p, uint32_t *__restrict__
             __hip_atomic_store(f, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
     }
+}
+
+template <int W>
+__device__ inline void store_group_sweep(const Params &p, uint32_t *__restrict__ dst, uint32_t g, const uint4 *v,
+                                         bool whole_chunk)
+{
+    uint32_t known[W];
+    load_known<W>(p, dst, g, known);
+    store_group_known<W>(p, dst, g, v, whole_chunk, known);
 }
 
 template <int W>
@@ -376,6 +402,8 @@ __device__ inline void x4_body(const Params &p, const uint32_t *__restrict__ src
 
     Group4<W> G;
     load_group_first<W>(p, src, gc, G);
+    uint32_t known[W];  // the output group's chunk flags, read early: their latency overlaps the decisions
+    if (active) load_known<W>(p, dst, g, known);
     const uint32_t i0 = gc * 4u;
     const uint32_t x0 = (p.X & (p.X - 1u)) == 0u ? (i0 & (p.X - 1u)) : i0 % p.X;
     const bool has_m = (x0 > 0) || (p.boundary == 0);
@@ -502,7 +530,7 @@ __device__ inline void x4_body(const Params &p, const uint32_t *__restrict__ src
         for (int k = 0; k < W; k++) set_lane(res[k], s, o[k]);
     }
     // A wave holds 64 consecutive groups starting at a multiple of 64.
-    store_group_sweep<W>(p, dst, g, res, true);
+    store_group_known<W>(p, dst, g, res, true, known);
     (void)P;
 }
 
@@ -889,6 +917,8 @@ __device__ inline void tile_body(const Params &p, const uint32_t *__restrict__ s
     zero_group<W>(G);
     const uint32_t gG = (base + r * stride) >> 2;
     if (valid) load_group_first<W>(p, src, gG, G);
+    uint32_t known[W];  // the output group's chunk flags, read early: their latency overlaps the tile's work
+    if (valid) load_known<W>(p, dst, gG, known);
     const bool top = valid && ly == 0;
     const bool bottom = valid && ly == nrows - 1;
     // Site types of the neighbouring rows along the axis, read directly (the
@@ -978,7 +1008,7 @@ __device__ inline void tile_body(const Params &p, const uint32_t *__restrict__ s
             }
         }
         // With X % 256 == 0 a wave is one aligned chunk of a row.
-        store_group_sweep<W>(p, dst, i0 >> 2, res, Gx == kChunkGroups && (p.X & 255u) == 0u);
+        store_group_known<W>(p, dst, i0 >> 2, res, Gx == kChunkGroups && (p.X & 255u) == 0u, known);
     }
     if (REACT && rl.rec) {
         // The output sites that hold particles go to the reaction list with
