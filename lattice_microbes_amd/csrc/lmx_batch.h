@@ -117,6 +117,11 @@ struct Member {
     uint32_t react;      // kReactNone, kReactFull or kReactList
     uint32_t nshards;    // list shards its z sweep fills (kReactList)
     uint32_t list_by_block;  // 1: a block per list shard (large members); 0: a wave per several shards
+    // timestep_hash(seed32, timestep, s) of the step about to run for the
+    // x, y and z sweeps: set by the host, advanced by the batched resolver,
+    // so that a batched block reads it with the descriptor instead of
+    // waiting for the control block's timestep and hashing it
+    unsigned long long h[3];
 };
 
 // Work units (x4 blocks, sweep tiles) per batched block: a block loops over

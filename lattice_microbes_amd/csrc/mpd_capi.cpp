@@ -1422,6 +1422,7 @@ void fill_member(const lmx_solver *s, bool use_list, uint32_t threads, lmx::Memb
     if (s->flag_words > 0) m.p.plane_flags = 1u;  // the batched kernels are the default ones
     m.lat[0] = s->d_lat[s->cur];
     m.lat[1] = s->d_lat[1 - s->cur];
+    for (uint32_t k = 0; k < 3; k++) m.h[k] = lmx::timestep_hash(m.p.seed32, (uint32_t)s->timestep, k);
     m.list_by_block = lmx::batch_list_by_block((uint32_t)s->N) ? 1u : 0u;
     m.geo[0] = lmx::tile_geom(m.p, 1, threads);
     m.geo[1] = lmx::tile_geom(m.p, 2, threads);

@@ -2173,7 +2173,7 @@ __global__ __launch_bounds__(256) void batch_x4_kernel(const Member *__restrict_
     const uint32_t r = member_of(map, n, blockIdx.x);
     const Member &M = m[r];
     const uint32_t nb = start[r + 1] - start[r], units = ((M.p.N >> 2) + 255u) / 256u;  // x4_blocks
-    const unsigned long long h = timestep_hash(M.p.seed32, M.p.ctl->timestep, 0);  // once per block
+    const unsigned long long h = M.h[0];
     for (uint32_t u = blockIdx.x - start[r]; u < units; u += nb) x4_body<W>(M.p, M.lat[0], M.lat[1], h, u);
 }
 
@@ -2191,7 +2191,7 @@ __global__ __launch_bounds__(T, 8) void batch_tile_kernel(const Member *__restri
     uint32_t *dst = M.lat[AXIS == 1 ? 0 : 1];
     const uint32_t nb = start[r + 1] - start[r], units = tile_blocks(M.geo[AXIS - 1]);
     const ReactionList rl = AXIS == 2 && M.react == kReactList ? M.rl : ReactionList{};
-    const unsigned long long h = timestep_hash(M.p.seed32, M.p.ctl->timestep, AXIS);  // once per block
+    const unsigned long long h = M.h[AXIS];
     __shared__ TileLds<W, T> lds;
     for (uint32_t u = blockIdx.x - start[r]; u < units; u += nb) {
         tile_body<W, AXIS == 2, T>(M.p, src, dst, M.geo[AXIS - 1], AXIS, h, u, rl, lds);

@@ -316,6 +316,8 @@ __global__ __launch_bounds__(kResolveThreads) void batch_resolve_kernel(Member *
     if (threadIdx.x == 0) {
         M.lat[0] = after;
         M.lat[1] = before;
+        const uint32_t t = M.p.ctl->timestep;  // the next step's, written by this thread in resolve_body
+        for (uint32_t s = 0; s < 3; s++) M.h[s] = timestep_hash(M.p.seed32, t, s);
     }
 }
 
