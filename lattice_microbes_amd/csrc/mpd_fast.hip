@@ -386,8 +386,8 @@ __device__ inline uint32_t site_bits(unsigned long long m, int s)
 // ---------------------------------------------------------------------------
 // Body of the x sweep for 256-thread block number `block` (the kernel below,
 // or one replicate's share of a batched launch); h is the step's x-sweep hash
-// (timestep_hash substep 0), which a batched block computes once for all the
-// units it loops over.
+// (timestep_hash substep 0), which a batched block reads from its member's
+// descriptor for all the units it loops over.
 template <int W>
 __device__ inline void x4_body(const Params &p, const uint32_t *__restrict__ src, uint32_t *__restrict__ dst,
                                unsigned long long h, uint32_t block)
@@ -861,9 +861,10 @@ struct TileLds {
 };
 
 // Body of a y or z tile sweep (substep 1 or 2) for block number `block`; h
-// is timestep_hash(seed32, timestep, substep).  A batched block computes h
-// once before its unit loop: computed inside the loop, its constants stay in
-// registers across the loop and cost the tile kernels a spill.
+// is timestep_hash(seed32, timestep, substep).  A batched block reads h from
+// its member's descriptor before its unit loop: computed inside the loop, its
+// constants stayed in registers across the loop and cost the tile kernels a
+// spill, and computed before it, the block waited for the timestep's load.
 template <int W, bool REACT, int T>
 __device__ inline void tile_body(const Params &p, const uint32_t *__restrict__ src, uint32_t *__restrict__ dst,
                                  const TileGeom &geo, uint32_t substep, unsigned long long h, uint32_t block,
